@@ -1,0 +1,536 @@
+// C ABI of libflf_hip: context, planning, PSF spectrum cache, projections and
+// the fused Richardson-Lucy update.  Declarations: include/flf_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/flf_hip.h"
+#include "flf_kernels.h"
+#include "flf_plan.h"
+
+using namespace flf;
+
+namespace {
+
+struct Group {
+    GridTables gt;
+    std::vector<int> planes;          // global plane ids, ascending
+    RadixPlan rpY, rpX, rpYr, rpXr;   // fY, fX, rY, rX
+    int RB = 1, CBp = 1, CBi = 1;     // FFT batching (psf rows, psf cols, ifft cols)
+    ChunkDesc* d_chunks = nullptr;
+    ItemDesc* d_items = nullptr;
+    int* d_bu = nullptr;
+    int* d_au = nullptr;
+    int* d_s2 = nullptr;
+    unsigned char* d_enable = nullptr;
+    std::vector<unsigned char> enable_host;
+    float2* d_G[2] = {nullptr, nullptr};
+    long long* d_srcoff_fwd = nullptr;
+    long long* d_Soff_fwd = nullptr;
+    float2* d_S_fwd = nullptr;
+    float2* d_S_bwd = nullptr;
+    float2* d_A_fwd = nullptr;
+    float2* d_A_bwd = nullptr;
+    float2* d_T_fwd = nullptr;
+    long long Tbwd_off = 0;           // offset of this group's planes in the shared backward buffer
+    std::vector<char> have[2];
+};
+
+}  // namespace
+
+struct flf_ctx {
+    int device = 0;
+    std::string err;
+    bool planned = false;
+    int N = 0, Y = 0, X = 0, T = 0, nz = 0;
+    std::vector<int> s2, group_of, idx_in_group;
+    std::vector<Group> groups;
+    RowPlaneDesc* d_rowdesc = nullptr;
+    float2* d_T_bwd = nullptr;
+    long long* d_zeros = nullptr;     // zero offsets (length max planes per group)
+    int* d_plane_ids = nullptr;
+    std::vector<int> plane_ids_host;
+    double* d_fitpart = nullptr;
+    double* d_fitout = nullptr;
+    size_t max_lds = 65536;
+    long long bytes = 0;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+const int FIT_BLOCKS = 256;
+const int FFT_THREADS = 512;
+const int FFT_MAXB = 16;
+
+int fail(flf_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(ctx, FLF_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+template <class T>
+int dalloc(flf_ctx* ctx, T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(ctx, FLF_ENOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+    }
+    ctx->allocs.push_back((void*)*p);
+    ctx->bytes += (long long)(count * sizeof(T));
+    return FLF_OK;
+}
+
+template <class T>
+int upload(flf_ctx* ctx, T** p, const std::vector<T>& v) {
+    int rc = dalloc(ctx, p, v.size());
+    if (rc) return rc;
+    if (!v.empty()) HIPCHK(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return FLF_OK;
+}
+
+void free_all(flf_ctx* ctx) {
+    for (void* p : ctx->allocs) (void)hipFree(p);
+    ctx->allocs.clear();
+    ctx->groups.clear();
+    ctx->bytes = 0;
+    ctx->planned = false;
+    ctx->d_rowdesc = nullptr;
+    ctx->d_T_bwd = nullptr;
+    ctx->d_zeros = nullptr;
+    ctx->d_plane_ids = nullptr;
+    ctx->d_fitpart = nullptr;
+    ctx->d_fitout = nullptr;
+}
+
+int min_radix(const RadixPlan& rp) {
+    int m = 5;
+    for (int i = 0; i < rp.n; ++i) m = std::min(m, rp.r[i]);
+    return rp.n ? m : 5;
+}
+
+// Largest batch of length-L transforms one FFT workgroup can hold.
+int fft_batch_cap(const RadixPlan& rp, size_t max_lds) {
+    long long by_regs = (long long)min_radix(rp) * FFT_MAXB * FFT_THREADS / rp.L;
+    long long by_lds = (long long)(max_lds / (sizeof(float2) * (size_t)rp.L));
+    return (int)std::max<long long>(0, std::min(by_regs, by_lds));
+}
+
+bool fft_fits(const RadixPlan& rp, int nbatch, size_t lds_bytes, size_t max_lds) {
+    long long per_pass = (long long)nbatch * rp.L / min_radix(rp);
+    return per_pass <= (long long)FFT_MAXB * FFT_THREADS && lds_bytes <= max_lds;
+}
+
+int resolve_planes(flf_ctx* ctx, const int* planes, int nplanes, std::vector<char>& sel) {
+    sel.assign(ctx->nz, 0);
+    if (!planes) {
+        std::fill(sel.begin(), sel.end(), 1);
+        return FLF_OK;
+    }
+    if (nplanes < 0) return fail(ctx, FLF_EINVAL, "negative plane count");
+    for (int i = 0; i < nplanes; ++i) {
+        if (planes[i] < 0 || planes[i] >= ctx->nz)
+            return fail(ctx, FLF_EINVAL, "plane index " + std::to_string(planes[i]) + " out of range");
+        sel[planes[i]] = 1;
+    }
+    return FLF_OK;
+}
+
+// Upload the enable mask of a group if it changed (synchronous copy of a few bytes).
+int set_enable(flf_ctx* ctx, Group& g, const std::vector<char>& sel, int dir, bool& any, hipStream_t st) {
+    std::vector<unsigned char> en(g.planes.size());
+    any = false;
+    for (size_t p = 0; p < g.planes.size(); ++p) {
+        en[p] = sel[g.planes[p]] ? 1 : 0;
+        if (en[p]) {
+            if (!g.have[dir][p])
+                return fail(ctx, FLF_ESTATE, std::string("PSF ") + (dir ? "Ht" : "H") + " of plane " +
+                                                 std::to_string(g.planes[p]) + " not uploaded");
+            any = true;
+        }
+    }
+    if (en != g.enable_host) {
+        g.enable_host = en;
+        HIPCHK(hipMemcpyAsync(g.d_enable, g.enable_host.data(), en.size(), hipMemcpyHostToDevice, st));
+    }
+    return FLF_OK;
+}
+
+// Small FFTs of the backward source (the image) for one group.
+int run_small_fft_image(flf_ctx* ctx, Group& g, const float* img, hipStream_t st) {
+    SmallFFTArgs a{};
+    a.src = img;
+    a.src_off = ctx->d_zeros;
+    a.src_tstride = (long long)ctx->Y * ctx->X;
+    a.src_pitch = ctx->X;
+    a.Y = ctx->Y; a.X = ctx->X; a.N = ctx->N;
+    a.rpY = g.rpYr; a.rpX = g.rpXr;
+    a.S = g.d_S_bwd;
+    a.S_off = ctx->d_zeros;
+    a.S_tstride = (long long)ctx->N * ctx->N * g.gt.rY * (g.gt.rX / 2 + 1);
+    HIPCHK(launch_small_fft(a, 1, ctx->T, st));
+    return FLF_OK;
+}
+
+int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, const float* trust, int mode,
+                 const int* planes, int nplanes, hipStream_t st) {
+    if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");
+    if (!img || !vol) return fail(ctx, FLF_EINVAL, "null image or volume pointer");
+    if (mode == 1 && !Htf) return fail(ctx, FLF_EINVAL, "Prevedel update needs Htf");
+    std::vector<char> sel;
+    int rc = resolve_planes(ctx, planes, nplanes, sel);
+    if (rc) return rc;
+    const int N = ctx->N, T = ctx->T;
+    for (auto& g : ctx->groups) {
+        bool any = false;
+        rc = set_enable(ctx, g, sel, 1, any, st);
+        if (rc) return rc;
+        if (!any) continue;
+        rc = run_small_fft_image(ctx, g, img, st);
+        if (rc) return rc;
+        const GridTables& gt = g.gt;
+        MacArgs m{};
+        m.G = g.d_G[1];
+        m.chunks = g.d_chunks; m.items = g.d_items; m.u_bu = g.d_bu; m.u_au = g.d_au;
+        m.S = g.d_S_bwd; m.S_off = ctx->d_zeros;
+        m.S_tstride = (long long)N * N * gt.rY * (gt.rX / 2 + 1);
+        m.s2 = g.d_s2; m.enable = g.d_enable;
+        m.nP = (int)g.planes.size();
+        m.plane_loop = 0;
+        m.A = g.d_A_bwd;
+        m.A_tstride = (long long)gt.fY * gt.rfX;
+        m.A_pstride = (long long)T * gt.fY * gt.rfX;
+        m.N = N; m.fY = gt.fY; m.fX = gt.fX; m.rY = gt.rY; m.rX = gt.rX; m.rfX = gt.rfX; m.nu = gt.nu;
+        HIPCHK(launch_mac(m, gt.slots, (int)gt.chunks.size(), m.nP, T, gt.max_res, st));
+        IfftColArgs ic{};
+        ic.A = g.d_A_bwd;
+        ic.A_bstride = (long long)gt.fY * gt.rfX;
+        ic.rfX = gt.rfX; ic.CB = g.CBi; ic.Yout = ctx->Y;
+        ic.rpY = g.rpY;
+        ic.out = ctx->d_T_bwd + g.Tbwd_off;
+        ic.out_bstride = (long long)ctx->Y * gt.rfX;
+        HIPCHK(launch_ifft_cols(ic, m.nP * T, st));
+    }
+    std::vector<int> ids;
+    int maxfX = 0;
+    for (int z = 0; z < ctx->nz; ++z)
+        if (sel[z]) {
+            ids.push_back(z);
+            maxfX = std::max(maxfX, ctx->groups[ctx->group_of[z]].gt.fX);
+        }
+    if (ids.empty()) return FLF_OK;
+    if (ids != ctx->plane_ids_host) {
+        ctx->plane_ids_host = ids;
+        HIPCHK(hipMemcpyAsync(ctx->d_plane_ids, ctx->plane_ids_host.data(), ids.size() * sizeof(int),
+                              hipMemcpyHostToDevice, st));
+    }
+    IfftRowPlaneArgs r{};
+    r.pd = ctx->d_rowdesc;
+    r.plane_ids = ctx->d_plane_ids;
+    r.Tbuf = ctx->d_T_bwd;
+    r.vol = vol; r.Htf = Htf; r.trust = trust;
+    r.mode = mode; r.T = T; r.Y = ctx->Y; r.X = ctx->X;
+    HIPCHK(launch_ifft_rows_plane(r, (int)ids.size(), maxfX, st));
+    return FLF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int flf_version(void) { return 1; }
+
+int flf_create(int device, flf_ctx** out) {
+    if (!out) return FLF_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return FLF_EHIP;
+    if (device < 0 || device >= ndev) return FLF_EINVAL;
+    flf_ctx* c = new flf_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete c; return FLF_EHIP; }
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+        c->max_lds = (size_t)lds;
+    if (configure_kernels(c->max_lds) != hipSuccess) c->max_lds = 65536;
+    *out = c;
+    return FLF_OK;
+}
+
+void flf_destroy(flf_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    free_all(ctx);
+    delete ctx;
+}
+
+const char* flf_last_error(const flf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2) {
+    if (!ctx) return FLF_EINVAL;
+    HIPCHK(hipSetDevice(ctx->device));
+    free_all(ctx);
+    ctx->err.clear();
+    if (nnum < 3 || nnum % 2 == 0 || nnum > FLF_MAX_N) return fail(ctx, FLF_EINVAL, "Nnum must be odd, 3..31");
+    if (Y < 1 || X < 1 || T < 1 || nz < 1 || !s2) return fail(ctx, FLF_EINVAL, "bad geometry");
+    for (int z = 0; z < nz; ++z)
+        if (s2[z] < 1 || s2[z] % 2 == 0) return fail(ctx, FLF_EINVAL, "PSF side must be odd and positive");
+    ctx->N = nnum; ctx->Y = Y; ctx->X = X; ctx->T = T; ctx->nz = nz;
+    ctx->s2.assign(s2, s2 + nz);
+    ctx->group_of.assign(nz, -1);
+    ctx->idx_in_group.assign(nz, -1);
+    const int NN = nnum * nnum;
+
+    // Group planes by Fourier grid.
+    for (int z = 0; z < nz; ++z) {
+        int fY = fourier_len(Y, s2[z], nnum), fX = fourier_len(X, s2[z], nnum);
+        int gi = -1;
+        for (size_t i = 0; i < ctx->groups.size(); ++i)
+            if (ctx->groups[i].gt.fY == fY && ctx->groups[i].gt.fX == fX) gi = (int)i;
+        if (gi < 0) {
+            if ((int)ctx->groups.size() >= FLF_MAX_GROUPS) return fail(ctx, FLF_EINVAL, "too many distinct Fourier grids");
+            ctx->groups.emplace_back();
+            Group& g = ctx->groups.back();
+            std::string e = build_grid_tables(nnum, fY, fX, g.gt);
+            if (!e.empty()) return fail(ctx, FLF_EINVAL, e);
+            if (!make_radix_plan(fY, g.rpY) || !make_radix_plan(fX, g.rpX) ||
+                !make_radix_plan(g.gt.rY, g.rpYr) || !make_radix_plan(g.gt.rX, g.rpXr))
+                return fail(ctx, FLF_EINVAL, "Fourier grid is not 5-smooth");
+            gi = (int)ctx->groups.size() - 1;
+        }
+        ctx->group_of[z] = gi;
+        ctx->idx_in_group[z] = (int)ctx->groups[gi].planes.size();
+        ctx->groups[gi].planes.push_back(z);
+    }
+
+    // Device tables and buffers.
+    int maxP = 1;
+    long long Tbwd_total = 0;
+    std::vector<RowPlaneDesc> rowdesc(nz);
+    for (auto& g : ctx->groups) {
+        GridTables& gt = g.gt;
+        const int nP = (int)g.planes.size();
+        maxP = std::max(maxP, nP);
+        const int rXh = gt.rX / 2 + 1;
+        // FFT workgroup shapes.
+        if (!fft_fits(g.rpXr, gt.rY, (size_t)gt.rY * gt.rX * sizeof(float2), ctx->max_lds) ||
+            !fft_fits(g.rpYr, rXh, (size_t)gt.rY * gt.rX * sizeof(float2), ctx->max_lds))
+            return fail(ctx, FLF_EINVAL, "reduced grid " + std::to_string(gt.rY) + "x" + std::to_string(gt.rX) +
+                                             " too large for the small-FFT kernel");
+        g.RB = std::min(16, fft_batch_cap(g.rpX, ctx->max_lds));
+        g.CBp = std::min(16, fft_batch_cap(g.rpY, ctx->max_lds));
+        g.CBi = g.CBp;
+        if (g.RB < 1 || g.CBp < 1) return fail(ctx, FLF_EINVAL, "Fourier grid too large for the LDS FFT");
+        int rc;
+        if ((rc = upload(ctx, &g.d_chunks, gt.chunks))) return rc;
+        if ((rc = upload(ctx, &g.d_items, gt.items))) return rc;
+        if ((rc = upload(ctx, &g.d_bu, gt.u_bu))) return rc;
+        if ((rc = upload(ctx, &g.d_au, gt.u_au))) return rc;
+        std::vector<int> s2g;
+        std::vector<long long> srcoff, soff;
+        const long long Sper = (long long)T * NN * gt.rY * rXh;
+        for (int p = 0; p < nP; ++p) {
+            int z = g.planes[p];
+            s2g.push_back(s2[z]);
+            srcoff.push_back((long long)z * T * Y * X);
+            soff.push_back((long long)p * Sper);
+        }
+        if ((rc = upload(ctx, &g.d_s2, s2g))) return rc;
+        if ((rc = upload(ctx, &g.d_srcoff_fwd, srcoff))) return rc;
+        if ((rc = upload(ctx, &g.d_Soff_fwd, soff))) return rc;
+        g.enable_host.assign(nP, 1);
+        if ((rc = upload(ctx, &g.d_enable, g.enable_host))) return rc;
+        const long long items = (long long)gt.items.size();
+        for (int d = 0; d < 2; ++d) {
+            if ((rc = dalloc(ctx, &g.d_G[d], (size_t)items * gt.slots * gt.nu * nP))) return rc;
+            g.have[d].assign(nP, 0);
+        }
+        if ((rc = dalloc(ctx, &g.d_S_fwd, (size_t)(nP * Sper)))) return rc;
+        if ((rc = dalloc(ctx, &g.d_S_bwd, (size_t)Sper))) return rc;
+        if ((rc = dalloc(ctx, &g.d_A_fwd, (size_t)T * gt.fY * gt.rfX))) return rc;
+        if ((rc = dalloc(ctx, &g.d_A_bwd, (size_t)nP * T * gt.fY * gt.rfX))) return rc;
+        if ((rc = dalloc(ctx, &g.d_T_fwd, (size_t)T * Y * gt.rfX))) return rc;
+        g.Tbwd_off = Tbwd_total;
+        for (int p = 0; p < nP; ++p) {
+            RowPlaneDesc& rd = rowdesc[g.planes[p]];
+            rd.T_off = Tbwd_total + (long long)p * T * Y * gt.rfX;
+            rd.T_tstride = (long long)Y * gt.rfX;
+            rd.rfX = gt.rfX;
+            rd.scale = 1.0f / ((float)gt.fY * (float)gt.fX);
+            rd.rp = g.rpX;
+        }
+        Tbwd_total += (long long)nP * T * Y * gt.rfX;
+        // ifft rows: one row of fX plus the accumulation row.
+        if ((size_t)gt.fX * sizeof(float2) + (size_t)X * sizeof(float) > ctx->max_lds ||
+            !fft_fits(g.rpX, 1, 0, ctx->max_lds))
+            return fail(ctx, FLF_EINVAL, "image row too long for the LDS FFT");
+    }
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->d_T_bwd, (size_t)Tbwd_total))) return rc;
+    if ((rc = upload(ctx, &ctx->d_rowdesc, rowdesc))) return rc;
+    if ((rc = upload(ctx, &ctx->d_zeros, std::vector<long long>(maxP, 0)))) return rc;
+    if ((rc = dalloc(ctx, &ctx->d_plane_ids, (size_t)nz))) return rc;
+    ctx->plane_ids_host.clear();
+    if ((rc = dalloc(ctx, &ctx->d_fitpart, (size_t)2 * FIT_BLOCKS * T))) return rc;
+    if ((rc = dalloc(ctx, &ctx->d_fitout, (size_t)2 * T))) return rc;
+    ctx->planned = true;
+    return FLF_OK;
+}
+
+int flf_plane_fshape(const flf_ctx* ctx, int z, int* fY, int* fX) {
+    if (!ctx || !ctx->planned || z < 0 || z >= ctx->nz || !fY || !fX) return FLF_EINVAL;
+    const Group& g = ctx->groups[ctx->group_of[z]];
+    *fY = g.gt.fY;
+    *fX = g.gt.fX;
+    return FLF_OK;
+}
+
+long long flf_device_bytes(const flf_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+int flf_set_psf(flf_ctx* ctx, int z, int dir, const float* quadrant_host) {
+    if (!ctx) return FLF_EINVAL;
+    if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");
+    if (z < 0 || z >= ctx->nz || (dir != 0 && dir != 1) || !quadrant_host)
+        return fail(ctx, FLF_EINVAL, "bad plane, direction or pointer");
+    HIPCHK(hipSetDevice(ctx->device));
+    Group& g = ctx->groups[ctx->group_of[z]];
+    const GridTables& gt = g.gt;
+    const int h = (ctx->N + 1) / 2, s2 = ctx->s2[z];
+    const size_t qn = (size_t)h * h * s2 * s2;
+    float* dQ = nullptr;
+    float2 *dtmp = nullptr, *dspec = nullptr;
+    auto cleanup = [&]() {
+        if (dQ) (void)hipFree(dQ);
+        if (dtmp) (void)hipFree(dtmp);
+        if (dspec) (void)hipFree(dspec);
+    };
+    if (hipMalloc((void**)&dQ, qn * sizeof(float)) != hipSuccess ||
+        hipMalloc((void**)&dtmp, (size_t)gt.nu * s2 * gt.rfX * sizeof(float2)) != hipSuccess ||
+        hipMalloc((void**)&dspec, (size_t)gt.nu * gt.fY * gt.rfX * sizeof(float2)) != hipSuccess) {
+        (void)hipGetLastError();
+        cleanup();
+        return fail(ctx, FLF_ENOMEM, "out of device memory computing PSF spectra");
+    }
+    hipError_t e = hipMemcpy(dQ, quadrant_host, qn * sizeof(float), hipMemcpyHostToDevice);
+    PsfRowArgs ra{};
+    ra.Q = dQ; ra.h = h; ra.s2 = s2; ra.RB = g.RB; ra.u_bu = g.d_bu; ra.u_au = g.d_au; ra.rpX = g.rpX; ra.tmp = dtmp;
+    if (e == hipSuccess) e = launch_psf_rows(ra, gt.nu, 0);
+    PsfColArgs ca{};
+    ca.tmp = dtmp; ca.s2 = s2; ca.rfX = gt.rfX; ca.CB = g.CBp; ca.rpY = g.rpY; ca.spec = dspec;
+    if (e == hipSuccess) e = launch_psf_cols(ca, gt.nu, 0);
+    SpecPackArgs pa{};
+    pa.spec = dspec; pa.chunks = g.d_chunks; pa.items = g.d_items; pa.u_bu = g.d_bu; pa.u_au = g.d_au;
+    pa.fY = gt.fY; pa.rfX = gt.rfX; pa.N = ctx->N; pa.slots = gt.slots; pa.nu = gt.nu;
+    pa.nP = (int)g.planes.size(); pa.p = ctx->idx_in_group[z]; pa.G = g.d_G[dir];
+    if (e == hipSuccess) e = launch_spec_pack(pa, (int)gt.chunks.size(), 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    cleanup();
+    if (e != hipSuccess) return fail(ctx, FLF_EHIP, std::string("PSF spectra: ") + hipGetErrorString(e));
+    g.have[dir][ctx->idx_in_group[z]] = 1;
+    return FLF_OK;
+}
+
+int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, int nplanes, void* stream) {
+    if (!ctx) return FLF_EINVAL;
+    if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");
+    if (!vol || !img) return fail(ctx, FLF_EINVAL, "null volume or image pointer");
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<char> sel;
+    int rc = resolve_planes(ctx, planes, nplanes, sel);
+    if (rc) return rc;
+    const int N = ctx->N, T = ctx->T, NN = N * N;
+    IfftRowSumArgs rs{};
+    rs.ngroups = (int)ctx->groups.size();
+    rs.X = ctx->X;
+    rs.img = img;
+    rs.img_tstride = (long long)ctx->Y * ctx->X;
+    rs.maxfX = 1;
+    for (size_t gi = 0; gi < ctx->groups.size(); ++gi) {
+        Group& g = ctx->groups[gi];
+        const GridTables& gt = g.gt;
+        bool any = false;
+        rc = set_enable(ctx, g, sel, 0, any, st);
+        if (rc) return rc;
+        rs.grp[gi].active = any ? 1 : 0;
+        rs.grp[gi].T = g.d_T_fwd;
+        rs.grp[gi].tstride = (long long)ctx->Y * gt.rfX;
+        rs.grp[gi].rfX = gt.rfX;
+        rs.grp[gi].scale = 1.0f / ((float)gt.fY * (float)gt.fX);
+        rs.grp[gi].rp = g.rpX;
+        if (!any) continue;
+        rs.maxfX = std::max(rs.maxfX, gt.fX);
+        const int nP = (int)g.planes.size();
+        const int rXh = gt.rX / 2 + 1;
+        SmallFFTArgs a{};
+        a.src = vol;
+        a.src_off = g.d_srcoff_fwd;
+        a.src_tstride = (long long)ctx->Y * ctx->X;
+        a.src_pitch = ctx->X;
+        a.Y = ctx->Y; a.X = ctx->X; a.N = N;
+        a.rpY = g.rpYr; a.rpX = g.rpXr;
+        a.S = g.d_S_fwd;
+        a.S_off = g.d_Soff_fwd;
+        a.S_tstride = (long long)NN * gt.rY * rXh;
+        HIPCHK(launch_small_fft(a, nP, T, st));
+        MacArgs m{};
+        m.G = g.d_G[0];
+        m.chunks = g.d_chunks; m.items = g.d_items; m.u_bu = g.d_bu; m.u_au = g.d_au;
+        m.S = g.d_S_fwd; m.S_off = g.d_Soff_fwd; m.S_tstride = a.S_tstride;
+        m.s2 = g.d_s2; m.enable = g.d_enable;
+        m.nP = nP;
+        m.plane_loop = 1;
+        m.A = g.d_A_fwd;
+        m.A_tstride = (long long)gt.fY * gt.rfX;
+        m.A_pstride = 0;
+        m.N = N; m.fY = gt.fY; m.fX = gt.fX; m.rY = gt.rY; m.rX = gt.rX; m.rfX = gt.rfX; m.nu = gt.nu;
+        HIPCHK(launch_mac(m, gt.slots, (int)gt.chunks.size(), 1, T, gt.max_res, st));
+        IfftColArgs ic{};
+        ic.A = g.d_A_fwd;
+        ic.A_bstride = (long long)gt.fY * gt.rfX;
+        ic.rfX = gt.rfX; ic.CB = g.CBi; ic.Yout = ctx->Y;
+        ic.rpY = g.rpY;
+        ic.out = g.d_T_fwd;
+        ic.out_bstride = (long long)ctx->Y * gt.rfX;
+        HIPCHK(launch_ifft_cols(ic, T, st));
+    }
+    HIPCHK(launch_ifft_rows_sum(rs, ctx->Y, T, st));
+    return FLF_OK;
+}
+
+int flf_backward(flf_ctx* ctx, const float* img, float* vol, const int* planes, int nplanes, void* stream) {
+    if (!ctx) return FLF_EINVAL;
+    return run_backward(ctx, img, vol, nullptr, nullptr, 0, planes, nplanes, (hipStream_t)stream);
+}
+
+int flf_backward_rl_update(flf_ctx* ctx, const float* img, float* X, const float* Htf, const float* trust,
+                           int mode, const int* planes, int nplanes, void* stream) {
+    if (!ctx) return FLF_EINVAL;
+    if (mode != 1 && mode != 2) return fail(ctx, FLF_EINVAL, "mode must be 1 (Prevedel) or 2 (standard)");
+    return run_backward(ctx, img, X, Htf, trust, mode, planes, nplanes, (hipStream_t)stream);
+}
+
+int flf_datafit(flf_ctx* ctx, const float* img, const float* im, double* out_host, void* stream) {
+    if (!ctx) return FLF_EINVAL;
+    if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");
+    if (!img || !im || !out_host) return fail(ctx, FLF_EINVAL, "null pointer");
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(launch_fit(img, im, (long long)ctx->Y * ctx->X, ctx->T, ctx->d_fitpart, FIT_BLOCKS, ctx->d_fitout, st));
+    HIPCHK(hipMemcpyAsync(out_host, ctx->d_fitout, sizeof(double) * 2 * ctx->T, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return FLF_OK;
+}
+
+}  // extern "C"

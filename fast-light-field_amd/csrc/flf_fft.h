@@ -1,0 +1,131 @@
+// Batched mixed-radix (2/3/4/5) complex FFTs staged in LDS.
+//
+// Stockham auto-sort formulation: pass p with radix R and Ns = product of
+// the radices of the earlier passes maps, for j in [0, L/R), k = j mod Ns,
+//     a_t = x[j + t*L/R] * exp(sign*2*pi*i*t*k/(Ns*R))      t in [0, R)
+//     y   = DFT_R(a)
+//     x'[(j/Ns)*Ns*R + k + q*Ns] = y_q                      q in [0, R)
+// and the output is in natural order after the last pass.  Each pass reads
+// all of its inputs into registers, synchronises, then writes in place, so one
+// LDS buffer suffices.  Every thread of the block must call these routines.
+#pragma once
+#include "flf_common.h"
+
+namespace flf {
+
+template <int R>
+__device__ __forceinline__ void dft_small(float2 (&a)[5], float sign) {
+    if constexpr (R == 2) {
+        float2 t = a[0];
+        a[0] = cadd(t, a[1]);
+        a[1] = csub(t, a[1]);
+    } else if constexpr (R == 4) {
+        float2 s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
+        float2 s13 = cadd(a[1], a[3]), d13 = csub(a[1], a[3]);
+        // j*d13 with j = sign*i
+        float2 jd13 = make_float2(-sign * d13.y, sign * d13.x);
+        a[0] = cadd(s02, s13);
+        a[1] = cadd(d02, jd13);
+        a[2] = csub(s02, s13);
+        a[3] = csub(d02, jd13);
+    } else if constexpr (R == 3) {
+        const float c = -0.5f;
+        const float s = sign * 0.86602540378443864676f;
+        float2 t1 = cadd(a[1], a[2]);
+        float2 t2 = csub(a[1], a[2]);
+        float2 m = make_float2(a[0].x + c * t1.x, a[0].y + c * t1.y);
+        float2 is = make_float2(-s * t2.y, s * t2.x);
+        a[0] = cadd(a[0], t1);
+        a[1] = cadd(m, is);
+        a[2] = csub(m, is);
+    } else if constexpr (R == 5) {
+        const float c1 = 0.30901699437494742410f;   // cos(2pi/5)
+        const float c2 = -0.80901699437494742410f;  // cos(4pi/5)
+        const float s1 = sign * 0.95105651629515357212f;
+        const float s2 = sign * 0.58778525229247312917f;
+        float2 t1 = cadd(a[1], a[4]), d1 = csub(a[1], a[4]);
+        float2 t2 = cadd(a[2], a[3]), d2 = csub(a[2], a[3]);
+        float2 m1 = make_float2(a[0].x + c1 * t1.x + c2 * t2.x, a[0].y + c1 * t1.y + c2 * t2.y);
+        float2 m2 = make_float2(a[0].x + c2 * t1.x + c1 * t2.x, a[0].y + c2 * t1.y + c1 * t2.y);
+        // i*(s1 d1 + s2 d2) and i*(s2 d1 - s1 d2)
+        float2 u1 = make_float2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y);
+        float2 u2 = make_float2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y);
+        float2 iu1 = make_float2(-u1.y, u1.x);
+        float2 iu2 = make_float2(-u2.y, u2.x);
+        a[0] = cadd(a[0], cadd(t1, t2));
+        a[1] = cadd(m1, iu1);
+        a[4] = csub(m1, iu1);
+        a[2] = cadd(m2, iu2);
+        a[3] = csub(m2, iu2);
+    }
+}
+
+// One Stockham pass of radix R over `nbatch` sequences of length L stored at
+// buf[b*bs + i*es].  MAXB bounds the butterflies a thread holds in registers:
+// nbatch*L/R <= MAXB*blockDim.x is required (checked by the host planner).
+template <int R, int MAXB>
+__device__ __forceinline__ void stockham_pass(float2* buf, int L, int nbatch, int es, int bs,
+                                              int Ns, float sign) {
+    const int M = L / R;
+    const int total = nbatch * M;
+    const int nth = blockDim.x;
+    const int tid = threadIdx.x;
+    float2 v[MAXB][5];
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        int idx = tid + q * nth;
+        if (idx < total) {
+            int b = idx / M;
+            int j = idx - b * M;
+            int k = j % Ns;
+            float2* base = buf + (long)b * bs;
+            float2 a[5];
+#pragma unroll
+            for (int t = 0; t < R; ++t) a[t] = base[(j + t * M) * es];
+            if (k != 0) {
+#pragma unroll
+                for (int t = 1; t < R; ++t) {
+                    float s, c;
+                    sincospif(sign * 2.0f * (float)(t * k) / (float)(Ns * R), &s, &c);
+                    a[t] = cmul(a[t], make_float2(c, s));
+                }
+            }
+            dft_small<R>(a, sign);
+#pragma unroll
+            for (int t = 0; t < R; ++t) v[q][t] = a[t];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        int idx = tid + q * nth;
+        if (idx < total) {
+            int b = idx / M;
+            int j = idx - b * M;
+            int k = j % Ns;
+            float2* base = buf + (long)b * bs;
+            int o = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int t = 0; t < R; ++t) base[(o + t * Ns) * es] = v[q][t];
+        }
+    }
+    __syncthreads();
+}
+
+// Full transform: sign = -1 forward, +1 inverse (unnormalised).
+template <int MAXB>
+__device__ void lds_fft(float2* buf, const RadixPlan& rp, int nbatch, int es, int bs, float sign) {
+    int Ns = 1;
+    for (int p = 0; p < rp.n; ++p) {
+        int R = rp.r[p];
+        switch (R) {
+            case 4: stockham_pass<4, MAXB>(buf, rp.L, nbatch, es, bs, Ns, sign); break;
+            case 2: stockham_pass<2, MAXB>(buf, rp.L, nbatch, es, bs, Ns, sign); break;
+            case 3: stockham_pass<3, MAXB>(buf, rp.L, nbatch, es, bs, Ns, sign); break;
+            default: stockham_pass<5, MAXB>(buf, rp.L, nbatch, es, bs, Ns, sign); break;
+        }
+        Ns *= R;
+    }
+}
+
+}  // namespace flf

@@ -1,0 +1,123 @@
+// Kernel argument blocks and host launchers (internal to libflf_hip).
+#pragma once
+#include "flf_common.h"
+
+#define FLF_MAX_GROUPS 8
+
+struct SmallFFTArgs {
+    const float* src;            // source base
+    const long long* src_off;    // per source plane z: element offset (device array)
+    long long src_tstride;       // per frame
+    int src_pitch;               // row pitch (elements)
+    int Y, X, N;
+    RadixPlan rpY, rpX;          // lengths rY, rX
+    float2* S;
+    const long long* S_off;      // per source plane z: element offset into S (device array)
+    long long S_tstride;         // per frame
+};
+
+struct PsfRowArgs {
+    const float* Q;              // quadrant (h, h, s2, s2)
+    int h, s2, RB;
+    const int* u_bu;
+    const int* u_au;
+    RadixPlan rpX;               // length fX
+    float2* tmp;                 // (nu, s2, rfX)
+};
+
+struct PsfColArgs {
+    const float2* tmp;           // (nu, s2, rfX)
+    int s2, rfX, CB;
+    RadixPlan rpY;               // length fY
+    float2* spec;                // (nu, fY, rfX)
+};
+
+struct SpecPackArgs {
+    const float2* spec;          // (nu, fY, rfX) natural layout, one plane
+    const ChunkDesc* chunks;
+    const ItemDesc* items;
+    const int* u_bu;
+    const int* u_au;
+    int fY, rfX, N, slots, nu, nP, p;
+    float2* G;                   // orbit-major spectra of the grid group
+};
+
+struct MacArgs {
+    const float2* G;
+    const ChunkDesc* chunks;
+    const ItemDesc* items;
+    const int* u_bu;
+    const int* u_au;
+    const float2* S;
+    const long long* S_off;      // per plane in group
+    long long S_tstride;
+    const int* s2;               // per plane in group
+    const unsigned char* enable; // per plane in group
+    int nP;
+    int plane_loop;              // 1: sum all planes of the group (forward); 0: plane = blockIdx.y
+    float2* A;
+    long long A_tstride, A_pstride;
+    int N, fY, fX, rY, rX, rfX, nu;
+};
+
+struct IfftColArgs {
+    const float2* A;
+    long long A_bstride;
+    int rfX, CB, Yout;
+    RadixPlan rpY;
+    float2* out;
+    long long out_bstride;
+};
+
+struct IfftRowGroup {
+    const float2* T;             // (T, Y, rfX)
+    long long tstride;
+    int rfX;
+    int active;
+    float scale;
+    RadixPlan rp;                // length fX
+};
+
+struct IfftRowSumArgs {
+    IfftRowGroup grp[FLF_MAX_GROUPS];
+    int ngroups;
+    int maxfX;
+    int X;
+    float* img;
+    long long img_tstride;
+};
+
+struct RowPlaneDesc {
+    long long T_off;             // element offset of this plane's (T, Y, rfX) block
+    long long T_tstride;
+    int rfX;
+    float scale;
+    RadixPlan rp;
+};
+
+struct IfftRowPlaneArgs {
+    const RowPlaneDesc* pd;      // per plane z
+    const int* plane_ids;        // planes to process (grid z)
+    const float2* Tbuf;
+    float* vol;                  // (nz, T, Y, X)
+    const float* Htf;            // mode 1
+    const float* trust;          // mode 2 (optional, (nz, 1, Y, X))
+    int mode;                    // 0 store, 1 Prevedel update, 2 standard update
+    int T, Y, X;
+};
+
+namespace flf {
+hipError_t launch_small_fft(const SmallFFTArgs& a, int nsrc, int T, hipStream_t s);
+hipError_t launch_psf_rows(const PsfRowArgs& a, int nu, hipStream_t s);
+hipError_t launch_psf_cols(const PsfColArgs& a, int nu, hipStream_t s);
+hipError_t launch_spec_pack(const SpecPackArgs& a, int nchunks, hipStream_t s);
+hipError_t launch_mac(const MacArgs& a, int slots, int nchunks, int nplanes_grid, int T, int max_res, hipStream_t s);
+hipError_t launch_ifft_cols(const IfftColArgs& a, int nbatch, hipStream_t s);
+hipError_t launch_ifft_rows_sum(const IfftRowSumArgs& a, int Y, int T, hipStream_t s);
+hipError_t launch_ifft_rows_plane(const IfftRowPlaneArgs& a, int nplanes, int maxfX, hipStream_t s);
+hipError_t launch_fit(const float* img, const float* im, long long n_per_frame, int T, double* part, int nblk,
+                      double* out, hipStream_t s);
+hipError_t launch_fill(float* p, long long n, float v, hipStream_t s);
+// Raise the dynamic-LDS limit of the FFT kernels to max_lds bytes.
+hipError_t configure_kernels(size_t max_lds);
+}  // namespace flf

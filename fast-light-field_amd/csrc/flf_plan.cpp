@@ -1,0 +1,208 @@
+// Host-side planning for the light-field projector: Fourier shapes, radix
+// plans and the orbit/item tables of the multiply-accumulate kernel.
+//
+// Items.  Every half-plane output bin k (k1 <= fX/2) of one plane needs the
+// stored spectra at k, (-k0, k1) [row flip], (k1, k0) and (-k1, k0)
+// [transposes; square grids only].  Grouping those bins into one item lets
+// the kernel read each stored value once and use it for all of them:
+//   square     quad  P0=(p0,p1) P1=(-p0,p1) P2=(p1,p0) P3=(-p1,p0),
+//              base p in D = {0 <= p1 <= p0 <= f/2}
+//   non-square pair  P0=(p0,p1) P1=(-p0,p1), p0 <= fY/2, p1 <= fX/2
+// Every half-plane bin is the slot of exactly one item (slots of the same item
+// may coincide on the boundaries of D; only the first writes its bin).
+//
+// Orbits.  The small FFTs S_{b,a} are periodic with the reduced grid r = f/N,
+// so the kernel needs S only at the residues (k mod r) of its items' bins.
+// Items are grouped by the orbit of their base residue under the symmetry
+// group of the item (the dihedral group of the reduced square grid, or the
+// sign flips of each axis for non-square grids): one orbit has at most 8
+// residues and about N*N items, so a workgroup holds the N*N small-FFT values
+// of <= 8 residues in LDS and streams the spectra of its items exactly once.
+//
+// Aliases.  A bin k = kt + m*r is written with kt the centred residue
+// (|kt| < r/2) and m its alias; for the residue r/2 (r even) kt = 0 and m is
+// a half integer.  Under this convention negating k negates m exactly, which
+// is what lets all flip/transposition phases reduce to per-offset constants
+// (folded into the s-table) and per-bin constants (the four classes).
+#include "flf_plan.h"
+
+#include <algorithm>
+#include <map>
+
+namespace flf {
+
+static int max_prime_factor(int n) {
+    int m = 1;
+    for (int p = 2; p * p <= n; ++p)
+        while (n % p == 0) { m = std::max(m, p); n /= p; }
+    if (n > 1) m = std::max(m, n);
+    return m;
+}
+
+int smooth5_at_least(int n) {
+    int m = std::max(n, 1);
+    while (m > 1 && max_prime_factor(m) > 5) ++m;
+    return m;
+}
+
+int fourier_len(int s1, int s2, int N) {
+    int need = s1 + s2 - 1;
+    int red = (need + N - 1) / N;
+    return N * smooth5_at_least(red);
+}
+
+bool make_radix_plan(int L, RadixPlan& rp) {
+    rp.L = L;
+    rp.n = 0;
+    int m = L;
+    const int order[4] = {4, 2, 3, 5};
+    for (int k = 0; k < 4; ++k) {
+        int R = order[k];
+        while (m % R == 0) {   // after the 4s at most one factor 2 remains
+            if (rp.n >= FLF_MAX_RADIX_PASSES) return false;
+            rp.r[rp.n++] = R;
+            m /= R;
+        }
+    }
+    return m == 1;
+}
+
+// Half-unit alias of coordinate x on an axis of length N*r.
+static int alias2(int x, int r, int N) {
+    int k = x % r, m = x / r;
+    if (2 * k < r) return (2 * m) % (2 * N);
+    if (2 * k > r) return (2 * (m + 1)) % (2 * N);
+    return (2 * m + 1) % (2 * N);
+}
+
+std::string build_grid_tables(int N, int fY, int fX, GridTables& gt) {
+    if (N % 2 == 0 || N < 3 || N > FLF_MAX_N) return "Nnum must be odd and in [3, 31]";
+    if (fY % N || fX % N) return "Fourier grid must be a multiple of Nnum";
+    gt = GridTables();
+    gt.fY = fY;
+    gt.fX = fX;
+    gt.rY = fY / N;
+    gt.rX = fX / N;
+    gt.rfX = fX / 2 + 1;
+    gt.square = (fY == fX);
+    gt.slots = gt.square ? 4 : 2;
+    const int h = (N + 1) / 2, c = N / 2;
+
+    // Stored spectra order (must match the category loops of k_mac).
+    gt.u_bu.push_back(c); gt.u_au.push_back(c);
+    if (gt.square) {
+        for (int b = 0; b < c; ++b) { gt.u_bu.push_back(b); gt.u_au.push_back(b); }   // (b, b)
+        for (int b = 0; b < c; ++b) { gt.u_bu.push_back(b); gt.u_au.push_back(c); }   // (b, c)
+        for (int b = 0; b < c; ++b)
+            for (int a = b + 1; a < c; ++a) { gt.u_bu.push_back(b); gt.u_au.push_back(a); }
+    } else {
+        for (int b = 0; b < c; ++b) { gt.u_bu.push_back(b); gt.u_au.push_back(c); }   // (b, c)
+        for (int a = 0; a < c; ++a) { gt.u_bu.push_back(c); gt.u_au.push_back(a); }   // (c, a) = transpose of (a, c)
+        for (int b = 0; b < c; ++b)
+            for (int a = 0; a < c; ++a) { gt.u_bu.push_back(b); gt.u_au.push_back(a); }
+    }
+    gt.nu = (int)gt.u_bu.size();
+    if (gt.nu != (gt.square ? h * (h + 1) / 2 : h * h)) return "internal: spectrum count";
+
+    const int rY = gt.rY, rX = gt.rX, rfX = gt.rfX;
+    auto canon = [&](int k0, int k1) -> long long {
+        long long best = -1;
+        int a0[2] = {k0 % rY, (rY - k0 % rY) % rY};
+        int a1[2] = {k1 % rX, (rX - k1 % rX) % rX};
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) {
+                long long v = (long long)a0[i] * rX + a1[j];
+                if (best < 0 || v < best) best = v;
+                if (gt.square) {
+                    long long w = (long long)a1[j] * rX + a0[i];
+                    if (w < best) best = w;
+                }
+            }
+        return best;
+    };
+
+    // Collect item bases per orbit (ordered by orbit key for determinism).
+    std::vector<std::vector<std::pair<int, int>>> orbit_items((size_t)rY * rX);
+    if (gt.square) {
+        const int half = fY / 2;
+        for (int p0 = 0; p0 <= half; ++p0)
+            for (int p1 = 0; p1 <= p0; ++p1)
+                orbit_items[canon(p0 % rY, p1 % rX)].push_back({p0, p1});
+    } else {
+        for (int p0 = 0; p0 <= fY / 2; ++p0)
+            for (int p1 = 0; p1 <= fX / 2; ++p1)
+                orbit_items[canon(p0 % rY, p1 % rX)].push_back({p0, p1});
+    }
+
+    gt.max_res = 0;
+    for (size_t key = 0; key < orbit_items.size(); ++key) {
+        auto& bases = orbit_items[key];
+        if (bases.empty()) continue;
+        // Residue list of the orbit (first appearance order).
+        std::vector<std::pair<int, int>> res;
+        std::vector<ItemDesc> its;
+        its.reserve(bases.size());
+        for (auto& bp : bases) {
+            int p0 = bp.first, p1 = bp.second;
+            int K0[4], K1[4];
+            K0[0] = p0; K1[0] = p1;
+            K0[1] = (fY - p0) % fY; K1[1] = p1;
+            if (gt.square) {
+                K0[2] = p1; K1[2] = p0;
+                K0[3] = (fY - p1) % fY; K1[3] = p0;
+            }
+            ItemDesc it{};
+            it.writemask = 0;
+            for (int j = 0; j < 4; ++j) {
+                if (j >= gt.slots) {
+                    it.pos[j] = it.pos[0]; it.res[j] = it.res[0]; it.m2[j] = it.m2[0]; it.n2[j] = it.n2[0];
+                    continue;
+                }
+                if (K1[j] < 0 || K1[j] >= rfX) return "internal: slot outside half plane";
+                it.pos[j] = K0[j] * rfX + K1[j];
+                bool dup = false;
+                for (int i = 0; i < j; ++i) dup |= (it.pos[i] == it.pos[j]);
+                if (!dup) it.writemask |= (1 << j);
+                std::pair<int, int> rr(K0[j] % rY, K1[j] % rX);
+                int ri = -1;
+                for (size_t q = 0; q < res.size(); ++q)
+                    if (res[q] == rr) ri = (int)q;
+                if (ri < 0) {
+                    res.push_back(rr);
+                    ri = (int)res.size() - 1;
+                    if (ri >= FLF_MAX_RES) return "internal: orbit has more than 8 residues";
+                }
+                it.res[j] = (unsigned char)ri;
+                it.m2[j] = (unsigned char)alias2(K0[j], rY, N);
+                it.n2[j] = (unsigned char)alias2(K1[j], rX, N);
+            }
+            its.push_back(it);
+        }
+        gt.max_res = std::max(gt.max_res, (int)res.size());
+        for (size_t s = 0; s < its.size(); s += FLF_CHUNK_MAX) {
+            ChunkDesc ch{};
+            ch.item_base = (long long)gt.items.size();
+            ch.n_items = (int)std::min<size_t>(FLF_CHUNK_MAX, its.size() - s);
+            ch.n_res = (int)res.size();
+            for (int q = 0; q < FLF_MAX_RES; ++q) {
+                ch.res0[q] = q < ch.n_res ? res[q].first : 0;
+                ch.res1[q] = q < ch.n_res ? res[q].second : 0;
+            }
+            gt.items.insert(gt.items.end(), its.begin() + s, its.begin() + s + ch.n_items);
+            gt.chunks.push_back(ch);
+        }
+    }
+    // Coverage check: every half-plane bin written exactly once.
+    std::vector<unsigned char> hit((size_t)fY * rfX, 0);
+    for (auto& it : gt.items)
+        for (int j = 0; j < gt.slots; ++j)
+            if ((it.writemask >> j) & 1) {
+                if (hit[it.pos[j]]) return "internal: bin written twice";
+                hit[it.pos[j]] = 1;
+            }
+    for (auto v : hit)
+        if (!v) return "internal: bin not covered";
+    return "";
+}
+
+}  // namespace flf

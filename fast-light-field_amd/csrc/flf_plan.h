@@ -1,0 +1,36 @@
+// Host-side planning: Fourier grid shapes, FFT radix plans and the orbit/item
+// tables that drive the multiply-accumulate kernel.
+#pragma once
+#include <string>
+#include <vector>
+#include "flf_types.h"
+
+namespace flf {
+
+// Smallest m >= n with no prime factor above 5.
+int smooth5_at_least(int n);
+
+// Fourier grid length for one axis: N * smooth5(ceil((s1 + s2 - 1) / N)).
+// Same linear convolution as convolutionShape (special_fftconvolve.py:142-174);
+// only the zero padding differs (5-smooth instead of 7-smooth or minimal).
+int fourier_len(int s1, int s2, int N);
+
+// Radix plan (4s first, then 2, 3, 5).  Returns false if L is not 5-smooth.
+bool make_radix_plan(int L, RadixPlan& rp);
+
+// Index tables for one Fourier grid (fY, fX).
+struct GridTables {
+    int fY = 0, fX = 0, rY = 0, rX = 0, rfX = 0;
+    bool square = false;
+    int slots = 0;               // 4 (square) or 2
+    int nu = 0;                  // stored spectra per plane: h(h+1)/2 or h*h
+    int max_res = 0;             // largest residue list of any chunk
+    std::vector<int> u_bu, u_au; // stored spectrum u -> quadrant (b, a) (b > a: transposed)
+    std::vector<ChunkDesc> chunks;
+    std::vector<ItemDesc> items;
+};
+
+// Builds the tables; returns an empty string or an error message.
+std::string build_grid_tables(int N, int fY, int fX, GridTables& gt);
+
+}  // namespace flf

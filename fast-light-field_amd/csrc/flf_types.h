@@ -1,0 +1,33 @@
+// Plain table types shared by the host planner and the kernels (no HIP
+// dependency, so the planner also builds with a host-only compiler).
+#pragma once
+
+#define FLF_MAX_RADIX_PASSES 16
+#define FLF_MAX_RES 8          // residues per orbit (D4 orbit of a reduced bin)
+#define FLF_CHUNK_MAX 256      // items per MAC workgroup
+#define FLF_MAX_N 31           // largest supported Nnum (s-table = N*N entries)
+
+struct RadixPlan {
+    int L;                      // transform length
+    int n;                      // number of passes
+    int r[FLF_MAX_RADIX_PASSES];
+};
+
+// One MAC workgroup: a chunk of the items whose bins fall on one residue orbit.
+struct ChunkDesc {
+    long long item_base;        // first item (global index within the grid group)
+    int n_items;
+    int n_res;
+    int res0[FLF_MAX_RES];      // residue row    (kappa0 in [0, rY))
+    int res1[FLF_MAX_RES];      // residue column (kappa1 in [0, rX))
+};
+
+// One item: 4 (square grid) or 2 (non-square) half-plane bins related by
+// row negation / transposition; the MAC computes all of them together.
+struct ItemDesc {
+    int pos[4];                 // k0 * rfX + k1 of each slot
+    unsigned char res[4];       // residue index of each slot's bin in the chunk's residue list
+    unsigned char m2[4];        // alias row index, in half-units mod 2N
+    unsigned char n2[4];        // alias column index, in half-units mod 2N
+    int writemask;              // bit j set: slot j writes its bin (first occurrence only)
+};

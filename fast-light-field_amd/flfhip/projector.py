@@ -1,0 +1,193 @@
+"""Projector_hip: the reference's projector interface on the MI355X path.
+
+Same methods, argument meaning and return types as the reference projectors
+(projector.py:769-790 Projector_base, 798-896 Projector_allC):
+ForwardProjectACC / BackwardProjectACC plus asnative / asnumpy / nativeZeros /
+nativeDivide / assertNative and the attributes cacheFH, storeVolumesOnGPU and
+name.  Arithmetic runs only in libflf_hip.so (hand-written HIP kernels);
+PyTorch is used for device memory and the current stream.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def _stream_handle():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Projector_hip(object):
+    def __init__(self, device=0):
+        if not torch.cuda.is_available():
+            raise RuntimeError("Projector_hip needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.lib = _lib.load()
+        self.device = torch.device("cuda", device)
+        self.name = "HIP gfx950"
+        self.storeVolumesOnGPU = False
+        self.cacheFH = True          # spectra always stay resident in HBM
+        ctx = ctypes.c_void_p()
+        rc = self.lib.flf_create(device, ctypes.byref(ctx))
+        if rc != 0:
+            raise _lib.FLFError("flf_create failed (%d)" % rc)
+        self._ctx = ctx
+        self._key = None
+        self._loaded = set()
+
+    def __del__(self):
+        lib = getattr(self, "lib", None)
+        ctx = getattr(self, "_ctx", None)
+        if lib is not None and ctx:
+            lib.flf_destroy(ctx)
+            self._ctx = None
+
+    # ---- reference helper interface (projector.py:777-790) ----
+    def asnative(self, m):
+        if isinstance(m, torch.Tensor):
+            return m.to(self.device, dtype=torch.float32).contiguous()
+        return torch.from_numpy(np.ascontiguousarray(m, dtype=np.float32)).to(self.device)
+
+    def asnumpy(self, m):
+        if isinstance(m, torch.Tensor):
+            return m.detach().cpu().numpy()
+        return np.asarray(m)
+
+    def nativeZeros(self, shape, dtype=float):
+        return torch.zeros(shape, dtype=torch.float32, device=self.device)
+
+    def nativeDivide(self, a, b, out):
+        return torch.div(a, b, out=out)
+
+    def assertNative(self, m):
+        # Volumes handed to rlUpdateFunc are device tensors when
+        # storeVolumesOnGPU is set and numpy arrays otherwise.
+        if self.storeVolumesOnGPU:
+            assert isinstance(m, torch.Tensor) and m.is_cuda
+        else:
+            assert isinstance(m, np.ndarray)
+
+    # ---- planning ----
+    def _check(self, rc, what):
+        _lib.check(self.lib, self._ctx, rc, what)
+
+    def _ensure_plan(self, hMatrix, T, Y, X):
+        key = (id(hMatrix), getattr(hMatrix, "HPathFormat", None), hMatrix.numZ, getattr(hMatrix, "zStart", 0),
+               hMatrix.Nnum, T, Y, X)
+        if key == self._key:
+            return
+        s2 = []
+        for cc in range(hMatrix.numZ):
+            ps = hMatrix.PSFShape(cc)
+            if ps[0] != ps[1]:
+                raise ValueError("PSF must be square")
+            s2.append(int(ps[0]))
+        arr = (ctypes.c_int * len(s2))(*s2)
+        self._key = None
+        self._loaded = set()
+        self._check(self.lib.flf_plan(self._ctx, int(hMatrix.Nnum), int(Y), int(X), int(T), len(s2), arr), "flf_plan")
+        self._key = key
+        self._hMatrix = hMatrix
+
+    def _ensure_psf(self, hMatrix, planes, direction):
+        for cc in planes:
+            if (cc, direction) in self._loaded:
+                continue
+            Q = np.ascontiguousarray(hMatrix.Hcc(cc, direction == 1), dtype=np.float32)
+            self._check(self.lib.flf_set_psf(self._ctx, int(cc), int(direction), Q.ctypes.data_as(ctypes.c_void_p)),
+                        "flf_set_psf")
+            self._loaded.add((cc, direction))
+
+    def prepare(self, hMatrix, T, Y, X, planes=None, directions=(0, 1)):
+        """Plan and compute all spectra up front (the first-iteration cost of
+        the reference's F(H) computation, done once)."""
+        planes = range(hMatrix.numZ) if planes is None else planes
+        self._ensure_plan(hMatrix, T, Y, X)
+        for d in directions:
+            self._ensure_psf(hMatrix, planes, d)
+
+    def fshape(self, cc):
+        fy, fx = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.flf_plane_fshape(self._ctx, int(cc), ctypes.byref(fy), ctypes.byref(fx)), "fshape")
+        return (fy.value, fx.value)
+
+    @staticmethod
+    def _plane_array(planes, numZ):
+        planes = list(range(numZ)) if planes is None else [int(p) for p in planes]
+        return planes, (ctypes.c_int * len(planes))(*planes)
+
+    # ---- device-level entry points (tensors on this device) ----
+    def forward_device(self, hMatrix, vol, planes=None):
+        """vol: (numZ, T, Y, X) float32 cuda tensor -> image (T, Y, X) tensor."""
+        nz, T, Y, X = vol.shape
+        if nz != hMatrix.numZ:
+            raise ValueError("volume has %d planes, HMatrix has %d" % (nz, hMatrix.numZ))
+        self._ensure_plan(hMatrix, T, Y, X)
+        planes, arr = self._plane_array(planes, hMatrix.numZ)
+        self._ensure_psf(hMatrix, planes, 0)
+        vol = vol.contiguous()
+        img = torch.empty((T, Y, X), dtype=torch.float32, device=self.device)
+        self._check(self.lib.flf_forward(self._ctx, _ptr(vol), _ptr(img), arr, len(planes), _stream_handle()),
+                    "flf_forward")
+        return img
+
+    def backward_device(self, hMatrix, img, planes=None, out=None):
+        """img: (T, Y, X) cuda tensor -> volume (numZ, T, Y, X); planes not
+        listed are zero (Projector_allC.BackwardProjectACC, projector.py:807)."""
+        T, Y, X = img.shape
+        self._ensure_plan(hMatrix, T, Y, X)
+        planes, arr = self._plane_array(planes, hMatrix.numZ)
+        self._ensure_psf(hMatrix, planes, 1)
+        img = img.contiguous()
+        if out is None:
+            out = torch.zeros((hMatrix.numZ, T, Y, X), dtype=torch.float32, device=self.device)
+        self._check(self.lib.flf_backward(self._ctx, _ptr(img), _ptr(out), arr, len(planes), _stream_handle()),
+                    "flf_backward")
+        return out
+
+    def rl_update_device(self, hMatrix, img, X, Htf=None, trust=None, mode=1, planes=None):
+        """Back-project img and update X in place (mode 1 Prevedel, 2 standard)."""
+        T, Y, Xw = img.shape
+        self._ensure_plan(hMatrix, T, Y, Xw)
+        planes, arr = self._plane_array(planes, hMatrix.numZ)
+        self._ensure_psf(hMatrix, planes, 1)
+        hp = _ptr(Htf) if Htf is not None else ctypes.c_void_p(0)
+        tp = _ptr(trust) if trust is not None else ctypes.c_void_p(0)
+        self._check(self.lib.flf_backward_rl_update(self._ctx, _ptr(img.contiguous()), _ptr(X), hp, tp, int(mode),
+                                                    arr, len(planes), _stream_handle()), "flf_backward_rl_update")
+
+    def datafit_device(self, img, im):
+        """Per-frame FP64 (sum((img-im)^2), sum(im^2))."""
+        T = img.shape[0]
+        out = (ctypes.c_double * (2 * T))()
+        self._check(self.lib.flf_datafit(self._ctx, _ptr(img.contiguous()), _ptr(im.contiguous()), out,
+                                         _stream_handle()), "flf_datafit")
+        return np.array(out[:], dtype=np.float64).reshape(T, 2)
+
+    # ---- reference interface (projector.py:806-896) ----
+    def BackwardProjectACC(self, hMatrix, projection, planes, progress=None, logPrint=False, numjobs=1,
+                           rlUpdateFunc=None):
+        if len(projection.shape) != 3:
+            raise ValueError("projection must be (T, Y, X)")
+        if isinstance(projection, np.ndarray) and projection.dtype != np.float32:
+            raise TypeError("projection must be float32")
+        img = self.asnative(projection)
+        vol = self.backward_device(hMatrix, img, planes)
+        if rlUpdateFunc is not None:
+            rlUpdateFunc(vol if self.storeVolumesOnGPU else self.asnumpy(vol), slice(None))
+            return None
+        return vol if self.storeVolumesOnGPU else self.asnumpy(vol)
+
+    def ForwardProjectACC(self, hMatrix, realspace, planes, progress=None, logPrint=False, numjobs=1):
+        if len(realspace.shape) != 4:
+            raise ValueError("realspace must be (numZ, T, Y, X)")
+        if isinstance(realspace, np.ndarray) and realspace.dtype != np.float32:
+            raise TypeError("realspace must be float32")
+        vol = self.asnative(realspace)
+        img = self.forward_device(hMatrix, vol, planes)
+        return img if self.storeVolumesOnGPU else self.asnumpy(img)

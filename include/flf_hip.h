@@ -1,0 +1,104 @@
+/*
+ * flf_hip.h - C ABI of the MI355X (gfx950) light-field projector and
+ * Richardson-Lucy loop.
+ *
+ * One context per device.  The caller owns every image/volume buffer (device
+ * pointers); the context owns the PSF spectrum cache and its work buffers.
+ * Every entry point returns FLF_OK (0) or a negative FLF_E* code and never
+ * aborts; flf_last_error() gives the message.  A context is not re-entrant:
+ * calls on one context must not overlap (distinct contexts are independent).
+ *
+ * Array layouts (C order, float32):
+ *   volume  [nz][T][Y][X]      image  [T][Y][X]
+ *   PSF quadrant of one plane [h][h][s2][s2], h = (Nnum+1)/2, entry [b][a]
+ *   used for b <= a (the reference's HMatrix.Hcc layout, psfmatrix.py:46-66).
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef FLF_HIP_H
+#define FLF_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FLF_OK 0
+#define FLF_EINVAL (-1)   /* bad argument, shape or stride  (reference: TypeError/ValueError) */
+#define FLF_ENOMEM (-2)   /* device allocation failed        (reference: MemoryError)         */
+#define FLF_EHIP (-3)     /* HIP runtime / launch failure                                      */
+#define FLF_ESTATE (-4)   /* not planned, or PSF of a requested plane not uploaded             */
+
+typedef struct flf_ctx flf_ctx;
+
+/* Library/ABI version (no reference counterpart; lets bindings check the .so). */
+int flf_version(void);
+
+/* Create a context on HIP device `device`.
+ * Replaces the module-global state of py_light_field (py_light_field.cpp:
+ * 15-37, 1319-1363: thread count, planning mode, F(H) cache). */
+int flf_create(int device, flf_ctx** out);
+
+/* Destroy a context and free its device memory (ClearFHCache,
+ * py_light_field.cpp:1323-1335). */
+void flf_destroy(flf_ctx* ctx);
+
+/* Message of the last error on this context ("" if none). */
+const char* flf_last_error(const flf_ctx* ctx);
+
+/* Plan the geometry: lenslet pitch nnum (odd), image Y x X, T frames per
+ * batch, nz planes with odd PSF sides s2[z].  Chooses the Fourier grid of
+ * each plane (convolutionShape, special_fftconvolve.py:142-174, padded to a
+ * 5-smooth multiple of nnum), builds the kernel tables and allocates the
+ * spectrum cache and work buffers.  Replaces the per-plane set-up of
+ * ProjectorForZ_base.__init__ (projector.py:106-151).  Drops any uploaded
+ * PSFs. */
+int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2);
+
+/* Fourier grid of plane z after flf_plan (fshape of projector.py:123). */
+int flf_plane_fshape(const flf_ctx* ctx, int z, int* fY, int* fX);
+
+/* Bytes of device memory held by the spectrum cache and work buffers. */
+long long flf_device_bytes(const flf_ctx* ctx);
+
+/* Upload the quadrant PSF of plane z for direction dir (0: H, used by the
+ * forward projection; 1: Ht, used by the backward projection) from HOST
+ * memory, and compute its spectra into the cache.  Replaces FHWorkItem /
+ * TransposeWorkItem / MirrorWorkItem with caching enabled
+ * (py_light_field.cpp:711-870, 1365-1385): spectra persist until the next
+ * flf_plan or flf_destroy. */
+int flf_set_psf(flf_ctx* ctx, int z, int dir, const float* quadrant_host);
+
+/* Forward projection: img[T][Y][X] = sum over the listed planes of the
+ * projection of vol[z] (device pointers).  planes == NULL means all nz.
+ * Replaces ProjectForZList(backwards=0) + InverseRFFTList + CropAfterFFT and
+ * the plane sum of Projector_allC.ForwardProjectACC (projector.py:851-896;
+ * py_light_field.cpp:1417-1740).  stream: hipStream_t or NULL. */
+int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, int nplanes, void* stream);
+
+/* Backward projection: vol[z] for each listed plane (others untouched) from
+ * img[T][Y][X].  Replaces Projector_allC.BackwardProjectACC with
+ * rlUpdateFunc=None (projector.py:806-849). */
+int flf_backward(flf_ctx* ctx, const float* img, float* vol, const int* planes, int nplanes, void* stream);
+
+/* Backward projection of img fused with the Richardson-Lucy update of X:
+ *   mode 1 (Prevedel, lfdeconv.py:112-126):  X[z] *= Htf[z] / B(img)[z]
+ *   mode 2 (standard, lfdeconv.py:151-164):  X[z] *= B(img)[z] [/ trust[z]]
+ * then NaN -> 0.  Replaces BackwardProjectACC(..., rlUpdateFunc=...)
+ * (projector.py:844-847).  Htf is required for mode 1; trust (nz,1,Y,X) is
+ * optional for mode 2. */
+int flf_backward_rl_update(flf_ctx* ctx, const float* img, float* X, const float* Htf, const float* trust,
+                           int mode, const int* planes, int nplanes, void* stream);
+
+/* Forward-model data fit per frame, FP64 sums: out[2t] = sum((img-im)^2),
+ * out[2t+1] = sum(im^2) (host array of 2*T doubles).  The reference has no
+ * counterpart (DeconvRL always runs maxIter, lfdeconv.py:185-196); this is
+ * the convergence measure of the build's RL driver. Synchronises `stream`. */
+int flf_datafit(flf_ctx* ctx, const float* img, const float* im, double* out_host, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLF_HIP_H */

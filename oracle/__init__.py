@@ -1,0 +1,43 @@
+"""CPU oracle for the light-field projector and Richardson-Lucy loop.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product (``fast-light-field_amd/``)
+imports this package; only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` use it, and only as the checker or as the
+CPU baseline that is timed beside the GPU path.
+
+Everything here is an FP64 numpy restatement of the reference algorithm, with
+each function citing the reference file:line it follows:
+
+* ``shapes``    - Fourier-grid shape rules (special_fftconvolve.py:71-80,
+                  122-174).
+* ``psf``       - quadrant <-> full PSF expansion by the 8-fold symmetry
+                  (py_light_field.cpp:1514-1562, projector.py:159-189) and the
+                  adjoint back-projection PSF set.
+* ``projector`` - the direct per-offset convolution projector
+                  (projector.py:1084-1126) and a literal restatement of the
+                  special-FFT projector (py_light_field.cpp:284-341, 711-1102,
+                  1417-1740; projector.py:103-230, 792-896).
+* ``rl``        - Richardson-Lucy, Prevedel and standard updates
+                  (lfdeconv.py:72-202).
+
+How each function is pinned (none is "parity unpinned"):
+
+* ``shapes.convolution_shape``: the reference's own known answers
+  (py_light_field.cpp:1904-1925) and the closed-form definition.
+* ``psf.expand_quadrant``: every one of the N*N offsets is produced exactly
+  once from the quadrant; flips/transposes checked element-wise.
+* ``psf.adjoint_psf``: <F x, y> == <x, B y> to 1e-12 with B built from it,
+  on random inputs, through the direct projector.
+* ``projector.project_direct``: against an explicit shift-and-add loop
+  (brute force) on tiny inputs, and against its own linearity.
+* ``projector.project_special``: against ``project_direct`` (a different
+  algorithm: masked full-grid convolution vs lenslet-subsampled Fourier
+  tiling with mirrored/transposed spectra) to 1e-12, for square and
+  non-square grids, odd and even Fourier sizes.
+* ``rl.deconv_rl``: closed form of one iteration on a 1-plane identity-PSF
+  system, and monotone data-fit on a small synthetic problem.
+
+The compiled reference (py_light_field + vendored FFTW 3.3.8) is not built
+here: FFTW needs the headers its own configure step generates, which the
+rules for this oracle forbid producing or standing in for.
+"""

@@ -1,0 +1,55 @@
+"""GPU parity of the HIP projector against the FP64 oracle (through the C ABI)."""
+import numpy as np
+import pytest
+
+from flfhip import synthetic
+from oracle import projector as oproj
+from oracle import psf as opsf
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def _hm_random(N, s2_list, seed):
+    """Random non-negative symmetric-by-construction quadrants (a quadrant is
+    any array; the symmetry expansion defines the full set)."""
+    rng = np.random.default_rng(seed)
+    h = (N + 1) // 2
+    H = [rng.random((h, h, s, s)).astype(np.float32) for s in s2_list]
+    Ht = [rng.random((h, h, s, s)).astype(np.float32) for s in s2_list]
+    from flfhip.psfmatrix import MemoryHMatrix
+    return MemoryHMatrix(H, Ht)
+
+
+# The reference self-test shapes (projector.py:1178-1179): square and non-square.
+@pytest.mark.parametrize("shape", [(2, 150, 150), (2, 150, 300), (2, 300, 150)])
+@pytest.mark.parametrize("backward", [False, True])
+def test_selftest_shapes(projector, shape, backward):
+    hm = _hm_random(15, [61], seed=0)
+    rng = np.random.default_rng(0)
+    x = rng.random(shape).astype(np.float32)
+    print("case", shape, backward, flush=True)
+    if backward:
+        got = projector.BackwardProjectACC(hm, x, [0])[0]
+        ref = oproj.project_special(x, hm.Hcc(0, True))
+    else:
+        got = projector.ForwardProjectACC(hm, x[None], [0])
+        ref = oproj.project_special(x, hm.Hcc(0, False))
+    err = rel_l2(got, ref)
+    mx = float(np.max(np.abs(got - ref)))
+    print("rel-L2 %.3e max-abs %.3e" % (err, mx), flush=True)
+    assert err < 1e-5
+    assert mx < 1e-4 * max(1.0, float(np.max(np.abs(ref))))
+
+
+@pytest.mark.parametrize("N,Y,X,s2", [(5, 20, 25, 7), (15, 45, 60, 15), (15, 105, 105, 31), (3, 12, 12, 5)])
+def test_small_direct(projector, N, Y, X, s2):
+    hm = _hm_random(N, [s2], seed=N + Y)
+    rng = np.random.default_rng(1)
+    x = rng.random((1, 1, Y, X)).astype(np.float32)
+    got = projector.ForwardProjectACC(hm, x, [0])
+    ref = oproj.project_direct(x[0], opsf.expand_quadrant(hm.Hcc(0, False).astype(np.float64)))
+    assert rel_l2(got, ref) < 1e-5
