@@ -59,6 +59,11 @@ struct flf_ctx {
     size_t max_lds = 65536;
     long long bytes = 0;
     std::vector<void*> allocs;
+    // Optional timing of the multiply-accumulate launches (flf_profile_*).
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct ProfRec { hipEvent_t a, b; double bytes; };
+    std::vector<ProfRec> prof_recs;
 };
 
 namespace {
@@ -169,6 +174,39 @@ int set_enable(flf_ctx* ctx, Group& g, const std::vector<char>& sel, int dir, bo
     return FLF_OK;
 }
 
+hipEvent_t take_event(flf_ctx* ctx) {
+    if (!ctx->ev_pool.empty()) {
+        hipEvent_t e = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Launch the multiply-accumulate kernel; when profiling, bracket it with
+// events on the same stream and record its algorithmic bytes (the compulsory
+// read of the cached spectra of the planes it covers).
+int run_mac(flf_ctx* ctx, const MacArgs& m, const Group& g, int nplanes_grid, hipStream_t st) {
+    const GridTables& gt = g.gt;
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (ctx->prof) {
+        ea = take_event(ctx);
+        eb = take_event(ctx);
+        if (ea && eb) HIPCHK(hipEventRecord(ea, st));
+    }
+    HIPCHK(launch_mac(m, gt.slots, (int)gt.chunks.size(), nplanes_grid, ctx->T, gt.max_res, st));
+    if (ctx->prof && ea && eb) {
+        HIPCHK(hipEventRecord(eb, st));
+        int np = 0;
+        for (unsigned char e : g.enable_host) np += e ? 1 : 0;
+        double bytes = (double)np * gt.nu * gt.fY * gt.rfX * sizeof(float2);
+        ctx->prof_recs.push_back({ea, eb, bytes});
+    }
+    return FLF_OK;
+}
+
 // Small FFTs of the backward source (the image) for one group.
 int run_small_fft_image(flf_ctx* ctx, Group& g, const float* img, hipStream_t st) {
     SmallFFTArgs a{};
@@ -214,7 +252,8 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
         m.A_tstride = (long long)gt.fY * gt.rfX;
         m.A_pstride = (long long)T * gt.fY * gt.rfX;
         m.N = N; m.fY = gt.fY; m.fX = gt.fX; m.rY = gt.rY; m.rX = gt.rX; m.rfX = gt.rfX; m.nu = gt.nu;
-        HIPCHK(launch_mac(m, gt.slots, (int)gt.chunks.size(), m.nP, T, gt.max_res, st));
+        rc = run_mac(ctx, m, g, m.nP, st);
+        if (rc) return rc;
         IfftColArgs ic{};
         ic.A = g.d_A_bwd;
         ic.A_bstride = (long long)gt.fY * gt.rfX;
@@ -274,6 +313,8 @@ void flf_destroy(flf_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     free_all(ctx);
+    for (auto& r : ctx->prof_recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     delete ctx;
 }
 
@@ -496,7 +537,8 @@ int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, i
         m.A_tstride = (long long)gt.fY * gt.rfX;
         m.A_pstride = 0;
         m.N = N; m.fY = gt.fY; m.fX = gt.fX; m.rY = gt.rY; m.rX = gt.rX; m.rfX = gt.rfX; m.nu = gt.nu;
-        HIPCHK(launch_mac(m, gt.slots, (int)gt.chunks.size(), 1, T, gt.max_res, st));
+        rc = run_mac(ctx, m, g, 1, st);
+        if (rc) return rc;
         IfftColArgs ic{};
         ic.A = g.d_A_fwd;
         ic.A_bstride = (long long)gt.fY * gt.rfX;
@@ -531,6 +573,54 @@ int flf_datafit(flf_ctx* ctx, const float* img, const float* im, double* out_hos
     HIPCHK(hipMemcpyAsync(out_host, ctx->d_fitout, sizeof(double) * 2 * ctx->T, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return FLF_OK;
+}
+
+int flf_profile_enable(flf_ctx* ctx, int enable) {
+    if (!ctx) return FLF_EINVAL;
+    ctx->prof = enable != 0;
+    return FLF_OK;
+}
+
+int flf_profile_read(flf_ctx* ctx, double* mac_ms, long long* mac_launches, double* mac_bytes) {
+    if (!ctx || !mac_ms || !mac_launches || !mac_bytes) return FLF_EINVAL;
+    double ms = 0.0, by = 0.0;
+    long long n = 0;
+    for (auto& r : ctx->prof_recs) {
+        HIPCHK(hipEventSynchronize(r.b));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+        ms += t;
+        by += r.bytes;
+        ++n;
+        ctx->ev_pool.push_back(r.a);
+        ctx->ev_pool.push_back(r.b);
+    }
+    ctx->prof_recs.clear();
+    *mac_ms = ms;
+    *mac_launches = n;
+    *mac_bytes = by;
+    return FLF_OK;
+}
+
+int flf_grid_check(int nnum, int fY, int fX, long long* stats) {
+    if (!stats) return FLF_EINVAL;
+    GridTables gt;
+    std::string e = build_grid_tables(nnum, fY, fX, gt);
+    if (!e.empty()) return FLF_EINVAL;
+    int max_items = 0;
+    for (auto& c : gt.chunks) max_items = std::max(max_items, c.n_items);
+    stats[0] = (long long)gt.chunks.size();
+    stats[1] = (long long)gt.items.size();
+    stats[2] = gt.max_res;
+    stats[3] = gt.nu;
+    stats[4] = gt.slots;
+    stats[5] = max_items;
+    return FLF_OK;
+}
+
+int flf_fourier_len(int s1, int s2, int nnum) {
+    if (s1 < 1 || s2 < 1 || nnum < 1) return FLF_EINVAL;
+    return fourier_len(s1, s2, nnum);
 }
 
 }  // extern "C"

@@ -34,6 +34,11 @@ SIGNATURES = {
     "flf_backward_rl_update": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_void_p,
                                           _c.c_int, _c.POINTER(_c.c_int), _c.c_int, _c.c_void_p]),
     "flf_datafit": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.POINTER(_c.c_double), _c.c_void_p]),
+    "flf_fourier_len": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int]),
+    "flf_grid_check": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.POINTER(_c.c_longlong)]),
+    "flf_profile_enable": (_c.c_int, [_c.c_void_p, _c.c_int]),
+    "flf_profile_read": (_c.c_int, [_c.c_void_p, _c.POINTER(_c.c_double), _c.POINTER(_c.c_longlong),
+                                    _c.POINTER(_c.c_double)]),
 }
 
 _lib = None

@@ -169,6 +169,17 @@ class Projector_hip(object):
                                          _stream_handle()), "flf_datafit")
         return np.array(out[:], dtype=np.float64).reshape(T, 2)
 
+    def profile(self, enable):
+        """Bracket every multiply-accumulate launch with HIP events."""
+        self._check(self.lib.flf_profile_enable(self._ctx, 1 if enable else 0), "flf_profile_enable")
+
+    def profile_read(self):
+        """(summed MAC kernel ms, launches, algorithmic bytes) since the last read."""
+        ms, n, by = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double()
+        self._check(self.lib.flf_profile_read(self._ctx, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(by)),
+                    "flf_profile_read")
+        return ms.value, n.value, by.value
+
     # ---- reference interface (projector.py:806-896) ----
     def BackwardProjectACC(self, hMatrix, projection, planes, progress=None, logPrint=False, numjobs=1,
                            rlUpdateFunc=None):

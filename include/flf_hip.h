@@ -97,6 +97,26 @@ int flf_backward_rl_update(flf_ctx* ctx, const float* img, float* X, const float
  * the convergence measure of the build's RL driver. Synchronises `stream`. */
 int flf_datafit(flf_ctx* ctx, const float* img, const float* im, double* out_host, void* stream);
 
+/* Kernel timing for roofline reporting (no reference counterpart; the
+ * reference times whole calls with time.time(), lf_performance_analysis.py:
+ * 259-268).  While enabled, every multiply-accumulate launch is bracketed by
+ * HIP events on its own stream.  flf_profile_read synchronises those events
+ * and returns the summed kernel time, the number of launches and their
+ * algorithmic bytes (compulsory spectrum reads), then resets the record. */
+int flf_profile_enable(flf_ctx* ctx, int enable);
+int flf_profile_read(flf_ctx* ctx, double* mac_ms, long long* mac_launches, double* mac_bytes);
+
+/* Host-only planning queries (no GPU needed).
+ * flf_fourier_len: Fourier grid length of one axis for image side s1 and PSF
+ * side s2 (convolutionShape, special_fftconvolve.py:142-174, with 5-smooth
+ * padding of the reduced length).
+ * flf_grid_check: builds the kernel tables of an fY x fX grid and verifies
+ * that every half-plane bin is written exactly once; stats[6] = {workgroups,
+ * items, max residues per orbit, stored spectra per plane, slots per item,
+ * max items per workgroup}.  Returns FLF_EINVAL if the tables are invalid. */
+int flf_fourier_len(int s1, int s2, int nnum);
+int flf_grid_check(int nnum, int fY, int fX, long long* stats);
+
 #ifdef __cplusplus
 }
 #endif
