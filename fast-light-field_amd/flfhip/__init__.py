@@ -13,10 +13,9 @@ __all__ = ["Projector_hip", "lfdeconv", "psfmatrix", "synthetic"]
 def __getattr__(name):
     # The projector and driver import torch and the HIP library lazily, so
     # that the PSF containers and generators stay importable without them.
+    import importlib
     if name == "Projector_hip":
-        from .projector import Projector_hip
-        return Projector_hip
+        return importlib.import_module(".projector", __name__).Projector_hip
     if name == "lfdeconv":
-        from . import lfdeconv
-        return lfdeconv
+        return importlib.import_module(".lfdeconv", __name__)
     raise AttributeError(name)
