@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -25,6 +27,7 @@ struct Group {
     ItemDesc* d_items = nullptr;
     int* d_bu = nullptr;
     int* d_au = nullptr;
+    int* d_uoff = nullptr;
     int* d_s2 = nullptr;
     unsigned char* d_enable = nullptr;
     std::vector<unsigned char> enable_host;
@@ -36,6 +39,9 @@ struct Group {
     float2* d_A_fwd = nullptr;
     float2* d_A_bwd = nullptr;
     float2* d_T_fwd = nullptr;
+    float2* d_tw = nullptr;           // twiddle tables: fY | fX | fX/2 | rY | rX
+    const float2 *twY = nullptr, *twX = nullptr, *twXh = nullptr, *twRY = nullptr, *twRX = nullptr;
+    RadixPlan rpXh{};                 // fX/2 (even fX)
     long long Tbwd_off = 0;           // offset of this group's planes in the shared backward buffer
     std::vector<char> have[2];
 };
@@ -57,6 +63,7 @@ struct flf_ctx {
     double* d_fitpart = nullptr;
     double* d_fitout = nullptr;
     size_t max_lds = 65536;
+    int ipt = 2;                      // items per MAC thread (FLF_ITEMS_PER_THREAD overrides)
     long long bytes = 0;
     std::vector<void*> allocs;
     // Optional timing of the multiply-accumulate launches (flf_profile_*).
@@ -119,6 +126,14 @@ void free_all(flf_ctx* ctx) {
     ctx->d_plane_ids = nullptr;
     ctx->d_fitpart = nullptr;
     ctx->d_fitout = nullptr;
+}
+
+// exp(-2 pi i m / L), m < L, in double then rounded to float.
+void append_twiddles(std::vector<float2>& v, int L) {
+    for (int m = 0; m < L; ++m) {
+        double ang = -2.0 * 3.14159265358979323846 * (double)m / (double)L;
+        v.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+    }
 }
 
 int min_radix(const RadixPlan& rp) {
@@ -196,7 +211,7 @@ int run_mac(flf_ctx* ctx, const MacArgs& m, const Group& g, int nplanes_grid, hi
         eb = take_event(ctx);
         if (ea && eb) HIPCHK(hipEventRecord(ea, st));
     }
-    HIPCHK(launch_mac(m, gt.slots, (int)gt.chunks.size(), nplanes_grid, ctx->T, gt.max_res, st));
+    HIPCHK(launch_mac(m, gt.slots, gt.ipt, (int)gt.chunks.size(), nplanes_grid, ctx->T, gt.max_res, st));
     if (ctx->prof && ea && eb) {
         HIPCHK(hipEventRecord(eb, st));
         int np = 0;
@@ -216,6 +231,7 @@ int run_small_fft_image(flf_ctx* ctx, Group& g, const float* img, hipStream_t st
     a.src_pitch = ctx->X;
     a.Y = ctx->Y; a.X = ctx->X; a.N = ctx->N;
     a.rpY = g.rpYr; a.rpX = g.rpXr;
+    a.twY = g.twRY; a.twX = g.twRX;
     a.S = g.d_S_bwd;
     a.S_off = ctx->d_zeros;
     a.S_tstride = (long long)ctx->N * ctx->N * g.gt.rY * (g.gt.rX / 2 + 1);
@@ -242,7 +258,7 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
         const GridTables& gt = g.gt;
         MacArgs m{};
         m.G = g.d_G[1];
-        m.chunks = g.d_chunks; m.items = g.d_items; m.u_bu = g.d_bu; m.u_au = g.d_au;
+        m.chunks = g.d_chunks; m.items = g.d_items; m.u_bu = g.d_bu; m.u_au = g.d_au; m.uoff = g.d_uoff;
         m.S = g.d_S_bwd; m.S_off = ctx->d_zeros;
         m.S_tstride = (long long)N * N * gt.rY * (gt.rX / 2 + 1);
         m.s2 = g.d_s2; m.enable = g.d_enable;
@@ -259,6 +275,7 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
         ic.A_bstride = (long long)gt.fY * gt.rfX;
         ic.rfX = gt.rfX; ic.CB = g.CBi; ic.Yout = ctx->Y;
         ic.rpY = g.rpY;
+        ic.twY = g.twY;
         ic.out = ctx->d_T_bwd + g.Tbwd_off;
         ic.out_bstride = (long long)ctx->Y * gt.rfX;
         HIPCHK(launch_ifft_cols(ic, m.nP * T, st));
@@ -282,6 +299,7 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
     r.Tbuf = ctx->d_T_bwd;
     r.vol = vol; r.Htf = Htf; r.trust = trust;
     r.mode = mode; r.T = T; r.Y = ctx->Y; r.X = ctx->X;
+    r.maxfX = maxfX;
     HIPCHK(launch_ifft_rows_plane(r, (int)ids.size(), maxfX, st));
     return FLF_OK;
 }
@@ -305,6 +323,10 @@ int flf_create(int device, flf_ctx** out) {
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
         c->max_lds = (size_t)lds;
     if (configure_kernels(c->max_lds) != hipSuccess) c->max_lds = 65536;
+    if (const char* e = getenv("FLF_ITEMS_PER_THREAD")) {
+        int v = atoi(e);
+        if (v == 1 || v == 2 || v == 4) c->ipt = v;
+    }
     *out = c;
     return FLF_OK;
 }
@@ -345,10 +367,11 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
             if ((int)ctx->groups.size() >= FLF_MAX_GROUPS) return fail(ctx, FLF_EINVAL, "too many distinct Fourier grids");
             ctx->groups.emplace_back();
             Group& g = ctx->groups.back();
-            std::string e = build_grid_tables(nnum, fY, fX, g.gt);
+            std::string e = build_grid_tables(nnum, fY, fX, g.gt, ctx->ipt);
             if (!e.empty()) return fail(ctx, FLF_EINVAL, e);
             if (!make_radix_plan(fY, g.rpY) || !make_radix_plan(fX, g.rpX) ||
-                !make_radix_plan(g.gt.rY, g.rpYr) || !make_radix_plan(g.gt.rX, g.rpXr))
+                !make_radix_plan(g.gt.rY, g.rpYr) || !make_radix_plan(g.gt.rX, g.rpXr) ||
+                (fX % 2 == 0 && !make_radix_plan(fX / 2, g.rpXh)))
                 return fail(ctx, FLF_EINVAL, "Fourier grid is not 5-smooth");
             gi = (int)ctx->groups.size() - 1;
         }
@@ -367,19 +390,40 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
         maxP = std::max(maxP, nP);
         const int rXh = gt.rX / 2 + 1;
         // FFT workgroup shapes.
-        if (!fft_fits(g.rpXr, gt.rY, (size_t)gt.rY * gt.rX * sizeof(float2), ctx->max_lds) ||
-            !fft_fits(g.rpYr, rXh, (size_t)gt.rY * gt.rX * sizeof(float2), ctx->max_lds))
-            return fail(ctx, FLF_EINVAL, "reduced grid " + std::to_string(gt.rY) + "x" + std::to_string(gt.rX) +
-                                             " too large for the small-FFT kernel");
         g.RB = std::min(16, fft_batch_cap(g.rpX, ctx->max_lds));
         g.CBp = std::min(16, fft_batch_cap(g.rpY, ctx->max_lds));
-        g.CBi = g.CBp;
-        if (g.RB < 1 || g.CBp < 1) return fail(ctx, FLF_EINVAL, "Fourier grid too large for the LDS FFT");
+        g.CBi = 8;
+        while (g.CBi > 1 && ifft_cols_lds(g.CBi, gt.fY) > ctx->max_lds) --g.CBi;
+        if (g.RB < 1 || g.CBp < 1 || ifft_cols_lds(g.CBi, gt.fY) > ctx->max_lds)
+            return fail(ctx, FLF_EINVAL, "Fourier grid too large for the LDS FFT");
+        // Wave-level transforms hold at most 2048 (columns, odd rows), 1024
+        // (half-length rows) and 128 (reduced grid) points.
+        if (gt.fY > 2048 || gt.fX > 2048 || gt.rY > 128 || gt.rX > 128)
+            return fail(ctx, FLF_EINVAL, "Fourier grid " + std::to_string(gt.fY) + "x" + std::to_string(gt.fX) +
+                                             " exceeds the transform sizes this build supports");
+        if (small_fft_lds(gt.rY, gt.rX) > ctx->max_lds)
+            return fail(ctx, FLF_EINVAL, "reduced grid too large for the small-FFT kernel");
+        {
+            std::vector<float2> tw;
+            append_twiddles(tw, gt.fY);
+            append_twiddles(tw, gt.fX);
+            append_twiddles(tw, std::max(1, gt.fX / 2));
+            append_twiddles(tw, gt.rY);
+            append_twiddles(tw, gt.rX);
+            int rc2 = upload(ctx, &g.d_tw, tw);
+            if (rc2) return rc2;
+            g.twY = g.d_tw;
+            g.twX = g.twY + gt.fY;
+            g.twXh = g.twX + gt.fX;
+            g.twRY = g.twXh + std::max(1, gt.fX / 2);
+            g.twRX = g.twRY + gt.rY;
+        }
         int rc;
         if ((rc = upload(ctx, &g.d_chunks, gt.chunks))) return rc;
         if ((rc = upload(ctx, &g.d_items, gt.items))) return rc;
         if ((rc = upload(ctx, &g.d_bu, gt.u_bu))) return rc;
         if ((rc = upload(ctx, &g.d_au, gt.u_au))) return rc;
+        if ((rc = upload(ctx, &g.d_uoff, gt.uoff))) return rc;
         std::vector<int> s2g;
         std::vector<long long> srcoff, soff;
         const long long Sper = (long long)T * NN * gt.rY * rXh;
@@ -412,12 +456,13 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
             rd.rfX = gt.rfX;
             rd.scale = 1.0f / ((float)gt.fY * (float)gt.fX);
             rd.rp = g.rpX;
+            rd.rph = g.rpXh;
+            rd.twf = g.twX;
+            rd.twh = g.twXh;
         }
         Tbwd_total += (long long)nP * T * Y * gt.rfX;
-        // ifft rows: one row of fX plus the accumulation row.
-        if ((size_t)gt.fX * sizeof(float2) + (size_t)X * sizeof(float) > ctx->max_lds ||
-            !fft_fits(g.rpX, 1, 0, ctx->max_lds))
-            return fail(ctx, FLF_EINVAL, "image row too long for the LDS FFT");
+        if (ifft_rows_sum_lds(gt.fX) > ctx->max_lds || X > 2048)
+            return fail(ctx, FLF_EINVAL, "image row too long for the row transform");
     }
     int rc;
     if ((rc = dalloc(ctx, &ctx->d_T_bwd, (size_t)Tbwd_total))) return rc;
@@ -475,7 +520,7 @@ int flf_set_psf(flf_ctx* ctx, int z, int dir, const float* quadrant_host) {
     SpecPackArgs pa{};
     pa.spec = dspec; pa.chunks = g.d_chunks; pa.items = g.d_items; pa.u_bu = g.d_bu; pa.u_au = g.d_au;
     pa.fY = gt.fY; pa.rfX = gt.rfX; pa.N = ctx->N; pa.slots = gt.slots; pa.nu = gt.nu;
-    pa.nP = (int)g.planes.size(); pa.p = ctx->idx_in_group[z]; pa.G = g.d_G[dir];
+    pa.nP = (int)g.planes.size(); pa.p = ctx->idx_in_group[z]; pa.G = g.d_G[dir]; pa.s2 = s2;
     if (e == hipSuccess) e = launch_spec_pack(pa, (int)gt.chunks.size(), 0);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     cleanup();
@@ -496,6 +541,7 @@ int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, i
     IfftRowSumArgs rs{};
     rs.ngroups = (int)ctx->groups.size();
     rs.X = ctx->X;
+    rs.Y = ctx->Y;
     rs.img = img;
     rs.img_tstride = (long long)ctx->Y * ctx->X;
     rs.maxfX = 1;
@@ -511,6 +557,9 @@ int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, i
         rs.grp[gi].rfX = gt.rfX;
         rs.grp[gi].scale = 1.0f / ((float)gt.fY * (float)gt.fX);
         rs.grp[gi].rp = g.rpX;
+        rs.grp[gi].rph = g.rpXh;
+        rs.grp[gi].twf = g.twX;
+        rs.grp[gi].twh = g.twXh;
         if (!any) continue;
         rs.maxfX = std::max(rs.maxfX, gt.fX);
         const int nP = (int)g.planes.size();
@@ -522,13 +571,14 @@ int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, i
         a.src_pitch = ctx->X;
         a.Y = ctx->Y; a.X = ctx->X; a.N = N;
         a.rpY = g.rpYr; a.rpX = g.rpXr;
+        a.twY = g.twRY; a.twX = g.twRX;
         a.S = g.d_S_fwd;
         a.S_off = g.d_Soff_fwd;
         a.S_tstride = (long long)NN * gt.rY * rXh;
         HIPCHK(launch_small_fft(a, nP, T, st));
         MacArgs m{};
         m.G = g.d_G[0];
-        m.chunks = g.d_chunks; m.items = g.d_items; m.u_bu = g.d_bu; m.u_au = g.d_au;
+        m.chunks = g.d_chunks; m.items = g.d_items; m.u_bu = g.d_bu; m.u_au = g.d_au; m.uoff = g.d_uoff;
         m.S = g.d_S_fwd; m.S_off = g.d_Soff_fwd; m.S_tstride = a.S_tstride;
         m.s2 = g.d_s2; m.enable = g.d_enable;
         m.nP = nP;
@@ -544,6 +594,7 @@ int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, i
         ic.A_bstride = (long long)gt.fY * gt.rfX;
         ic.rfX = gt.rfX; ic.CB = g.CBi; ic.Yout = ctx->Y;
         ic.rpY = g.rpY;
+        ic.twY = g.twY;
         ic.out = g.d_T_fwd;
         ic.out_bstride = (long long)ctx->Y * gt.rfX;
         HIPCHK(launch_ifft_cols(ic, T, st));

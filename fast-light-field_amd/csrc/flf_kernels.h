@@ -11,6 +11,8 @@ struct SmallFFTArgs {
     int src_pitch;               // row pitch (elements)
     int Y, X, N;
     RadixPlan rpY, rpX;          // lengths rY, rX
+    const float2* twY;           // exp(-2 pi i m / rY), m < rY
+    const float2* twX;           // exp(-2 pi i m / rX)
     float2* S;
     const long long* S_off;      // per source plane z: element offset into S (device array)
     long long S_tstride;         // per frame
@@ -39,6 +41,7 @@ struct SpecPackArgs {
     const int* u_bu;
     const int* u_au;
     int fY, rfX, N, slots, nu, nP, p;
+    int s2;                      // PSF side of the plane (sets the folded phase ratios)
     float2* G;                   // orbit-major spectra of the grid group
 };
 
@@ -48,6 +51,7 @@ struct MacArgs {
     const ItemDesc* items;
     const int* u_bu;
     const int* u_au;
+    const int* uoff;             // [nu][8 or 4]: s-table column of each use, N*N if absent
     const float2* S;
     const long long* S_off;      // per plane in group
     long long S_tstride;
@@ -65,6 +69,7 @@ struct IfftColArgs {
     long long A_bstride;
     int rfX, CB, Yout;
     RadixPlan rpY;
+    const float2* twY;           // exp(-2 pi i m / fY)
     float2* out;
     long long out_bstride;
 };
@@ -76,13 +81,16 @@ struct IfftRowGroup {
     int active;
     float scale;
     RadixPlan rp;                // length fX
+    RadixPlan rph;               // length fX/2 (even fX)
+    const float2* twf;           // exp(-2 pi i m / fX)
+    const float2* twh;           // exp(-2 pi i m / (fX/2))
 };
 
 struct IfftRowSumArgs {
     IfftRowGroup grp[FLF_MAX_GROUPS];
     int ngroups;
     int maxfX;
-    int X;
+    int X, Y;
     float* img;
     long long img_tstride;
 };
@@ -92,7 +100,10 @@ struct RowPlaneDesc {
     long long T_tstride;
     int rfX;
     float scale;
-    RadixPlan rp;
+    RadixPlan rp;                // length fX
+    RadixPlan rph;               // length fX/2 (even fX)
+    const float2* twf;
+    const float2* twh;
 };
 
 struct IfftRowPlaneArgs {
@@ -104,14 +115,20 @@ struct IfftRowPlaneArgs {
     const float* trust;          // mode 2 (optional, (nz, 1, Y, X))
     int mode;                    // 0 store, 1 Prevedel update, 2 standard update
     int T, Y, X;
+    int maxfX;                   // largest fX of the processed planes (LDS sizing)
 };
 
 namespace flf {
+size_t small_fft_lds(int rY, int rX);
+size_t ifft_cols_lds(int CB, int fY);
+size_t ifft_rows_sum_lds(int maxfX);
+size_t ifft_rows_plane_lds(int maxfX);
 hipError_t launch_small_fft(const SmallFFTArgs& a, int nsrc, int T, hipStream_t s);
 hipError_t launch_psf_rows(const PsfRowArgs& a, int nu, hipStream_t s);
 hipError_t launch_psf_cols(const PsfColArgs& a, int nu, hipStream_t s);
 hipError_t launch_spec_pack(const SpecPackArgs& a, int nchunks, hipStream_t s);
-hipError_t launch_mac(const MacArgs& a, int slots, int nchunks, int nplanes_grid, int T, int max_res, hipStream_t s);
+hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int nchunks, int nplanes_grid, int T, int max_res,
+                      hipStream_t s);
 hipError_t launch_ifft_cols(const IfftColArgs& a, int nbatch, hipStream_t s);
 hipError_t launch_ifft_rows_sum(const IfftRowSumArgs& a, int Y, int T, hipStream_t s);
 hipError_t launch_ifft_rows_plane(const IfftRowPlaneArgs& a, int nplanes, int maxfX, hipStream_t s);

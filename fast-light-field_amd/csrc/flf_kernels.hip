@@ -12,6 +12,7 @@
 #include "flf_common.h"
 #include "flf_fft.h"
 #include "flf_kernels.h"
+#include "flf_wfft.h"
 
 using namespace flf;
 
@@ -30,35 +31,62 @@ __device__ __forceinline__ int centred_res(int k, int r) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Small FFTs of the lenslet-subsampled source (py_light_field.cpp:1001-1027).
-// grid (N*N, T, nsrc); dynamic LDS rY*rX float2.
-// Output half spectrum: S[z][t][off][k0][k1], k1 in [0, rX/2].
+// Small FFTs of the lenslet-subsampled source (py_light_field.cpp:1001-1027):
+// S_{b,a} = FFT_r(src[b::N, a::N] zero-padded to rY x rX), half spectrum.
+// grid (N*N, T, nsrc), block 256 (4 waves).  Rows are real, so two rows are
+// transformed as one complex sequence and separated afterwards; then each
+// wave transforms columns of the half spectrum.
+// Output S[z][t][off][k0][k1], k1 in [0, rX/2].
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(FFT_THREADS) void k_small_fft(SmallFFTArgs a) {
-    extern __shared__ float2 sh[];
+__global__ __launch_bounds__(256) void k_small_fft(SmallFFTArgs a) {
+    extern __shared__ float2 lds[];
     const int off = blockIdx.x;
     const int t = blockIdx.y;
     const int z = blockIdx.z;
     const int N = a.N;
     const int b = off / N, aa = off % N;
     const int rY = a.rpY.L, rX = a.rpX.L, rXh = rX / 2 + 1;
+    float2* H = lds;                              // [rY][rXh]
+    float2* twY = H + rY * rXh;                   // [rY]
+    float2* twX = twY + rY;                       // [rX]
+    float2* scratch = twX + rX;                   // [4][rX]
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < rY; i += blockDim.x) twY[i] = a.twY[i];
+    for (int i = threadIdx.x; i < rX; i += blockDim.x) twX[i] = a.twX[i];
     const float* src = a.src + a.src_off[z] + (long long)t * a.src_tstride;
     const int Ys = (a.Y - b + N - 1) / N;
     const int Xs = (a.X - aa + N - 1) / N;
-    for (int i = threadIdx.x; i < rY * rX; i += blockDim.x) sh[i] = make_float2(0.f, 0.f);
+    for (int i = Ys * rXh + threadIdx.x; i < rY * rXh; i += blockDim.x) H[i] = make_float2(0.f, 0.f);
     __syncthreads();
-    for (int i = threadIdx.x; i < Ys * Xs; i += blockDim.x) {
-        int y2 = i / Xs, x2 = i - y2 * Xs;
-        sh[y2 * rX + x2] = make_float2(src[(long long)(b + N * y2) * a.src_pitch + aa + N * x2], 0.f);
+    float2* sc = scratch + wv * rX;
+    for (int pr = wv; 2 * pr < Ys; pr += 4) {
+        const int y0 = 2 * pr, y1 = 2 * pr + 1;
+        const float* r0 = src + (long long)(b + N * y0) * a.src_pitch + aa;
+        const float* r1 = src + (long long)(b + N * y1) * a.src_pitch + aa;
+        for (int x = lane; x < rX; x += 64) {
+            float v0 = 0.f, v1 = 0.f;
+            if (x < Xs) {
+                v0 = r0[N * x];
+                if (y1 < Ys) v1 = r1[N * x];
+            }
+            sc[x] = make_float2(v0, v1);
+        }
+        wave_sync();
+        wave_fft<128>(sc, 1, a.rpX, twX, false, lane);
+        for (int k = lane; k < rXh; k += 64) {
+            const float2 zk = sc[k];
+            const float2 zc = cconj(sc[(rX - k) % rX]);
+            // row y0: (Z + Zc)/2 ; row y1: (Z - Zc)/(2i)
+            H[y0 * rXh + k] = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
+            if (y1 < Ys) H[y1 * rXh + k] = make_float2(0.5f * (zk.y - zc.y), -0.5f * (zk.x - zc.x));
+        }
+        wave_sync();
     }
     __syncthreads();
-    lds_fft<FFT_MAXB>(sh, a.rpX, Ys, 1, rX, -1.f);          // rows (only non-zero ones)
-    lds_fft<FFT_MAXB>(sh, a.rpY, rXh, rX, 1, -1.f);         // columns of the half plane
+    for (int k = wv; k < rXh; k += 4) wave_fft<128>(H + k, rXh, a.rpY, twY, false, lane);
+    __syncthreads();
     float2* S = a.S + a.S_off[z] + (long long)t * a.S_tstride + (long long)off * rY * rXh;
-    for (int i = threadIdx.x; i < rY * rXh; i += blockDim.x) {
-        int k0 = i / rXh, k1 = i - k0 * rXh;
-        S[i] = sh[k0 * rX + k1];
-    }
+    for (int i = threadIdx.x; i < rY * rXh; i += blockDim.x) S[i] = H[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -125,63 +153,50 @@ __global__ __launch_bounds__(FFT_THREADS) void k_psf_cols(PsfColArgs a) {
 // grid (nchunks, nu), block FLF_CHUNK_MAX.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(FLF_CHUNK_MAX) void k_spec_pack(SpecPackArgs a) {
-    const ChunkDesc ch = a.chunks[blockIdx.x];
+    const ChunkDesc* ch = a.chunks + blockIdx.x;
     const int u = blockIdx.y;
     const int i = threadIdx.x;
-    if (i >= ch.n_items) return;
-    const ItemDesc it = a.items[ch.item_base + i];
+    const int n_items = ch->n_items;
+    if (i >= n_items) return;
+    const long long item_base = ch->item_base;
+    const ItemDesc it = a.items[item_base + i];
     const int bu = a.u_bu[u], au = a.u_au[u];
+    const int sig = a.s2 - 2 + a.N;
+    const int m2 = it.m2[0], n2 = it.n2[0];
+    // rho_i^{-1/2}: the phase ratio of the two uses reading slot i, the same
+    // for all output slots of the item (exponents in 1/(4N) turns).
+    int rho[4];
+    rho[0] = sig * (m2 + n2);
+    rho[1] = sig * (n2 - m2);
+    rho[2] = sig * (m2 + n2);
+    rho[3] = sig * (m2 - n2);
     const float2* spec = a.spec + (long long)u * a.fY * a.rfX;
-    float2* G = a.G + ch.item_base * a.slots * a.nu * a.nP +
-                (((long long)a.p * a.nu + u) * ch.n_items + i) * a.slots;
+    float2* G = a.G + item_base * a.slots * a.nu * a.nP +
+                (((long long)a.p * a.nu + u) * n_items + i) * a.slots;
     for (int j = 0; j < a.slots; ++j) {
         float2 v = spec[it.pos[j]];
-        float2 w = phase_frac((long long)bu * it.m2[j] + (long long)au * it.n2[j], 2 * a.N);
-        G[j] = cmul(v, w);
+        long long e = 2LL * ((long long)bu * it.m2[j] + (long long)au * it.n2[j]) - rho[j];
+        G[j] = cmul(v, phase_frac(e, 4 * a.N));
     }
 }
 
 // ---------------------------------------------------------------------------
-// The multiply-accumulate over the 225 lenslet offsets.
-//
-// For an output bin k with residue kappa and (half-unit) alias (m2, n2),
-//   A(k) = C0 + w^(sig*m2) Cm + w^(sig*n2) Cn + w^(sig*(m2+n2)) Cmn,
-//   sig = s2 - 2 + N,
-// where each class sum runs over the stored spectra u and their uses
-// (identity, flips, transposes), a use contributing s_off(kappa) * G'_u(q)
-// or s_off(kappa) * conj(G'_u(q)) with q one of the item's slots.  s_off is
-// the small FFT of offset (b, a) times exp(-2 pi i (beta_b kt0/fY +
-// alpha_a kt1/fX)), beta/alpha the offset index plus s2-1 when flipped.
-// This is the accumulation of py_light_field.cpp:303-341, 1029-1087 with the
-// mirror multipliers of projector.py:137-151 and the 'same' crop
-// (projector.py:792-796) folded into phases.
+// The multiply-accumulate over the N*N lenslet offsets (k_mac below).
 // ---------------------------------------------------------------------------
 namespace {
 
-// Use types: 0 id, 1 both-flip, 2 row-flip, 3 col-flip, 4..7 the same for the
-// transposed PSF.  Per use: (which read of the output's bin set, conj, class).
-//   read 0: k, 1: (-k0, k1), 2: (k1, k0), 3: (-k1, k0)
-__device__ constexpr int kUseRead[8] = {0, 0, 1, 1, 2, 2, 3, 3};
-__device__ constexpr int kUseConj[8] = {0, 1, 0, 1, 0, 1, 1, 0};
-__device__ constexpr int kUseClass[8] = {0, 3, 1, 2, 0, 3, 1, 2};
-// Quad slots P0=(p0,p1) P1=(-p0,p1) P2=(p1,p0) P3=(-p1,p0): for output slot j,
-// the slot holding read r, and whether Hermitian symmetry adds a conj.
-__device__ constexpr int kQuadSrc[4][4] = {{0, 1, 2, 3}, {1, 0, 3, 2}, {2, 3, 0, 1}, {3, 2, 1, 0}};
-__device__ constexpr int kQuadHc[4][4] = {{0, 0, 0, 0}, {0, 0, 1, 1}, {0, 0, 0, 0}, {0, 0, 1, 1}};
-
-template <int SLOTS, int TYPE>
-__device__ __forceinline__ void mac_use(float2 (&acc)[4][4], const float2 (&g)[4],
-                                        const float2* __restrict__ s_tab, const int (&rb)[4], int off) {
-#pragma unroll
-    for (int j = 0; j < SLOTS; ++j) {
-        constexpr int rd = kUseRead[TYPE];
-        const int src = (SLOTS == 4) ? kQuadSrc[j][rd] : (rd == 0 ? j : 1 - j);
-        const int cj = kUseConj[TYPE] ^ ((SLOTS == 4) ? kQuadHc[j][rd] : 0);
-        const float2 s = s_tab[rb[j] + off];
-        if (cj) cmacc(acc[j][kUseClass[TYPE]], s, g[src]);
-        else    cmac(acc[j][kUseClass[TYPE]], s, g[src]);
-    }
-}
+// Use types (order of the per-spectrum use table): 0 id, 1 both flips,
+// 2 row flip, 3 column flip, 4..7 the same for the transposed PSF.
+// The two uses that read the same slot of an item always pair a plain and a
+// conjugated read; their pair types (plain, conj) are
+//   quad:  0 (id, both)  1 (row, col)  2 (Tid, Tboth)  3 (Tcol, Trow)
+//   pair:  0 (id, both)  1 (row, col)
+__device__ constexpr int kPairPlain[4] = {0, 2, 4, 7};
+__device__ constexpr int kPairConj[4] = {1, 3, 5, 6};
+// Output slot j, source slot i -> pair type and sign of the M term.
+__device__ constexpr int kQuadPT[4][4] = {{0, 1, 2, 3}, {1, 0, 3, 2}, {2, 3, 0, 1}, {3, 2, 1, 0}};
+__device__ constexpr int kQuadSG[4][4] = {{1, 1, 1, 1}, {1, 1, -1, -1}, {1, 1, 1, 1}, {-1, -1, 1, 1}};
+__device__ constexpr int kPairPT[2][2] = {{0, 1}, {1, 0}};
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -204,267 +219,337 @@ __device__ __forceinline__ void load_g(float2 (&g)[4], const float2* p) {
 
 }  // namespace
 
-template <int SLOTS>
-__global__ __launch_bounds__(FLF_CHUNK_MAX) void k_mac(MacArgs a) {
-    extern __shared__ float2 s_tab[];
-    const ChunkDesc ch = a.chunks[blockIdx.x];
-    const int tid = threadIdx.x;
-    const bool valid = tid < ch.n_items;
-    const int N = a.N, NN = N * N, c = N / 2, twoN = 2 * N;
-    const int t = blockIdx.z;
-    ItemDesc it;
-    if (valid) it = a.items[ch.item_base + tid];
-    else {
-        for (int j = 0; j < 4; ++j) { it.pos[j] = 0; it.res[j] = 0; it.m2[j] = 0; it.n2[j] = 0; }
-        it.writemask = 0;
+// acc[a][j] += P * Re(g[a][i]) + sg * M * Im(g[a][i]) for every output slot
+// j, source slot i and item a of the thread (the items share residues, so
+// each 16-byte LDS entry (P, M) serves all of them).
+// P = s_plain + s_conj, M = i (s_plain - s_conj).
+template <int SLOTS, int IPT>
+__device__ __forceinline__ void mac_spectrum(float2 (&acc)[IPT][4], const float2 (&g)[IPT][4],
+                                             const float4* __restrict__ pm, const int (&rb)[4], int ucol) {
+    constexpr int NPT = SLOTS == 4 ? 4 : 2;
+#pragma unroll
+    for (int j = 0; j < SLOTS; ++j) {
+        const float4* row = pm + (rb[j] + ucol) * NPT;
+#pragma unroll
+        for (int i = 0; i < SLOTS; ++i) {
+            const int pt = SLOTS == 4 ? kQuadPT[j][i] : kPairPT[j][i];
+            const float sg = SLOTS == 4 ? (float)kQuadSG[j][i] : 1.0f;
+            const float4 q = row[pt];          // (P.x, P.y, M.x, M.y)
+            const float mx = sg * q.z, my = sg * q.w;
+#pragma unroll
+            for (int a = 0; a < IPT; ++a) {
+                acc[a][j].x = fmaf(q.x, g[a][i].x, acc[a][j].x);
+                acc[a][j].y = fmaf(q.y, g[a][i].x, acc[a][j].y);
+                acc[a][j].x = fmaf(mx, g[a][i].y, acc[a][j].x);
+                acc[a][j].y = fmaf(my, g[a][i].y, acc[a][j].y);
+            }
+        }
     }
+}
+
+#ifndef FLF_MAC_UNR
+#define FLF_MAC_UNR 2
+#endif
+#ifndef FLF_MAC_WAVES
+#define FLF_MAC_WAVES 3
+#endif
+
+// For an item with base alias (m2, n2) (half units) and output slots
+// P0..P3, every output accumulates
+//   A_j = w4^(outer_j) * sum_u sum_i [ P_{j,i,u} Re g_i + M_{j,i,u} Im g_i ]
+// with g_i the stored spectrum value of slot i (pre-multiplied by
+// w2^(bu m2_i + au n2_i) and rho_i^{-1/2}), w4 = exp(-2 pi i/(4N)),
+// outer = sig*(m2+n2), sig*(n2-m2), sig*(n2+m2), sig*(m2-n2), sig = s2-2+N.
+// P and M combine the small-FFT values s of the two uses reading slot i at
+// output j, s = S_{b,a}(kappa) exp(-2 pi i (beta_b kt0/fY + alpha_a kt1/fX)),
+// beta = b (b <= N/2) or b + s2 - 1 (row-flipped offsets), likewise alpha.
+// This is the accumulation of py_light_field.cpp:303-341, 1029-1087 with the
+// mirror multipliers of projector.py:137-151 and the 'same' crop
+// (projector.py:792-796) folded into phases.  Each thread handles IPT
+// consecutive items that share their residues.
+template <int SLOTS, int IPT>
+__global__ __launch_bounds__(FLF_CHUNK_MAX) __attribute__((amdgpu_waves_per_eu(IPT >= 4 ? 2 : FLF_MAC_WAVES)))
+void k_mac(MacArgs a) {
+    extern __shared__ float4 pm_tab[];
+    constexpr int NUSE = SLOTS == 4 ? 8 : 4;
+    constexpr int NPT = SLOTS == 4 ? 4 : 2;
+    constexpr int UNR = IPT >= 4 ? 1 : FLF_MAC_UNR;   // spectra whose loads are issued together
+    const ChunkDesc* ch = a.chunks + blockIdx.x;
+    const long long item_base = ch->item_base;
+    const int n_items = ch->n_items;
+    const int n_res = ch->n_res;
+    const int tid = threadIdx.x;
+    const bool valid = tid * IPT < n_items;
+    const int tsafe = valid ? tid : 0;           // idle lanes read item 0 (in bounds) and never write
+    const int N = a.N, NN = N * N, c = N / 2, fourN = 4 * N;
+    const int t = blockIdx.z;
+    const int nu = a.nu;
+    ItemDesc it[IPT];
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) it[q] = a.items[item_base + tsafe * IPT + q];
     int rb[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) rb[j] = it.res[j] * NN;
-    int k0s[4], k1s[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) { k0s[j] = it.pos[j] / a.rfX; k1s[j] = it.pos[j] - k0s[j] * a.rfX; }
+    for (int j = 0; j < 4; ++j) rb[j] = it[0].res[j] * nu;
 
-    float2 tot[4];
+    float2 tot[IPT][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) tot[j] = make_float2(0.f, 0.f);
+    for (int q = 0; q < IPT; ++q)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) tot[q][j] = make_float2(0.f, 0.f);
 
     const int pb = a.plane_loop ? 0 : blockIdx.y;
     const int pe = a.plane_loop ? a.nP : blockIdx.y + 1;
     const int rXh = a.rX / 2 + 1;
     const long long S_offstride = (long long)a.rY * rXh;
-    const int nu = a.nu;
+    const long long ustride = (long long)n_items * SLOTS;
 
     for (int p = pb; p < pe; ++p) {
         if (!a.enable[p]) continue;
         const int s2 = a.s2[p];
         __syncthreads();
-        // ---- s-table: n_res residues x N*N offsets, phases for this plane ----
+        // ---- P/M table: n_res residues x nu spectra x NPT pair types ----
         const float2* Sp = a.S + a.S_off[p] + (long long)t * a.S_tstride;
-        for (int idx = tid; idx < ch.n_res * NN; idx += blockDim.x) {
-            const int rho = idx / NN, off = idx - rho * NN;
-            const int bo = off / N, ao = off - bo * N;
-            const int k0 = ch.res0[rho], k1 = ch.res1[rho];
-            float2 v;
-            if (2 * k1 <= a.rX) v = Sp[off * S_offstride + (long long)k0 * rXh + k1];
-            else v = cconj(Sp[off * S_offstride + (long long)((a.rY - k0) % a.rY) * rXh + (a.rX - k1)]);
-            const long long beta = bo <= c ? bo : bo + s2 - 1;
-            const long long alpha = ao <= c ? ao : ao + s2 - 1;
-            const long long kt0 = centred_res(k0, a.rY), kt1 = centred_res(k1, a.rX);
-            long long n0 = (beta * kt0) % a.fY; if (n0 < 0) n0 += a.fY;
-            long long n1 = (alpha * kt1) % a.fX; if (n1 < 0) n1 += a.fX;
-            float fr = (float)n0 / (float)a.fY + (float)n1 / (float)a.fX;
-            fr -= rintf(fr);
-            float sn, cs;
-            sincospif(-2.0f * fr, &sn, &cs);
-            s_tab[idx] = cmul(v, make_float2(cs, sn));
+        for (int idx = tid; idx < n_res * nu * NPT; idx += blockDim.x) {
+            const int pt = idx % NPT;
+            const int ru = idx / NPT;
+            const int u = ru % nu, rho = ru / nu;
+            const int k0 = ch->res0[rho], k1 = ch->res1[rho];
+            const int kt0 = centred_res(k0, a.rY), kt1 = centred_res(k1, a.rX);
+            float2 sv[2];
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+                const int off = a.uoff[u * NUSE + (w == 0 ? kPairPlain[pt] : kPairConj[pt])];
+                float2 r = make_float2(0.f, 0.f);
+                if (off < NN) {
+                    const int bo = off / N, ao = off - bo * N;
+                    float2 v;
+                    if (2 * k1 <= a.rX) v = Sp[off * S_offstride + (long long)k0 * rXh + k1];
+                    else v = cconj(Sp[off * S_offstride + (long long)((a.rY - k0) % a.rY) * rXh + (a.rX - k1)]);
+                    const int beta = bo <= c ? bo : bo + s2 - 1;
+                    const int alpha = ao <= c ? ao : ao + s2 - 1;
+                    int n0 = (int)(((long long)beta * kt0) % a.fY); if (n0 < 0) n0 += a.fY;
+                    int n1 = (int)(((long long)alpha * kt1) % a.fX); if (n1 < 0) n1 += a.fX;
+                    float fr = (float)n0 / (float)a.fY + (float)n1 / (float)a.fX;
+                    fr -= rintf(fr);
+                    float sn, cs;
+                    sincospif(-2.0f * fr, &sn, &cs);
+                    r = cmul(v, make_float2(cs, sn));
+                }
+                sv[w] = r;
+            }
+            const float2 P = cadd(sv[0], sv[1]);
+            const float2 d = csub(sv[0], sv[1]);
+            pm_tab[idx] = make_float4(P.x, P.y, -d.y, d.x);     // M = i * d
         }
         __syncthreads();
 
-        float2 acc[4][4];
+        float2 acc[IPT][4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int q = 0; q < IPT; ++q)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc[j][q] = make_float2(0.f, 0.f);
+            for (int j = 0; j < 4; ++j) acc[q][j] = make_float2(0.f, 0.f);
 
-        const float2* Gp = a.G + ch.item_base * SLOTS * nu * a.nP +
-                           ((long long)p * nu) * ch.n_items * SLOTS + (long long)tid * SLOTS;
-        const long long ustride = (long long)ch.n_items * SLOTS;
-        float2 g[4];
-
-        if constexpr (SLOTS == 4) {
-            // u = 0: (c, c) -> identity only
-            if (valid) {
-                load_g<4>(g, Gp);
-                mac_use<4, 0>(acc, g, s_tab, rb, c * N + c);
-            }
-            // u in [1, 1+c): (b, b) -> id, both, row, col
-            for (int u = 1; u < 1 + c; ++u) {
-                const int bu = u - 1;
-                if (valid) {
-                    load_g<4>(g, Gp + u * ustride);
-                    const int bb = N - 1 - bu;
-                    mac_use<4, 0>(acc, g, s_tab, rb, bu * N + bu);
-                    mac_use<4, 1>(acc, g, s_tab, rb, bb * N + bb);
-                    mac_use<4, 2>(acc, g, s_tab, rb, bb * N + bu);
-                    mac_use<4, 3>(acc, g, s_tab, rb, bu * N + bb);
-                }
-            }
-            // u in [1+c, 1+2c): (b, c) -> id, row, Tid, Tcol
-            for (int u = 1 + c; u < 1 + 2 * c; ++u) {
-                const int bu = u - 1 - c;
-                if (valid) {
-                    load_g<4>(g, Gp + u * ustride);
-                    const int bb = N - 1 - bu;
-                    mac_use<4, 0>(acc, g, s_tab, rb, bu * N + c);
-                    mac_use<4, 2>(acc, g, s_tab, rb, bb * N + c);
-                    mac_use<4, 4>(acc, g, s_tab, rb, c * N + bu);
-                    mac_use<4, 7>(acc, g, s_tab, rb, c * N + bb);
-                }
-            }
-            // u in [1+2c, nu): (b, a), b < a < c -> all eight uses
-            for (int u = 1 + 2 * c; u < nu; ++u) {
-                const int bu = a.u_bu[u], au = a.u_au[u];
-                if (valid) {
-                    load_g<4>(g, Gp + u * ustride);
-                    const int bb = N - 1 - bu, ab = N - 1 - au;
-                    mac_use<4, 0>(acc, g, s_tab, rb, bu * N + au);
-                    mac_use<4, 1>(acc, g, s_tab, rb, bb * N + ab);
-                    mac_use<4, 2>(acc, g, s_tab, rb, bb * N + au);
-                    mac_use<4, 3>(acc, g, s_tab, rb, bu * N + ab);
-                    mac_use<4, 4>(acc, g, s_tab, rb, au * N + bu);
-                    mac_use<4, 5>(acc, g, s_tab, rb, ab * N + bb);
-                    mac_use<4, 6>(acc, g, s_tab, rb, ab * N + bu);
-                    mac_use<4, 7>(acc, g, s_tab, rb, au * N + bb);
-                }
-            }
-        } else {
-            // Non-square grids: 64 stored spectra (transposes stored), pairs.
-            if (valid) {
-                load_g<2>(g, Gp);
-                mac_use<2, 0>(acc, g, s_tab, rb, c * N + c);
-            }
-            // (b, c): id, row
-            for (int u = 1; u < 1 + c; ++u) {
-                const int bu = u - 1;
-                if (valid) {
-                    load_g<2>(g, Gp + u * ustride);
-                    mac_use<2, 0>(acc, g, s_tab, rb, bu * N + c);
-                    mac_use<2, 2>(acc, g, s_tab, rb, (N - 1 - bu) * N + c);
-                }
-            }
-            // (c, a): id, col
-            for (int u = 1 + c; u < 1 + 2 * c; ++u) {
-                const int au = u - 1 - c;
-                if (valid) {
-                    load_g<2>(g, Gp + u * ustride);
-                    mac_use<2, 0>(acc, g, s_tab, rb, c * N + au);
-                    mac_use<2, 3>(acc, g, s_tab, rb, c * N + (N - 1 - au));
-                }
-            }
-            // (b, a), b, a < c: id, both, row, col
-            for (int u = 1 + 2 * c; u < nu; ++u) {
-                const int bu = a.u_bu[u], au = a.u_au[u];
-                if (valid) {
-                    load_g<2>(g, Gp + u * ustride);
-                    const int bb = N - 1 - bu, ab = N - 1 - au;
-                    mac_use<2, 0>(acc, g, s_tab, rb, bu * N + au);
-                    mac_use<2, 1>(acc, g, s_tab, rb, bb * N + ab);
-                    mac_use<2, 2>(acc, g, s_tab, rb, bb * N + au);
-                    mac_use<2, 3>(acc, g, s_tab, rb, bu * N + ab);
-                }
-            }
+        const float2* Gp = a.G + item_base * SLOTS * nu * a.nP + ((long long)p * nu) * ustride +
+                           (long long)tsafe * IPT * SLOTS;
+        int u = 0;
+        for (; u + UNR <= nu; u += UNR) {
+            float2 g[UNR][IPT][4];
+#pragma unroll
+            for (int k = 0; k < UNR; ++k)
+#pragma unroll
+                for (int q = 0; q < IPT; ++q) load_g<SLOTS>(g[k][q], Gp + (u + k) * ustride + q * SLOTS);
+#pragma unroll
+            for (int k = 0; k < UNR; ++k) mac_spectrum<SLOTS, IPT>(acc, g[k], pm_tab, rb, u + k);
+        }
+        for (; u < nu; ++u) {
+            float2 g[IPT][4];
+#pragma unroll
+            for (int q = 0; q < IPT; ++q) load_g<SLOTS>(g[q], Gp + u * ustride + q * SLOTS);
+            mac_spectrum<SLOTS, IPT>(acc, g, pm_tab, rb, u);
         }
 
-        // ---- epilogue: class constants and the 'same'-crop shift ----
+        // ---- epilogue: outer phase and the 'same'-crop shift ----
         const int sig = s2 - 2 + N;
         const int c2 = (s2 - 1) / 2;
 #pragma unroll
-        for (int j = 0; j < SLOTS; ++j) {
-            const int m2 = it.m2[j], n2 = it.n2[j];
-            float2 v = acc[j][0];
-            v = cadd(v, cmul(phase_frac((long long)sig * m2, twoN), acc[j][1]));
-            v = cadd(v, cmul(phase_frac((long long)sig * n2, twoN), acc[j][2]));
-            v = cadd(v, cmul(phase_frac((long long)sig * (m2 + n2), twoN), acc[j][3]));
-            // exp(+2 pi i c2 (k0/fY + k1/fX))
-            long long e0 = ((long long)c2 * k0s[j]) % a.fY;
-            long long e1 = ((long long)c2 * k1s[j]) % a.fX;
-            float fr = (float)e0 / (float)a.fY + (float)e1 / (float)a.fX;
-            fr -= rintf(fr);
-            float sn, cs;
-            sincospif(2.0f * fr, &sn, &cs);
-            tot[j] = cadd(tot[j], cmul(v, make_float2(cs, sn)));
+        for (int q = 0; q < IPT; ++q) {
+            const int m2 = it[q].m2[0], n2 = it[q].n2[0];
+            int outer[4];
+            outer[0] = sig * (m2 + n2);
+            outer[1] = sig * (n2 - m2);
+            outer[2] = sig * (n2 + m2);
+            outer[3] = sig * (m2 - n2);
+#pragma unroll
+            for (int j = 0; j < SLOTS; ++j) {
+                const int k0 = it[q].pos[j] / a.rfX, k1 = it[q].pos[j] - k0 * a.rfX;
+                float2 v = cmul(acc[q][j], phase_frac(outer[j], fourN));
+                // exp(+2 pi i c2 (k0/fY + k1/fX))
+                const int e0 = (int)(((long long)c2 * k0) % a.fY);
+                const int e1 = (int)(((long long)c2 * k1) % a.fX);
+                float fr = (float)e0 / (float)a.fY + (float)e1 / (float)a.fX;
+                fr -= rintf(fr);
+                float sn, cs;
+                sincospif(2.0f * fr, &sn, &cs);
+                tot[q][j] = cadd(tot[q][j], cmul(v, make_float2(cs, sn)));
+            }
         }
     }
 
     if (valid) {
         float2* A = a.A + (long long)t * a.A_tstride + (a.plane_loop ? 0 : (long long)blockIdx.y * a.A_pstride);
 #pragma unroll
-        for (int j = 0; j < SLOTS; ++j)
-            if ((it.writemask >> j) & 1) A[it.pos[j]] = tot[j];
+        for (int q = 0; q < IPT; ++q)
+#pragma unroll
+            for (int j = 0; j < SLOTS; ++j)
+                if ((it[q].writemask >> j) & 1) A[it[q].pos[j]] = tot[q][j];
     }
 }
 
 // ---------------------------------------------------------------------------
 // Inverse transform, stage 1: column inverse FFTs (length fY) of the half
-// plane, keeping output rows y < Yout.  grid (ceil(rfX/CB), nbatch).
+// plane, keeping output rows y < Yout.  grid (ceil(rfX/CB), nbatch), one
+// wave per column (blockDim = 64*CB); LDS CB*fY + fY (twiddles).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(FFT_THREADS) void k_ifft_cols(IfftColArgs a) {
-    extern __shared__ float2 sh[];
-    const int fY = a.rpY.L, rfX = a.rfX;
-    const int c0 = blockIdx.x * a.CB;
-    const int ncol = min(a.CB, rfX - c0);
+__global__ __launch_bounds__(512) void k_ifft_cols(IfftColArgs a) {
+    extern __shared__ float2 lds[];
+    const int fY = a.rpY.L, rfX = a.rfX, CB = a.CB;
+    float2* C = lds;                 // [CB][fY]
+    float2* tw = C + CB * fY;        // [fY]
+    const int c0 = blockIdx.x * CB;
+    const int ncol = min(CB, rfX - c0);
     const float2* in = a.A + (long long)blockIdx.y * a.A_bstride;
-    for (int i = threadIdx.x; i < fY * a.CB; i += blockDim.x) {
-        int y = i / a.CB, cc = i - y * a.CB;
-        float2 v = make_float2(0.f, 0.f);
-        if (cc < ncol) v = in[(long long)y * rfX + c0 + cc];
-        sh[cc * fY + y] = v;
+    for (int i = threadIdx.x; i < fY; i += blockDim.x) tw[i] = a.twY[i];
+    for (int i = threadIdx.x; i < fY * CB; i += blockDim.x) {
+        const int y = i / CB, cc = i - y * CB;
+        C[cc * fY + y] = cc < ncol ? in[(long long)y * rfX + c0 + cc] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    lds_fft<FFT_MAXB>(sh, a.rpY, a.CB, 1, fY, +1.f);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (wv < ncol) wave_fft<2048>(C + wv * fY, 1, a.rpY, tw, true, lane);
+    __syncthreads();
     float2* out = a.out + (long long)blockIdx.y * a.out_bstride;
-    for (int i = threadIdx.x; i < a.Yout * a.CB; i += blockDim.x) {
-        int y = i / a.CB, cc = i - y * a.CB;
-        if (cc < ncol) out[(long long)y * rfX + c0 + cc] = sh[cc * fY + y];
+    for (int i = threadIdx.x; i < a.Yout * CB; i += blockDim.x) {
+        const int y = i / CB, cc = i - y * CB;
+        if (cc < ncol) out[(long long)y * rfX + c0 + cc] = C[cc * fY + y];
     }
 }
 
-// c2r of one row held in LDS: Hermitian-extend the rfX inputs to fX, inverse
-// complex FFT; the real part is the c2r result (imaginary parts of the
-// self-conjugate bins drop out, as in a c2r transform).
-__device__ void row_c2r(float2* sh, const float2* row, int fX, int rfX, const RadixPlan& rp) {
-    for (int i = threadIdx.x; i < fX; i += blockDim.x) {
-        float2 v;
-        if (i < rfX) v = row[i];
-        else v = cconj(row[fX - i]);
-        sh[i] = v;
+// c2r of one row by one wave.  row: rfX = L/2+1 half-spectrum values (global);
+// buf: LDS work area (L/2 entries for even L, L for odd L).  For even L the
+// length-L/2 complex transform of Z[k] = E[k] + i O[k],
+//   E = X[k] + conj(X[L/2-k]),  O = (X[k] - conj(X[L/2-k])) exp(+2 pi i k/L),
+// gives x[2m] = Re z[m], x[2m+1] = Im z[m]; odd L uses the Hermitian
+// extension and a full-length transform.  The imaginary parts of X[0] and
+// X[L/2] are dropped, as a c2r transform does.  Returns with the result in
+// buf (interleaved real samples for even L, real parts for odd L).
+__device__ __forceinline__ void wave_c2r(float2* buf, const float2* __restrict__ row, int L,
+                                         const RadixPlan& rph, const float2* twh,
+                                         const RadixPlan& rpf, const float2* twf, int lane) {
+    if ((L & 1) == 0) {
+        const int L2 = L / 2;
+        for (int k = lane; k < L2; k += 64) {
+            float2 x = row[k];
+            float2 xr = cconj(row[L2 - k]);
+            if (k == 0) { x.y = 0.f; xr.y = 0.f; }
+            const float2 E = cadd(x, xr);
+            const float2 d = csub(x, xr);
+            const float2 w = cconj(twf[k]);                  // exp(+2 pi i k / L)
+            const float2 O = cmul(d, w);
+            buf[k] = make_float2(E.x - O.y, E.y + O.x);      // E + i O
+        }
+        wave_sync();
+        wave_fft<1024>(buf, 1, rph, twh, true, lane);
+    } else {
+        const int rfX = L / 2 + 1;
+        for (int k = lane; k < L; k += 64) {
+            float2 v = k < rfX ? row[k] : cconj(row[L - k]);
+            if (k == 0) v.y = 0.f;
+            buf[k] = v;
+        }
+        wave_sync();
+        wave_fft<2048>(buf, 1, rpf, twf, true, lane);
     }
-    __syncthreads();
-    lds_fft<FFT_MAXB>(sh, rp, 1, 1, fX, +1.f);
+}
+
+__device__ __forceinline__ float c2r_sample(const float2* buf, int L, int x) {
+    if ((L & 1) == 0) {
+        const float2 v = buf[x >> 1];
+        return (x & 1) ? v.y : v.x;
+    }
+    return buf[x].x;
 }
 
 // Forward projection, stage 2: per output row, c2r each grid group's row and
-// sum the groups in a fixed order.  grid (Y, T).
-__global__ __launch_bounds__(FFT_THREADS) void k_ifft_rows_sum(IfftRowSumArgs a) {
-    extern __shared__ float2 sh[];
-    float* accum = reinterpret_cast<float*>(sh + a.maxfX);
-    const int y = blockIdx.x, t = blockIdx.y;
-    for (int x = threadIdx.x; x < a.X; x += blockDim.x) accum[x] = 0.f;
+// sum the groups in a fixed order.  One wave per (row, frame);
+// grid (ceil(Y/4), T), block 256.
+__global__ __launch_bounds__(256) void k_ifft_rows_sum(IfftRowSumArgs a) {
+    extern __shared__ float2 lds[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int y = blockIdx.x * 4 + wv, t = blockIdx.y;
+    float2* twf = lds;                          // [maxfX]
+    float2* twh = twf + a.maxfX;                // [maxfX/2]
+    float2* buf = twh + a.maxfX / 2 + 1 + wv * a.maxfX;
+    constexpr int XPL = 2048 / 64;              // output samples per lane (X <= 2048)
+    float acc[XPL];
+#pragma unroll
+    for (int q = 0; q < XPL; ++q) acc[q] = 0.f;
     for (int g = 0; g < a.ngroups; ++g) {
         const IfftRowGroup& G = a.grp[g];
         if (!G.active) continue;
+        const int L = G.rp.L;
         __syncthreads();
-        const float2* row = G.T + (long long)t * G.tstride + (long long)y * G.rfX;
-        row_c2r(sh, row, G.rp.L, G.rfX, G.rp);
-        const float sc = G.scale;
-        for (int x = threadIdx.x; x < a.X; x += blockDim.x) accum[x] += sh[x].x * sc;
+        for (int i = threadIdx.x; i < L; i += blockDim.x) twf[i] = G.twf[i];
+        if ((L & 1) == 0)
+            for (int i = threadIdx.x; i < L / 2; i += blockDim.x) twh[i] = G.twh[i];
+        __syncthreads();
+        if (y < a.Y) {
+            const float2* row = G.T + (long long)t * G.tstride + (long long)y * G.rfX;
+            wave_c2r(buf, row, L, G.rph, twh, G.rp, twf, lane);
+            const float sc = G.scale;
+#pragma unroll
+            for (int q = 0; q < XPL; ++q) {
+                const int x = lane + 64 * q;
+                if (x < a.X) acc[q] += c2r_sample(buf, L, x) * sc;
+            }
+            wave_sync();
+        }
     }
-    __syncthreads();
-    float* out = a.img + (long long)t * a.img_tstride + (long long)y * a.X;
-    for (int x = threadIdx.x; x < a.X; x += blockDim.x) out[x] = accum[x];
+    if (y < a.Y) {
+        float* out = a.img + (long long)t * a.img_tstride + (long long)y * a.X;
+#pragma unroll
+        for (int q = 0; q < XPL; ++q) {
+            const int x = lane + 64 * q;
+            if (x < a.X) out[x] = acc[q];
+        }
+    }
 }
 
 // Backward projection, stage 2: per (row, frame, plane) c2r; then either store
 // the plane, or apply the Richardson-Lucy update in place
 // (lfdeconv.py:112-126 Prevedel: X *= Htf / v;  lfdeconv.py:151-164 standard:
-// X *= v [/ trust]; NaN -> 0 after the update).  grid (Y, T, nplanes).
-__global__ __launch_bounds__(FFT_THREADS) void k_ifft_rows_plane(IfftRowPlaneArgs a) {
-    extern __shared__ float2 sh[];
-    const int y = blockIdx.x, t = blockIdx.y;
+// X *= v [/ trust]; NaN -> 0 after the update).  One wave per row;
+// grid (ceil(Y/4), T, nplanes), block 256.  Twiddles of every grid group are
+// read from global memory (they stay in the caches).
+__global__ __launch_bounds__(256) void k_ifft_rows_plane(IfftRowPlaneArgs a) {
+    extern __shared__ float2 lds[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int y = blockIdx.x * 4 + wv, t = blockIdx.y;
     const int z = a.plane_ids[blockIdx.z];
-    const RowPlaneDesc pd = a.pd[z];
+    const RowPlaneDesc& pd = a.pd[z];           // read in place (no private copy)
+    const int L = pd.rp.L;
+    float2* buf = lds + wv * a.maxfX;
+    if (y >= a.Y) return;                       // no block-level barrier below
     const float2* row = a.Tbuf + pd.T_off + (long long)t * pd.T_tstride + (long long)y * pd.rfX;
-    row_c2r(sh, row, pd.rp.L, pd.rfX, pd.rp);
+    wave_c2r(buf, row, L, pd.rph, pd.twh, pd.rp, pd.twf, lane);
     const float sc = pd.scale;
     const long long o = (((long long)z * a.T + t) * a.Y + y) * a.X;
-    for (int x = threadIdx.x; x < a.X; x += blockDim.x) {
-        float v = sh[x].x * sc;
+    for (int x = lane; x < a.X; x += 64) {
+        const float v = c2r_sample(buf, L, x) * sc;
         if (a.mode == 0) {
             a.vol[o + x] = v;
         } else if (a.mode == 1) {
-            float r = a.Htf[o + x] / v;
-            float xn = a.vol[o + x] * r;
+            const float r = a.Htf[o + x] / v;
+            const float xn = a.vol[o + x] * r;
             a.vol[o + x] = __builtin_isnan(xn) ? 0.f : xn;
         } else {
             float xn = a.vol[o + x] * v;
@@ -536,9 +621,12 @@ __global__ void k_fill(float* p, long long n, float v) {
 // ---------------------------------------------------------------------------
 namespace flf {
 
+size_t small_fft_lds(int rY, int rX) {
+    return ((size_t)rY * (rX / 2 + 1) + rY + rX + 4 * rX) * sizeof(float2);
+}
 hipError_t launch_small_fft(const SmallFFTArgs& a, int nsrc, int T, hipStream_t s) {
-    size_t lds = (size_t)a.rpY.L * a.rpX.L * sizeof(float2);
-    hipLaunchKernelGGL(k_small_fft, dim3(a.N * a.N, T, nsrc), dim3(FFT_THREADS), lds, s, a);
+    size_t lds = small_fft_lds(a.rpY.L, a.rpX.L);
+    hipLaunchKernelGGL(k_small_fft, dim3(a.N * a.N, T, nsrc), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 hipError_t launch_psf_rows(const PsfRowArgs& a, int nu, hipStream_t s) {
@@ -555,26 +643,38 @@ hipError_t launch_spec_pack(const SpecPackArgs& a, int nchunks, hipStream_t s) {
     hipLaunchKernelGGL(k_spec_pack, dim3(nchunks, a.nu), dim3(FLF_CHUNK_MAX), 0, s, a);
     return hipGetLastError();
 }
-hipError_t launch_mac(const MacArgs& a, int slots, int nchunks, int nplanes_grid, int T, int max_res, hipStream_t s) {
-    size_t lds = (size_t)max_res * a.N * a.N * sizeof(float2);
+hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int nchunks, int nplanes_grid, int T, int max_res,
+                      hipStream_t s) {
+    size_t lds = (size_t)max_res * a.nu * (slots == 4 ? 4 : 2) * sizeof(float4);
     dim3 grid(nchunks, nplanes_grid, T);
-    if (slots == 4) hipLaunchKernelGGL(k_mac<4>, grid, dim3(FLF_CHUNK_MAX), lds, s, a);
-    else            hipLaunchKernelGGL(k_mac<2>, grid, dim3(FLF_CHUNK_MAX), lds, s, a);
+    dim3 block(FLF_CHUNK_MAX / ipt);
+    if (slots == 4) {
+        if (ipt == 1) hipLaunchKernelGGL((k_mac<4, 1>), grid, block, lds, s, a);
+        else if (ipt == 2) hipLaunchKernelGGL((k_mac<4, 2>), grid, block, lds, s, a);
+        else hipLaunchKernelGGL((k_mac<4, 4>), grid, block, lds, s, a);
+    } else {
+        if (ipt == 1) hipLaunchKernelGGL((k_mac<2, 1>), grid, block, lds, s, a);
+        else if (ipt == 2) hipLaunchKernelGGL((k_mac<2, 2>), grid, block, lds, s, a);
+        else hipLaunchKernelGGL((k_mac<2, 4>), grid, block, lds, s, a);
+    }
     return hipGetLastError();
 }
+size_t ifft_cols_lds(int CB, int fY) { return ((size_t)CB * fY + fY) * sizeof(float2); }
 hipError_t launch_ifft_cols(const IfftColArgs& a, int nbatch, hipStream_t s) {
-    size_t lds = (size_t)a.CB * a.rpY.L * sizeof(float2);
-    hipLaunchKernelGGL(k_ifft_cols, dim3((a.rfX + a.CB - 1) / a.CB, nbatch), dim3(FFT_THREADS), lds, s, a);
+    size_t lds = ifft_cols_lds(a.CB, a.rpY.L);
+    hipLaunchKernelGGL(k_ifft_cols, dim3((a.rfX + a.CB - 1) / a.CB, nbatch), dim3(64 * a.CB), lds, s, a);
     return hipGetLastError();
 }
+size_t ifft_rows_sum_lds(int maxfX) { return ((size_t)maxfX + maxfX / 2 + 1 + 4 * (size_t)maxfX) * sizeof(float2); }
 hipError_t launch_ifft_rows_sum(const IfftRowSumArgs& a, int Y, int T, hipStream_t s) {
-    size_t lds = (size_t)a.maxfX * sizeof(float2) + (size_t)a.X * sizeof(float);
-    hipLaunchKernelGGL(k_ifft_rows_sum, dim3(Y, T), dim3(FFT_THREADS), lds, s, a);
+    size_t lds = ifft_rows_sum_lds(a.maxfX);
+    hipLaunchKernelGGL(k_ifft_rows_sum, dim3((Y + 3) / 4, T), dim3(256), lds, s, a);
     return hipGetLastError();
 }
+size_t ifft_rows_plane_lds(int maxfX) { return 4 * (size_t)maxfX * sizeof(float2); }
 hipError_t launch_ifft_rows_plane(const IfftRowPlaneArgs& a, int nplanes, int maxfX, hipStream_t s) {
-    size_t lds = (size_t)maxfX * sizeof(float2);
-    hipLaunchKernelGGL(k_ifft_rows_plane, dim3(a.Y, a.T, nplanes), dim3(FFT_THREADS), lds, s, a);
+    size_t lds = ifft_rows_plane_lds(maxfX);
+    hipLaunchKernelGGL(k_ifft_rows_plane, dim3((a.Y + 3) / 4, a.T, nplanes), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 hipError_t launch_fit(const float* img, const float* im, long long n_per_frame, int T, double* part, int nblk,
@@ -586,7 +686,8 @@ hipError_t launch_fit(const float* img, const float* im, long long n_per_frame, 
 hipError_t configure_kernels(size_t max_lds) {
     const void* ks[] = {(const void*)k_small_fft, (const void*)k_psf_rows, (const void*)k_psf_cols,
                         (const void*)k_ifft_cols, (const void*)k_ifft_rows_sum, (const void*)k_ifft_rows_plane,
-                        (const void*)k_mac<4>, (const void*)k_mac<2>};
+                        (const void*)k_mac<4, 1>, (const void*)k_mac<4, 2>, (const void*)k_mac<4, 4>,
+                        (const void*)k_mac<2, 1>, (const void*)k_mac<2, 2>, (const void*)k_mac<2, 4>};
     for (const void* k : ks) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
         if (e != hipSuccess) return e;

@@ -75,10 +75,12 @@ static int alias2(int x, int r, int N) {
     return (2 * m + 1) % (2 * N);
 }
 
-std::string build_grid_tables(int N, int fY, int fX, GridTables& gt) {
+std::string build_grid_tables(int N, int fY, int fX, GridTables& gt, int items_per_thread) {
     if (N % 2 == 0 || N < 3 || N > FLF_MAX_N) return "Nnum must be odd and in [3, 31]";
     if (fY % N || fX % N) return "Fourier grid must be a multiple of Nnum";
+    if (items_per_thread < 1 || items_per_thread > 4) return "items per thread must be 1..4";
     gt = GridTables();
+    gt.ipt = items_per_thread;
     gt.fY = fY;
     gt.fX = fX;
     gt.rY = fY / N;
@@ -103,6 +105,30 @@ std::string build_grid_tables(int N, int fY, int fX, GridTables& gt) {
     }
     gt.nu = (int)gt.u_bu.size();
     if (gt.nu != (gt.square ? h * (h + 1) / 2 : h * h)) return "internal: spectrum count";
+
+    // Uses of each stored spectrum, in the order of the kernel's use types:
+    // identity, both flips, row flip (N-1-b), column flip (N-1-a), then the
+    // same four for the transposed PSF (py_light_field.cpp:1514-1560).
+    const int NN = N * N;
+    gt.nuse = gt.square ? 8 : 4;
+    int present = 0;
+    for (int u = 0; u < gt.nu; ++u) {
+        const int bu = gt.u_bu[u], au = gt.u_au[u], bb = N - 1 - bu, ab = N - 1 - au;
+        const bool mx = bu != c, my = au != c, tr = gt.square && au != bu;
+        int o[8] = {bu * N + au,
+                    (mx && my) ? bb * N + ab : NN,
+                    mx ? bb * N + au : NN,
+                    my ? bu * N + ab : NN,
+                    tr ? au * N + bu : NN,
+                    (tr && my && mx) ? ab * N + bb : NN,
+                    (tr && my) ? ab * N + bu : NN,
+                    (tr && mx) ? au * N + bb : NN};
+        for (int q = 0; q < gt.nuse; ++q) {
+            gt.uoff.push_back(o[q]);
+            present += o[q] < NN;
+        }
+    }
+    if (present != NN) return "internal: uses do not cover every offset once";
 
     const int rY = gt.rY, rX = gt.rX, rfX = gt.rfX;
     auto canon = [&](int k0, int k1) -> long long {
@@ -179,10 +205,38 @@ std::string build_grid_tables(int N, int fY, int fX, GridTables& gt) {
             its.push_back(it);
         }
         gt.max_res = std::max(gt.max_res, (int)res.size());
-        for (size_t s = 0; s < its.size(); s += FLF_CHUNK_MAX) {
+        gt.real_items += (long long)its.size();
+        // Order by residue tuple so that runs of ipt items share their
+        // residues; pad every class to a multiple of ipt with non-writing copies.
+        const int ipt = gt.ipt;
+        if (ipt > 1) {
+            std::stable_sort(its.begin(), its.end(), [](const ItemDesc& x, const ItemDesc& y) {
+                for (int j = 0; j < 4; ++j)
+                    if (x.res[j] != y.res[j]) return x.res[j] < y.res[j];
+                return false;
+            });
+            std::vector<ItemDesc> padded;
+            size_t i0 = 0;
+            while (i0 < its.size()) {
+                size_t i1 = i0;
+                while (i1 < its.size() && std::equal(its[i1].res, its[i1].res + 4, its[i0].res)) ++i1;
+                for (size_t q = i0; q < i1; ++q) padded.push_back(its[q]);
+                size_t n = i1 - i0;
+                while (n % ipt) {
+                    ItemDesc d = its[i0];
+                    d.writemask = 0;
+                    padded.push_back(d);
+                    ++n;
+                }
+                i0 = i1;
+            }
+            its.swap(padded);
+        }
+        const size_t cmax = (size_t)(FLF_CHUNK_MAX / ipt) * ipt;
+        for (size_t s = 0; s < its.size(); s += cmax) {
             ChunkDesc ch{};
             ch.item_base = (long long)gt.items.size();
-            ch.n_items = (int)std::min<size_t>(FLF_CHUNK_MAX, its.size() - s);
+            ch.n_items = (int)std::min<size_t>(cmax, its.size() - s);
             ch.n_res = (int)res.size();
             for (int q = 0; q < FLF_MAX_RES; ++q) {
                 ch.res0[q] = q < ch.n_res ? res[q].first : 0;

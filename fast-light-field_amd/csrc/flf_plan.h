@@ -25,12 +25,20 @@ struct GridTables {
     int slots = 0;               // 4 (square) or 2
     int nu = 0;                  // stored spectra per plane: h(h+1)/2 or h*h
     int max_res = 0;             // largest residue list of any chunk
+    int ipt = 1;                 // items per MAC thread
+    long long real_items = 0;    // items excluding padding copies
     std::vector<int> u_bu, u_au; // stored spectrum u -> quadrant (b, a) (b > a: transposed)
+    std::vector<int> uoff;       // [nu][uses]: offset b*N+a of each use of u, N*N if absent
+    int nuse = 0;                // uses per stored spectrum: 8 (quads) or 4 (pairs)
     std::vector<ChunkDesc> chunks;
     std::vector<ItemDesc> items;
 };
 
 // Builds the tables; returns an empty string or an error message.
-std::string build_grid_tables(int N, int fY, int fX, GridTables& gt);
+// items_per_thread > 1 groups the items of each chunk so that every run of
+// items_per_thread consecutive items shares its residues (one MAC thread
+// handles such a run); classes whose size is not a multiple are padded with
+// copies that never write.
+std::string build_grid_tables(int N, int fY, int fX, GridTables& gt, int items_per_thread = 1);
 
 }  // namespace flf

@@ -1,0 +1,83 @@
+// Wave-level mixed-radix (2/3/4/5) FFTs in LDS.
+//
+// One 64-lane wave transforms one sequence held in LDS (element stride es).
+// Stockham passes as in flf_fft.h, but each pass is executed by the wave
+// alone: all lanes read their butterflies' inputs into registers, the wave
+// synchronises (LDS operations of one wave complete in order; the barrier
+// stops the compiler from moving accesses across it), then writes the
+// outputs in place.  Twiddles come from a table tw[m] = exp(-2 pi i m / L)
+// (computed on the host in double precision), so no transcendental is
+// evaluated per butterfly.
+#pragma once
+#include "flf_common.h"
+#include "flf_fft.h"
+
+namespace flf {
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int R, int LMAX>
+__device__ __forceinline__ void wave_pass(float2* buf, int es, int L, int Ns, const float2* __restrict__ tw,
+                                          bool inv, int lane) {
+    constexpr int MAXQ = (LMAX / R + 63) / 64;
+    const int M = L / R;
+    const int tstep = L / (Ns * R);
+    const float sign = inv ? 1.f : -1.f;
+    float2 v[MAXQ][R];
+#pragma unroll
+    for (int q = 0; q < MAXQ; ++q) {
+        const int j = lane + 64 * q;
+        if (j < M) {
+            const int k = j % Ns;
+            float2 a[5];
+#pragma unroll
+            for (int t = 0; t < R; ++t) a[t] = buf[(j + t * M) * es];
+            if (k != 0) {
+#pragma unroll
+                for (int t = 1; t < R; ++t) {
+                    float2 w = tw[t * k * tstep];
+                    if (inv) w.y = -w.y;
+                    a[t] = cmul(a[t], w);
+                }
+            }
+            dft_small<R>(a, sign);
+#pragma unroll
+            for (int t = 0; t < R; ++t) v[q][t] = a[t];
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < MAXQ; ++q) {
+        const int j = lane + 64 * q;
+        if (j < M) {
+            const int k = j % Ns;
+            const int o = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int t = 0; t < R; ++t) buf[(o + t * Ns) * es] = v[q][t];
+        }
+    }
+    wave_sync();
+}
+
+// In-place transform of one sequence by one wave (inv: unnormalised inverse).
+template <int LMAX>
+__device__ void wave_fft(float2* buf, int es, const RadixPlan& rp, const float2* __restrict__ tw, bool inv,
+                         int lane) {
+    int Ns = 1;
+    for (int p = 0; p < rp.n; ++p) {
+        const int R = rp.r[p];
+        switch (R) {
+            case 4: wave_pass<4, LMAX>(buf, es, rp.L, Ns, tw, inv, lane); break;
+            case 2: wave_pass<2, LMAX>(buf, es, rp.L, Ns, tw, inv, lane); break;
+            case 3: wave_pass<3, LMAX>(buf, es, rp.L, Ns, tw, inv, lane); break;
+            default: wave_pass<5, LMAX>(buf, es, rp.L, Ns, tw, inv, lane); break;
+        }
+        Ns *= R;
+    }
+}
+
+}  // namespace flf

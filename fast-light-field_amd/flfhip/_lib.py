@@ -7,7 +7,7 @@ import ctypes
 import os
 
 LIB_NAME = "libflf_hip.so"
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+LIB_PATH = os.environ.get("FLF_HIP_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
 FLF_OK = 0
 FLF_EINVAL = -1
