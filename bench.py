@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="rl1455", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (frames or planes would be "
@@ -126,6 +128,13 @@ def main():
         Xv.fill_(1.0)
         deconv_rl_device(proj, hm, Htf, iters, Xv, im=im)
 
+    if args.calib_copy_mib:
+        n = args.calib_copy_mib * (1 << 20) // 4
+        a_ = torch.ones(n, dtype=torch.float32, device=proj.device)
+        b_ = torch.empty_like(a_)
+        b_.copy_(a_)
+        torch.cuda.synchronize()
+        del a_, b_
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()

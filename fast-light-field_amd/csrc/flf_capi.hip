@@ -222,6 +222,12 @@ int run_mac(flf_ctx* ctx, const MacArgs& m, const Group& g, int nplanes_grid, hi
     return FLF_OK;
 }
 
+// Elements of one frame's small-FFT output: [chunk][N*N][NH], NH = slots
+// (half of each orbit's residue list; the rest follows by conjugation).
+long long s_frame_stride(const flf_ctx* ctx, const Group& g) {
+    return (long long)g.gt.chunks.size() * ctx->N * ctx->N * g.gt.slots;
+}
+
 // Small FFTs of the backward source (the image) for one group.
 int run_small_fft_image(flf_ctx* ctx, Group& g, const float* img, hipStream_t st) {
     SmallFFTArgs a{};
@@ -234,7 +240,9 @@ int run_small_fft_image(flf_ctx* ctx, Group& g, const float* img, hipStream_t st
     a.twY = g.twRY; a.twX = g.twRX;
     a.S = g.d_S_bwd;
     a.S_off = ctx->d_zeros;
-    a.S_tstride = (long long)ctx->N * ctx->N * g.gt.rY * (g.gt.rX / 2 + 1);
+    a.chunks = g.d_chunks; a.nchunks = (int)g.gt.chunks.size(); a.NH = g.gt.slots;
+    a.fY = g.gt.fY; a.fX = g.gt.fX;
+    a.S_tstride = s_frame_stride(ctx, g);
     HIPCHK(launch_small_fft(a, 1, ctx->T, st));
     return FLF_OK;
 }
@@ -260,7 +268,7 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
         m.G = g.d_G[1];
         m.chunks = g.d_chunks; m.items = g.d_items; m.u_bu = g.d_bu; m.u_au = g.d_au; m.uoff = g.d_uoff;
         m.S = g.d_S_bwd; m.S_off = ctx->d_zeros;
-        m.S_tstride = (long long)N * N * gt.rY * (gt.rX / 2 + 1);
+        m.S_tstride = s_frame_stride(ctx, g);
         m.s2 = g.d_s2; m.enable = g.d_enable;
         m.nP = (int)g.planes.size();
         m.plane_loop = 0;
@@ -355,7 +363,6 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
     ctx->s2.assign(s2, s2 + nz);
     ctx->group_of.assign(nz, -1);
     ctx->idx_in_group.assign(nz, -1);
-    const int NN = nnum * nnum;
 
     // Group planes by Fourier grid.
     for (int z = 0; z < nz; ++z) {
@@ -388,7 +395,6 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
         GridTables& gt = g.gt;
         const int nP = (int)g.planes.size();
         maxP = std::max(maxP, nP);
-        const int rXh = gt.rX / 2 + 1;
         // FFT workgroup shapes.
         g.RB = std::min(16, fft_batch_cap(g.rpX, ctx->max_lds));
         g.CBp = std::min(16, fft_batch_cap(g.rpY, ctx->max_lds));
@@ -426,7 +432,7 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
         if ((rc = upload(ctx, &g.d_uoff, gt.uoff))) return rc;
         std::vector<int> s2g;
         std::vector<long long> srcoff, soff;
-        const long long Sper = (long long)T * NN * gt.rY * rXh;
+        const long long Sper = (long long)T * s_frame_stride(ctx, g);
         for (int p = 0; p < nP; ++p) {
             int z = g.planes[p];
             s2g.push_back(s2[z]);
@@ -537,7 +543,7 @@ int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, i
     std::vector<char> sel;
     int rc = resolve_planes(ctx, planes, nplanes, sel);
     if (rc) return rc;
-    const int N = ctx->N, T = ctx->T, NN = N * N;
+    const int N = ctx->N, T = ctx->T;
     IfftRowSumArgs rs{};
     rs.ngroups = (int)ctx->groups.size();
     rs.X = ctx->X;
@@ -563,7 +569,6 @@ int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, i
         if (!any) continue;
         rs.maxfX = std::max(rs.maxfX, gt.fX);
         const int nP = (int)g.planes.size();
-        const int rXh = gt.rX / 2 + 1;
         SmallFFTArgs a{};
         a.src = vol;
         a.src_off = g.d_srcoff_fwd;
@@ -574,7 +579,9 @@ int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, i
         a.twY = g.twRY; a.twX = g.twRX;
         a.S = g.d_S_fwd;
         a.S_off = g.d_Soff_fwd;
-        a.S_tstride = (long long)NN * gt.rY * rXh;
+        a.chunks = g.d_chunks; a.nchunks = (int)gt.chunks.size(); a.NH = gt.slots;
+        a.fY = gt.fY; a.fX = gt.fX;
+        a.S_tstride = s_frame_stride(ctx, g);
         HIPCHK(launch_small_fft(a, nP, T, st));
         MacArgs m{};
         m.G = g.d_G[0];

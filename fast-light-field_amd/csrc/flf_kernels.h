@@ -13,7 +13,10 @@ struct SmallFFTArgs {
     RadixPlan rpY, rpX;          // lengths rY, rX
     const float2* twY;           // exp(-2 pi i m / rY), m < rY
     const float2* twX;           // exp(-2 pi i m / rX)
-    float2* S;
+    const ChunkDesc* chunks;     // MAC chunks of the grid (output order)
+    int nchunks, NH;             // chunks; residues stored per chunk (4 quad / 2 pair)
+    int fY, fX;
+    float2* S;                   // [chunk][N*N][NH] per source plane and frame
     const long long* S_off;      // per source plane z: element offset into S (device array)
     long long S_tstride;         // per frame
 };

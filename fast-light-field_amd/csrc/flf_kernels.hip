@@ -9,6 +9,8 @@
 //                  groups (forward) or written / RL-updated per plane (backward)
 // and once per PSF set: k_psf_rows + k_psf_cols (pruned 2-D FFT of each
 // quadrant PSF) and k_spec_pack (gather into the MAC's orbit-major layout).
+#include <algorithm>
+
 #include "flf_common.h"
 #include "flf_fft.h"
 #include "flf_kernels.h"
@@ -84,9 +86,35 @@ __global__ __launch_bounds__(256) void k_small_fft(SmallFFTArgs a) {
     }
     __syncthreads();
     for (int k = wv; k < rXh; k += 4) wave_fft<128>(H + k, rXh, a.rpY, twY, false, lane);
+    // Sub-sampling phases of this offset: exp(-2 pi i b kt0 / fY) and
+    // exp(-2 pi i a kt1 / fX), kt the centred residue (0 for r/2).
+    float2* phY = scratch;                        // reuse: [rY]
+    float2* phX = scratch + rY;                   // [rX]
     __syncthreads();
-    float2* S = a.S + a.S_off[z] + (long long)t * a.S_tstride + (long long)off * rY * rXh;
-    for (int i = threadIdx.x; i < rY * rXh; i += blockDim.x) S[i] = H[i];
+    for (int i = threadIdx.x; i < rY + rX; i += blockDim.x) {
+        const bool isY = i < rY;
+        const int k = isY ? i : i - rY;
+        const int r = isY ? rY : rX;
+        const int f = isY ? a.fY : a.fX;
+        const int kt = (2 * k < r) ? k : ((2 * k > r) ? k - r : 0);
+        const int m = ((isY ? b : aa) * kt) % f;
+        float sn, cs;
+        sincospif(-2.0f * (float)m / (float)f, &sn, &cs);
+        (isY ? phY : phX)[k] = make_float2(cs, sn);
+    }
+    __syncthreads();
+    // Orbit-ordered, phase-folded output S[chunk][off][j] for the first NH
+    // residues of each chunk's residue list (the MAC reads one contiguous
+    // block per chunk).
+    float2* S = a.S + a.S_off[z] + (long long)t * a.S_tstride;
+    const int NH = a.NH, NN = N * N;
+    for (int i = threadIdx.x; i < a.nchunks * NH; i += blockDim.x) {
+        const int w = i / NH, j = i - w * NH;
+        const int k0 = a.chunks[w].res0[j], k1 = a.chunks[w].res1[j];
+        float2 v = (2 * k1 <= rX) ? H[k0 * rXh + k1] : cconj(H[((rY - k0) % rY) * rXh + (rX - k1)]);
+        v = cmul(v, cmul(phY[k0], phX[k1]));
+        S[((long long)w * NN + off) * NH + j] = v;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -270,20 +298,23 @@ __device__ __forceinline__ void mac_spectrum(float2 (&acc)[IPT][4], const float2
 template <int SLOTS, int IPT>
 __global__ __launch_bounds__(FLF_CHUNK_MAX) __attribute__((amdgpu_waves_per_eu(IPT >= 4 ? 2 : FLF_MAC_WAVES)))
 void k_mac(MacArgs a) {
-    extern __shared__ float4 pm_tab[];
+    extern __shared__ float4 pm_tab[];            // [NRES][nu][NPT], then staging
     constexpr int NUSE = SLOTS == 4 ? 8 : 4;
     constexpr int NPT = SLOTS == 4 ? 4 : 2;
+    constexpr int NRES = SLOTS == 4 ? 8 : 4;      // residues per orbit (canonical list)
+    constexpr int NH = NRES / 2;                  // residues stored by the small-FFT kernel
     constexpr int UNR = IPT >= 4 ? 1 : FLF_MAC_UNR;   // spectra whose loads are issued together
     const ChunkDesc* ch = a.chunks + blockIdx.x;
     const long long item_base = ch->item_base;
     const int n_items = ch->n_items;
-    const int n_res = ch->n_res;
     const int tid = threadIdx.x;
     const bool valid = tid * IPT < n_items;
     const int tsafe = valid ? tid : 0;           // idle lanes read item 0 (in bounds) and never write
     const int N = a.N, NN = N * N, c = N / 2, fourN = 4 * N;
     const int t = blockIdx.z;
     const int nu = a.nu;
+    float2* s_stage = reinterpret_cast<float2*>(pm_tab + NRES * nu * NPT);   // [NN][NH]
+    float2* psi = s_stage + NN * NH;                                        // [NRES][2]
     ItemDesc it[IPT];
 #pragma unroll
     for (int q = 0; q < IPT; ++q) it[q] = a.items[item_base + tsafe * IPT + q];
@@ -299,22 +330,36 @@ void k_mac(MacArgs a) {
 
     const int pb = a.plane_loop ? 0 : blockIdx.y;
     const int pe = a.plane_loop ? a.nP : blockIdx.y + 1;
-    const int rXh = a.rX / 2 + 1;
-    const long long S_offstride = (long long)a.rY * rXh;
     const long long ustride = (long long)n_items * SLOTS;
 
     for (int p = pb; p < pe; ++p) {
         if (!a.enable[p]) continue;
         const int s2 = a.s2[p];
         __syncthreads();
-        // ---- P/M table: n_res residues x nu spectra x NPT pair types ----
-        const float2* Sp = a.S + a.S_off[p] + (long long)t * a.S_tstride;
-        for (int idx = tid; idx < n_res * nu * NPT; idx += blockDim.x) {
+        // ---- stage this chunk's small-FFT block S[off][j] (j < NH) ----
+        const float2* Sp = a.S + a.S_off[p] + (long long)t * a.S_tstride + (long long)blockIdx.x * NN * NH;
+        for (int i = tid; i < NN * NH; i += blockDim.x) s_stage[i] = Sp[i];
+        // flip factors per residue: exp(-2 pi i (s2-1) kt / f)
+        if (tid < 2 * NRES) {
+            const int rho = tid >> 1;
+            const bool isY = (tid & 1) == 0;
+            const int k = isY ? ch->res0[rho] : ch->res1[rho];
+            const int r = isY ? a.rY : a.rX;
+            const int f = isY ? a.fY : a.fX;
+            const int kt = (2 * k < r) ? k : ((2 * k > r) ? k - r : 0);
+            int m = (int)(((long long)(s2 - 1) * kt) % f);
+            if (m < 0) m += f;
+            float sn, cs;
+            sincospif(-2.0f * (float)m / (float)f, &sn, &cs);
+            psi[tid] = make_float2(cs, sn);
+        }
+        __syncthreads();
+        // ---- P/M table: NRES residues x nu spectra x NPT pair types ----
+        for (int idx = tid; idx < NRES * nu * NPT; idx += blockDim.x) {
             const int pt = idx % NPT;
             const int ru = idx / NPT;
             const int u = ru % nu, rho = ru / nu;
-            const int k0 = ch->res0[rho], k1 = ch->res1[rho];
-            const int kt0 = centred_res(k0, a.rY), kt1 = centred_res(k1, a.rX);
+            const int rb0 = rho < NH ? rho : rho - NH;
             float2 sv[2];
 #pragma unroll
             for (int w = 0; w < 2; ++w) {
@@ -322,18 +367,10 @@ void k_mac(MacArgs a) {
                 float2 r = make_float2(0.f, 0.f);
                 if (off < NN) {
                     const int bo = off / N, ao = off - bo * N;
-                    float2 v;
-                    if (2 * k1 <= a.rX) v = Sp[off * S_offstride + (long long)k0 * rXh + k1];
-                    else v = cconj(Sp[off * S_offstride + (long long)((a.rY - k0) % a.rY) * rXh + (a.rX - k1)]);
-                    const int beta = bo <= c ? bo : bo + s2 - 1;
-                    const int alpha = ao <= c ? ao : ao + s2 - 1;
-                    int n0 = (int)(((long long)beta * kt0) % a.fY); if (n0 < 0) n0 += a.fY;
-                    int n1 = (int)(((long long)alpha * kt1) % a.fX); if (n1 < 0) n1 += a.fX;
-                    float fr = (float)n0 / (float)a.fY + (float)n1 / (float)a.fX;
-                    fr -= rintf(fr);
-                    float sn, cs;
-                    sincospif(-2.0f * fr, &sn, &cs);
-                    r = cmul(v, make_float2(cs, sn));
+                    r = s_stage[off * NH + rb0];
+                    if (rho >= NH) r = cconj(r);
+                    if (bo > c) r = cmul(r, psi[2 * rho]);
+                    if (ao > c) r = cmul(r, psi[2 * rho + 1]);
                 }
                 sv[w] = r;
             }
@@ -622,7 +659,7 @@ __global__ void k_fill(float* p, long long n, float v) {
 namespace flf {
 
 size_t small_fft_lds(int rY, int rX) {
-    return ((size_t)rY * (rX / 2 + 1) + rY + rX + 4 * rX) * sizeof(float2);
+    return ((size_t)rY * (rX / 2 + 1) + rY + rX + (size_t)std::max(4 * rX, rY + rX)) * sizeof(float2);
 }
 hipError_t launch_small_fft(const SmallFFTArgs& a, int nsrc, int T, hipStream_t s) {
     size_t lds = small_fft_lds(a.rpY.L, a.rpX.L);
@@ -645,7 +682,10 @@ hipError_t launch_spec_pack(const SpecPackArgs& a, int nchunks, hipStream_t s) {
 }
 hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int nchunks, int nplanes_grid, int T, int max_res,
                       hipStream_t s) {
-    size_t lds = (size_t)max_res * a.nu * (slots == 4 ? 4 : 2) * sizeof(float4);
+    const int nres = slots == 4 ? 8 : 4;
+    size_t lds = (size_t)nres * a.nu * (slots == 4 ? 4 : 2) * sizeof(float4) +
+                 ((size_t)a.N * a.N * (nres / 2) + 2 * nres) * sizeof(float2);
+    (void)max_res;
     dim3 grid(nchunks, nplanes_grid, T);
     dim3 block(FLF_CHUNK_MAX / ipt);
     if (slots == 4) {

@@ -164,8 +164,21 @@ std::string build_grid_tables(int N, int fY, int fX, GridTables& gt, int items_p
     for (size_t key = 0; key < orbit_items.size(); ++key) {
         auto& bases = orbit_items[key];
         if (bases.empty()) continue;
-        // Residue list of the orbit (first appearance order).
+        // Residue list of the orbit in canonical order: the residues of the
+        // first item's slots (its base residue k and the row-negated /
+        // transposed ones), then their negatives.  The small-FFT kernel stores
+        // S at the first half only; the MAC gets the second half by
+        // conjugation (S(-k) = conj S(k) for real input).
         std::vector<std::pair<int, int>> res;
+        {
+            const int k0 = bases[0].first % rY, k1 = bases[0].second % rX;
+            const int n0 = (rY - k0) % rY, n1 = (rX - k1) % rX;
+            if (gt.square) {
+                res = {{k0, k1}, {n0, k1}, {k1, k0}, {n1, k0}, {n0, n1}, {k0, n1}, {n1, n0}, {k1, n0}};
+            } else {
+                res = {{k0, k1}, {n0, k1}, {n0, n1}, {k0, n1}};
+            }
+        }
         std::vector<ItemDesc> its;
         its.reserve(bases.size());
         for (auto& bp : bases) {
@@ -191,13 +204,9 @@ std::string build_grid_tables(int N, int fY, int fX, GridTables& gt, int items_p
                 if (!dup) it.writemask |= (1 << j);
                 std::pair<int, int> rr(K0[j] % rY, K1[j] % rX);
                 int ri = -1;
-                for (size_t q = 0; q < res.size(); ++q)
+                for (size_t q = 0; q < res.size() && ri < 0; ++q)
                     if (res[q] == rr) ri = (int)q;
-                if (ri < 0) {
-                    res.push_back(rr);
-                    ri = (int)res.size() - 1;
-                    if (ri >= FLF_MAX_RES) return "internal: orbit has more than 8 residues";
-                }
+                if (ri < 0) return "internal: slot residue outside its orbit";
                 it.res[j] = (unsigned char)ri;
                 it.m2[j] = (unsigned char)alias2(K0[j], rY, N);
                 it.n2[j] = (unsigned char)alias2(K1[j], rX, N);
