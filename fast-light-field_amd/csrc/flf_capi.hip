@@ -63,7 +63,7 @@ struct flf_ctx {
     double* d_fitpart = nullptr;
     double* d_fitout = nullptr;
     size_t max_lds = 65536;
-    int ipt = 2;                      // items per MAC thread (FLF_ITEMS_PER_THREAD overrides)
+    int ipt = 1;                      // items per MAC thread (FLF_ITEMS_PER_THREAD overrides)
     long long bytes = 0;
     std::vector<void*> allocs;
     // Optional timing of the multiply-accumulate launches (flf_profile_*).

@@ -279,7 +279,7 @@ __device__ __forceinline__ void mac_spectrum(float2 (&acc)[IPT][4], const float2
 #define FLF_MAC_UNR 2
 #endif
 #ifndef FLF_MAC_WAVES
-#define FLF_MAC_WAVES 3
+#define FLF_MAC_WAVES 4
 #endif
 
 // For an item with base alias (m2, n2) (half units) and output slots
@@ -331,12 +331,13 @@ void k_mac(MacArgs a) {
     const int pb = a.plane_loop ? 0 : blockIdx.y;
     const int pe = a.plane_loop ? a.nP : blockIdx.y + 1;
     const long long ustride = (long long)n_items * SLOTS;
+    const int nb = (nu + UNR - 1) / UNR;          // spectrum batches per plane
 
-    for (int p = pb; p < pe; ++p) {
-        if (!a.enable[p]) continue;
+    // Per-plane set-up: stage the chunk's small-FFT block, flip factors and
+    // the P/M table (all threads; contains barriers).
+    auto prologue = [&](int p) {
         const int s2 = a.s2[p];
         __syncthreads();
-        // ---- stage this chunk's small-FFT block S[off][j] (j < NH) ----
         const float2* Sp = a.S + a.S_off[p] + (long long)t * a.S_tstride + (long long)blockIdx.x * NN * NH;
         for (int i = tid; i < NN * NH; i += blockDim.x) s_stage[i] = Sp[i];
         // flip factors per residue: exp(-2 pi i (s2-1) kt / f)
@@ -354,7 +355,6 @@ void k_mac(MacArgs a) {
             psi[tid] = make_float2(cs, sn);
         }
         __syncthreads();
-        // ---- P/M table: NRES residues x nu spectra x NPT pair types ----
         for (int idx = tid; idx < NRES * nu * NPT; idx += blockDim.x) {
             const int pt = idx % NPT;
             const int ru = idx / NPT;
@@ -379,33 +379,26 @@ void k_mac(MacArgs a) {
             pm_tab[idx] = make_float4(P.x, P.y, -d.y, d.x);     // M = i * d
         }
         __syncthreads();
+    };
 
-        float2 acc[IPT][4];
+    auto gbase = [&](int p) {
+        return a.G + item_base * SLOTS * nu * a.nP + ((long long)p * nu) * ustride + (long long)tsafe * IPT * SLOTS;
+    };
+    // Loads of one batch; spectra past nu re-read the last one (in bounds, unused).
+    auto load_batch = [&](float2 (&g)[UNR][IPT][4], int p, int b) {
+        const float2* Gp = gbase(p);
 #pragma unroll
-        for (int q = 0; q < IPT; ++q)
+        for (int k = 0; k < UNR; ++k) {
+            const int u = min(b * UNR + k, nu - 1);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[q][j] = make_float2(0.f, 0.f);
-
-        const float2* Gp = a.G + item_base * SLOTS * nu * a.nP + ((long long)p * nu) * ustride +
-                           (long long)tsafe * IPT * SLOTS;
-        int u = 0;
-        for (; u + UNR <= nu; u += UNR) {
-            float2 g[UNR][IPT][4];
-#pragma unroll
-            for (int k = 0; k < UNR; ++k)
-#pragma unroll
-                for (int q = 0; q < IPT; ++q) load_g<SLOTS>(g[k][q], Gp + (u + k) * ustride + q * SLOTS);
-#pragma unroll
-            for (int k = 0; k < UNR; ++k) mac_spectrum<SLOTS, IPT>(acc, g[k], pm_tab, rb, u + k);
+            for (int q = 0; q < IPT; ++q) load_g<SLOTS>(g[k][q], Gp + u * ustride + q * SLOTS);
         }
-        for (; u < nu; ++u) {
-            float2 g[IPT][4];
-#pragma unroll
-            for (int q = 0; q < IPT; ++q) load_g<SLOTS>(g[q], Gp + u * ustride + q * SLOTS);
-            mac_spectrum<SLOTS, IPT>(acc, g, pm_tab, rb, u);
-        }
+    };
 
-        // ---- epilogue: outer phase and the 'same'-crop shift ----
+    float2 acc[IPT][4];
+    // Epilogue of plane p: outer phase and the 'same'-crop shift.
+    auto epilogue = [&](int p) {
+        const int s2 = a.s2[p];
         const int sig = s2 - 2 + N;
         const int c2 = (s2 - 1) / 2;
 #pragma unroll
@@ -429,6 +422,42 @@ void k_mac(MacArgs a) {
                 sincospif(2.0f * fr, &sn, &cs);
                 tot[q][j] = cadd(tot[q][j], cmul(v, make_float2(cs, sn)));
             }
+        }
+    };
+
+    // Software pipeline over (plane, batch): the loads of the next batch (also
+    // across a plane boundary, i.e. before the next plane's set-up) are in
+    // flight while the current batch is accumulated.
+    int p = pb;
+    while (p < pe && !a.enable[p]) ++p;
+    if (p < pe) {
+        float2 gA[UNR][IPT][4], gB[UNR][IPT][4];
+        load_batch(gA, p, 0);
+        int b = 0;
+        auto step = [&](float2 (&cur)[UNR][IPT][4], float2 (&nxt)[UNR][IPT][4]) -> bool {
+            if (b == 0) {
+                prologue(p);
+#pragma unroll
+                for (int q = 0; q < IPT; ++q)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[q][j] = make_float2(0.f, 0.f);
+            }
+            int pn = p, bn = b + 1;
+            if (bn == nb) {
+                bn = 0;
+                pn = p + 1;
+                while (pn < pe && !a.enable[pn]) ++pn;
+            }
+            if (pn < pe) load_batch(nxt, pn, bn);
+#pragma unroll
+            for (int k = 0; k < UNR; ++k)
+                if (b * UNR + k < nu) mac_spectrum<SLOTS, IPT>(acc, cur[k], pm_tab, rb, b * UNR + k);
+            if (b == nb - 1) epilogue(p);
+            p = pn;
+            b = bn;
+            return pn < pe;
+        };
+        while (step(gA, gB) && step(gB, gA)) {
         }
     }
 
