@@ -1,0 +1,737 @@
+// Learned initialiser: a lenslet-aware convolutional network that maps a
+// rectified light-field image to a first guess of the volume for the
+// Richardson-Lucy loop.  The reference has no counterpart; RL from a uniform
+// start (lfdeconv.py:97-99 with Xguess=None) stays the fallback.
+//
+// Architecture (all 3x3 convolutions, zero padding 1, at lenslet resolution):
+//   x   = pixel-unshuffle(image / mean(image))      N*N channels (padded to 16k)
+//   h0  = relu(conv0(x))                             width channels
+//   h_i = h_{i-1} + conv_b(relu(conv_a(h_{i-1})))    i = 1..nblocks
+//   o   = conv_head(h_n)                             nz channels (padded)
+//   y   = softplus(o) * mean(image)                  per plane, per lenslet
+// The volume guess is y upsampled (nearest) to full resolution.
+//
+// Arithmetic: one BF16 MFMA GEMM kernel (v_mfma_f32_32x32x16_bf16, FP32
+// accumulation) C = A * B^T with both operands K-contiguous, fed by im2col /
+// transposition kernels:
+//   forward  out[p, co]   = sum_k im2col(x)[p, k] * W[co, k]          k = (tap, ci)
+//   dgrad    dx[p, ci]    = sum_k im2col_flip(dy)[p, k] * Wt[ci, k]   k = (tap, co)
+//   wgrad    dW[co, k]    = sum_p dy^T[co, p] * im2col^T(x)[k, p]     (split over p)
+// Master weights and Adam state are FP32; activations and GEMM operands BF16.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/flf_hip.h"
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+namespace {
+
+__device__ __forceinline__ bf16 to_bf16(float x) { return (bf16)x; }
+__device__ __forceinline__ float from_bf16(bf16 x) { return (float)x; }
+
+// ---------------------------------------------------------------------------
+// C[M x N] (+)= A[M x K] * B[N x K]^T, bf16 operands, fp32 accumulation.
+// Wave tile 32 x 64 (two 32x32x16 MFMAs sharing the A fragment); workgroup
+// 4 waves = 64 x 128.  Operand layout per lane l (32x32x16 bf16):
+//   A[i = l % 32][k = 8 (l / 32) + e]     B[k = 8 (l / 32) + e][j = l % 32]
+//   D[i = 8 (r / 4) + 4 (l / 32) + r % 4][j = l % 32]   r = 0..15
+// K must be a multiple of 16.  grid (ceil(N/128), ceil(M/64), splitK).
+// ---------------------------------------------------------------------------
+struct GemmArgs {
+    const bf16* A; long long lda;
+    const bf16* B; long long ldb;
+    int M, N, K;
+    int kchunk;                 // K per split (multiple of 16)
+    // epilogue
+    const float* bias;          // [N] or null
+    const bf16* resid;          // [M][ldr] or null (added before the store)
+    long long ldr;
+    int relu;
+    bf16* Cb; long long ldcb;   // bf16 output or null
+    float* Cf; long long ldcf;  // fp32 output or null (split z writes at Cf + z * M * ldcf)
+};
+
+__device__ __forceinline__ bf16x8 load8(const bf16* p, bool ok) {
+    if (ok) return *reinterpret_cast<const bf16x8*>(p);
+    bf16x8 z;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) z[e] = (bf16)0.0f;
+    return z;
+}
+
+__global__ __launch_bounds__(256) void k_gemm_nt(GemmArgs g) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int m0 = blockIdx.y * 64 + (wv >> 1) * 32;
+    const int n0 = blockIdx.x * 128 + (wv & 1) * 64;
+    const int kb = blockIdx.z * g.kchunk;
+    const int ke = min(g.K, kb + g.kchunk);
+    const int li = lane & 31, lk = (lane >> 5) * 8;
+    const int am = m0 + li;
+    const int bn0 = n0 + li, bn1 = n0 + 32 + li;
+    const bool aok = am < g.M, b0ok = bn0 < g.N, b1ok = bn1 < g.N;
+    const bf16* pa = g.A + (long long)(aok ? am : 0) * g.lda + lk;
+    const bf16* pb0 = g.B + (long long)(b0ok ? bn0 : 0) * g.ldb + lk;
+    const bf16* pb1 = g.B + (long long)(b1ok ? bn1 : 0) * g.ldb + lk;
+    f32x16 c0 = {}, c1 = {};
+    if (kb < ke) {
+        bf16x8 a = load8(pa + kb, aok), b0 = load8(pb0 + kb, b0ok), b1 = load8(pb1 + kb, b1ok);
+        for (int k = kb; k < ke; k += 16) {
+            bf16x8 an = a, b0n = b0, b1n = b1;
+            if (k + 16 < ke) {
+                an = load8(pa + k + 16, aok);
+                b0n = load8(pb0 + k + 16, b0ok);
+                b1n = load8(pb1 + k + 16, b1ok);
+            }
+            c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1, c1, 0, 0, 0);
+            a = an; b0 = b0n; b1 = b1n;
+        }
+    }
+    // epilogue
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        const f32x16& c = half ? c1 : c0;
+        const int j = n0 + half * 32 + li;
+        if (j >= g.N) continue;
+        const float bj = g.bias ? g.bias[j] : 0.f;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = m0 + 8 * (r / 4) + 4 * (lane >> 5) + (r & 3);
+            if (i >= g.M) continue;
+            float v = c[r] + bj;
+            if (g.relu) v = fmaxf(v, 0.f);
+            if (g.resid) v += from_bf16(g.resid[(long long)i * g.ldr + j]);
+            if (g.Cf) g.Cf[(long long)blockIdx.z * g.M * g.ldcf + (long long)i * g.ldcf + j] = v;
+            if (g.Cb) g.Cb[(long long)i * g.ldcb + j] = to_bf16(v);
+        }
+    }
+}
+
+// im2col of an NHWC bf16 tensor for a 3x3, pad-1 convolution:
+// out[p][tap*Cp + c] = x[shift(p, tap)][c]  (zero outside the image or c >= C).
+// flip selects the mirrored tap (dgrad).  Cp multiple of 8.
+__global__ void k_im2col(const bf16* x, int n, int H, int W, int C, int ldx, int Cp, int flip, bf16* out) {
+    const long long P = (long long)n * H * W;
+    const long long total = P * 9 * (Cp / 8);
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int c8 = (int)(idx % (Cp / 8));
+        const long long r = idx / (Cp / 8);
+        const int tap = (int)(r % 9);
+        const long long p = r / 9;
+        const int xx = (int)(p % W), yy = (int)((p / W) % H);
+        const long long nn = p / ((long long)W * H);
+        const int tt = flip ? 8 - tap : tap;
+        const int sy = yy + tt / 3 - 1, sx = xx + tt % 3 - 1;
+        bf16x8 v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = (bf16)0.0f;
+        if (sy >= 0 && sy < H && sx >= 0 && sx < W) {
+            const bf16* src = x + ((nn * H + sy) * W + sx) * ldx;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int cc = c8 * 8 + e;
+                if (cc < C) v[e] = src[cc];
+            }
+        }
+        *reinterpret_cast<bf16x8*>(out + (p * 9 + tap) * Cp + c8 * 8) = v;
+    }
+}
+
+// Transposed im2col for wgrad: out[tap*Cp + c][p] (row length ldo >= P, zero
+// padded to a multiple of 16), from NHWC x.
+__global__ void k_im2col_t(const bf16* x, int n, int H, int W, int C, int ldx, int Cp, long long ldo, bf16* out) {
+    const long long P = (long long)n * H * W;
+    const long long total = (long long)9 * Cp * ldo;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long p = idx % ldo;
+        const long long row = idx / ldo;
+        const int c = (int)(row % Cp), tap = (int)(row / Cp);
+        bf16 v = (bf16)0.0f;
+        if (p < P && c < C) {
+            const int xx = (int)(p % W), yy = (int)((p / W) % H);
+            const long long nn = p / ((long long)W * H);
+            const int sy = yy + tap / 3 - 1, sx = xx + tap % 3 - 1;
+            if (sy >= 0 && sy < H && sx >= 0 && sx < W) v = x[((nn * H + sy) * W + sx) * ldx + c];
+        }
+        out[idx] = v;
+    }
+}
+
+// y[c][p] = x[p][c] for c < C, p < P; zero padding up to ldy.
+__global__ void k_transpose(const bf16* x, long long P, int C, int ldx, long long ldy, int Crows, bf16* y) {
+    const long long total = (long long)Crows * ldy;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long p = idx % ldy;
+        const int c = (int)(idx / ldy);
+        y[idx] = (p < P && c < C) ? x[p * ldx + c] : (bf16)0.0f;
+    }
+}
+
+// Pixel-unshuffle with per-frame normalisation:
+// x[(t*Ly + ly)*Lx + lx][b*N + a] = img[t][ly*N + b][lx*N + a] / scale[t].
+__global__ void k_unshuffle(const float* img, int T, int Ly, int Lx, int N, const float* scale, int Cp, bf16* x) {
+    const int NN = N * N;
+    const long long total = (long long)T * Ly * Lx * Cp;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int ch = (int)(idx % Cp);
+        const long long p = idx / Cp;
+        const int lx = (int)(p % Lx), ly = (int)((p / Lx) % Ly);
+        const int t = (int)(p / ((long long)Lx * Ly));
+        float v = 0.f;
+        if (ch < NN) {
+            const int b = ch / N, a = ch % N;
+            v = img[((long long)t * Ly * N + ly * N + b) * ((long long)Lx * N) + lx * N + a] / scale[t];
+        }
+        x[idx] = to_bf16(v);
+    }
+}
+
+// Per-frame mean of the image (deterministic two-level reduction).
+__global__ __launch_bounds__(256) void k_frame_mean(const float* img, long long n, float* out) {
+    __shared__ double sh[256];
+    const float* a = img + (long long)blockIdx.x * n;
+    double s = 0.0;
+    for (long long i = threadIdx.x; i < n; i += blockDim.x) s += a[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = (float)fmax(sh[0] / (double)n, 1e-12);
+}
+
+__device__ __forceinline__ float softplusf(float x) { return x > 20.f ? x : log1pf(expf(x)); }
+__device__ __forceinline__ float sigmoidf(float x) { return 1.f / (1.f + expf(-x)); }
+
+// Head output -> volume guess at lenslet resolution, out[t][z][ly][lx].
+__global__ void k_head_out(const float* o, int P, int ldo, int nz, int Ly, int Lx, const float* scale, float* out) {
+    const long long total = (long long)P * nz;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(idx % nz);
+        const long long p = idx / nz;
+        const int lx = (int)(p % Lx), ly = (int)((p / Lx) % Ly);
+        const int t = (int)(p / ((long long)Lx * Ly));
+        out[(((long long)t * nz + z) * Ly + ly) * Lx + lx] = softplusf(o[p * ldo + z]) * scale[t];
+    }
+}
+
+// MSE loss on the normalised output and its gradient w.r.t. the head
+// pre-activation: target tgt[t][z][ly][lx] (volume units).
+__global__ __launch_bounds__(256) void k_loss(const float* o, int P, int ldo, int nz, int Ly, int Lx,
+                                              const float* scale, const float* tgt, bf16* dO, int ldd,
+                                              double* part) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    const long long total = (long long)P * ldd;
+    const double inv = 1.0 / ((double)P * nz);
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(idx % ldd);
+        const long long p = idx / ldd;
+        float gsc = 0.f;
+        if (z < nz) {
+            const int lx = (int)(p % Lx), ly = (int)((p / Lx) % Ly);
+            const int t = (int)(p / ((long long)Lx * Ly));
+            const float x = o[p * ldo + z];
+            const float y = softplusf(x);
+            const float tv = tgt[(((long long)t * nz + z) * Ly + ly) * Lx + lx] / scale[t];
+            const float d = y - tv;
+            acc += (double)d * d;
+            gsc = (float)(2.0 * inv) * d * sigmoidf(x);
+        }
+        dO[p * ldd + z] = to_bf16(gsc);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0] * inv;
+}
+
+__global__ void k_sum_parts(const double* part, int n, double* out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += part[i];
+        *out = s;
+    }
+}
+
+// dx *= (act > 0)  (ReLU backward on the stored post-activation).
+__global__ void k_relu_bwd(bf16* dx, const bf16* act, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        if (!(from_bf16(act[i]) > 0.f)) dx[i] = (bf16)0.0f;
+}
+
+// Sum split-K partials: out[i] = sum_z part[z][i]; bias gradient from dy:
+// db[c] = sum_p dy[p][c] (fixed order, one thread per channel).
+__global__ void k_reduce_splits(const float* part, int splits, long long n, float* out) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (int z = 0; z < splits; ++z) s += part[(long long)z * n + i];
+        out[i] = s;
+    }
+}
+__global__ __launch_bounds__(256) void k_bias_grad(const bf16* dy, long long P, int C, int ldy, float* db) {
+    __shared__ float sh[256];
+    const int c = blockIdx.x;
+    float s = 0.f;
+    for (long long p = threadIdx.x; p < P; p += blockDim.x) s += from_bf16(dy[p * ldy + c]);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[c] = sh[0];
+}
+
+// Adam on fp32 master weights (the bf16 copies are refreshed afterwards).
+__global__ void k_adam(float* w, const float* g, float* m, float* v, long long n, float lr, float b1, float b2,
+                       float eps, float bc1, float bc2) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        w[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    }
+}
+__global__ void k_weights_bf16(const float* w, int Co, int Ci, int Cip, int Cop, bf16* Wb, bf16* Wt) {
+    // w: [Co][9][Ci] fp32 (unpadded);  Wb: [Co][9][Cip];  Wt: [Ci][9][Cop], the
+    // dgrad operand (the tap mirroring is done by the flipped im2col).
+    const long long total = (long long)Co * 9 * Ci;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int ci = (int)(idx % Ci);
+        const int tap = (int)((idx / Ci) % 9);
+        const int co = (int)(idx / (9LL * Ci));
+        const bf16 b = to_bf16(w[idx]);
+        Wb[((long long)co * 9 + tap) * Cip + ci] = b;
+        Wt[((long long)ci * 9 + tap) * Cop + co] = b;
+    }
+}
+
+// Nearest upsampling of the lenslet-resolution guess to the volume:
+// vol[z][t][y][x] = g[t][z][y / N][x / N].
+__global__ void k_upsample(const float* g, int T, int nz, int Ly, int Lx, int N, float* vol) {
+    const int Y = Ly * N, X = Lx * N;
+    const long long total = (long long)nz * T * Y * X;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(idx % X);
+        const int y = (int)((idx / X) % Y);
+        const int t = (int)((idx / ((long long)X * Y)) % T);
+        const int z = (int)(idx / ((long long)X * Y * T));
+        vol[idx] = g[(((long long)t * nz + z) * Ly + y / N) * Lx + x / N];
+    }
+}
+
+__global__ void k_fill0(float* p, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0.f;
+}
+
+int blocks_for(long long n) { return (int)std::max<long long>(1, std::min<long long>(65536, (n + 255) / 256)); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct NetLayer {
+    int Ci, Co, Cip, Cop;        // channels and their padded sizes (multiples of 16)
+    long long woff, boff;        // offsets into the flat parameter vector (W [Co][9][Ci], b [Co])
+    bf16 *Wb = nullptr, *Wt = nullptr;
+};
+
+struct flf_net {
+    int device = 0;
+    std::string err;
+    int N = 15, nz = 0, width = 128, nblocks = 4;
+    std::vector<NetLayer> L;
+    long long nparam = 0;
+    float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+    int step = 0;
+    // activation buffers (sized for the largest batch seen)
+    long long cap_P = 0;
+    bf16 *x = nullptr, *col = nullptr, *colt = nullptr, *dyt = nullptr, *dbuf0 = nullptr, *dbuf1 = nullptr;
+    std::vector<bf16*> h, t;     // h[0..nblocks], t[1..nblocks] (t[0] unused)
+    float *o = nullptr, *scale = nullptr, *wpart = nullptr;
+    double *lpart = nullptr, *lout = nullptr;
+    bf16* dO = nullptr;
+    std::vector<void*> acts;
+};
+
+namespace {
+
+int nfail(flf_net* n, int code, const std::string& msg) {
+    if (n) n->err = msg;
+    return code;
+}
+
+#define NCHK(call)                                                                             \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) return nfail(net, FLF_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int pad16(int c) { return (c + 15) / 16 * 16; }
+
+template <class T>
+int nalloc(flf_net* net, T** p, size_t count, std::vector<void*>& list) {
+    if (hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return nfail(net, FLF_ENOMEM, "hipMalloc failed in the network");
+    }
+    list.push_back((void*)*p);
+    return FLF_OK;
+}
+
+void free_acts(flf_net* net) {
+    for (void* p : net->acts) (void)hipFree(p);
+    net->acts.clear();
+    net->h.clear();
+    net->t.clear();
+    net->cap_P = 0;
+}
+
+int ensure_acts(flf_net* net, long long P) {
+    if (P <= net->cap_P) return FLF_OK;
+    free_acts(net);
+    const int Cin = net->L[0].Cip, Wd = net->width, Cop = net->L.back().Cop;
+    const int maxC = std::max(Cin, Wd);
+    const long long Pp = (P + 15) / 16 * 16;
+    auto& A = net->acts;
+    int rc;
+    if ((rc = nalloc(net, &net->x, (size_t)P * Cin, A))) return rc;
+    if ((rc = nalloc(net, &net->col, (size_t)P * 9 * maxC, A))) return rc;
+    if ((rc = nalloc(net, &net->colt, (size_t)9 * maxC * Pp, A))) return rc;
+    if ((rc = nalloc(net, &net->dyt, (size_t)std::max(Wd, Cop) * Pp, A))) return rc;
+    if ((rc = nalloc(net, &net->dbuf0, (size_t)P * std::max(Wd, Cop), A))) return rc;
+    if ((rc = nalloc(net, &net->dbuf1, (size_t)P * std::max(Wd, Cop), A))) return rc;
+    net->h.assign(net->nblocks + 1, nullptr);
+    net->t.assign(net->nblocks + 1, nullptr);
+    for (int i = 0; i <= net->nblocks; ++i)
+        if ((rc = nalloc(net, &net->h[i], (size_t)P * Wd, A))) return rc;
+    for (int i = 1; i <= net->nblocks; ++i)
+        if ((rc = nalloc(net, &net->t[i], (size_t)P * Wd, A))) return rc;
+    if ((rc = nalloc(net, &net->o, (size_t)P * Cop, A))) return rc;
+    if ((rc = nalloc(net, &net->dO, (size_t)P * Cop, A))) return rc;
+    if ((rc = nalloc(net, &net->scale, (size_t)1024, A))) return rc;
+    if ((rc = nalloc(net, &net->lpart, (size_t)1024, A))) return rc;
+    if ((rc = nalloc(net, &net->lout, (size_t)1, A))) return rc;
+    // split-K partials for wgrad: splits x Co x 9*Ci
+    if ((rc = nalloc(net, &net->wpart, (size_t)33 * 9 * maxC * std::max(Wd, Cop), A))) return rc;
+    net->cap_P = P;
+    return FLF_OK;
+}
+
+hipError_t gemm(const GemmArgs& a, int splits, hipStream_t s) {
+    dim3 grid((a.N + 127) / 128, (a.M + 63) / 64, splits);
+    hipLaunchKernelGGL(k_gemm_nt, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// out = act(conv(x) + b) [+ resid]; x NHWC with ldx, out NHWC [P][ldo].
+int conv_fwd(flf_net* net, const NetLayer& l, const bf16* x, int ldx, int n, int H, int W, int relu, const bf16* resid,
+             bf16* outb, float* outf, int ldo, hipStream_t s) {
+    const long long P = (long long)n * H * W;
+    hipLaunchKernelGGL(k_im2col, dim3(blocks_for(P * 9 * l.Cip / 8)), dim3(256), 0, s, x, n, H, W, l.Ci, ldx, l.Cip,
+                       0, net->col);
+    NCHK(hipGetLastError());
+    GemmArgs a{};
+    a.A = net->col; a.lda = 9LL * l.Cip;
+    a.B = l.Wb; a.ldb = 9LL * l.Cip;
+    a.M = (int)P; a.N = l.Co; a.K = 9 * l.Cip; a.kchunk = a.K;
+    a.bias = net->w + l.boff;
+    a.resid = resid; a.ldr = ldo;
+    a.relu = relu;
+    a.Cb = outb; a.ldcb = ldo;
+    a.Cf = outf; a.ldcf = ldo;
+    NCHK(gemm(a, 1, s));
+    return FLF_OK;
+}
+
+// dx = dgrad(dy) (+ add if accumulate); dy NHWC [P][ldy] with l.Co channels.
+int conv_dgrad(flf_net* net, const NetLayer& l, const bf16* dy, int ldy, int n, int H, int W, const bf16* add,
+               bf16* dx, int ldx, hipStream_t s) {
+    const long long P = (long long)n * H * W;
+    hipLaunchKernelGGL(k_im2col, dim3(blocks_for(P * 9 * l.Cop / 8)), dim3(256), 0, s, dy, n, H, W, l.Co, ldy, l.Cop,
+                       1, net->col);
+    NCHK(hipGetLastError());
+    GemmArgs a{};
+    a.A = net->col; a.lda = 9LL * l.Cop;
+    a.B = l.Wt; a.ldb = 9LL * l.Cop;
+    a.M = (int)P; a.N = l.Ci; a.K = 9 * l.Cop; a.kchunk = a.K;
+    a.resid = add; a.ldr = ldx;
+    a.Cb = dx; a.ldcb = ldx;
+    NCHK(gemm(a, 1, s));
+    return FLF_OK;
+}
+
+// dW (fp32, [Co][9][Ci] in the gradient vector) and db from x and dy.
+int conv_wgrad(flf_net* net, const NetLayer& l, const bf16* x, int ldx, const bf16* dy, int ldy, int n, int H, int W,
+               hipStream_t s) {
+    const long long P = (long long)n * H * W;
+    const long long Pp = (P + 15) / 16 * 16;
+    hipLaunchKernelGGL(k_im2col_t, dim3(blocks_for(9LL * l.Cip * Pp)), dim3(256), 0, s, x, n, H, W, l.Ci, ldx, l.Cip,
+                       Pp, net->colt);
+    hipLaunchKernelGGL(k_transpose, dim3(blocks_for((long long)l.Co * Pp)), dim3(256), 0, s, dy, P, l.Co, ldy, Pp,
+                       l.Co, net->dyt);
+    NCHK(hipGetLastError());
+    int splits = (int)std::min<long long>(32, std::max<long long>(1, Pp / 2048));
+    int kchunk = (int)((Pp + splits - 1) / splits);
+    kchunk = (kchunk + 15) / 16 * 16;
+    splits = (int)((Pp + kchunk - 1) / kchunk);
+    GemmArgs a{};
+    a.A = net->dyt; a.lda = Pp;
+    a.B = net->colt; a.ldb = Pp;
+    a.M = l.Co; a.N = 9 * l.Cip; a.K = (int)Pp; a.kchunk = kchunk;
+    a.Cf = net->wpart; a.ldcf = 9LL * l.Cip;
+    NCHK(gemm(a, splits, s));
+    // reduce splits into the padded layout, then compact into [Co][9][Ci]
+    const long long n_pad = (long long)l.Co * 9 * l.Cip;
+    hipLaunchKernelGGL(k_reduce_splits, dim3(blocks_for(n_pad)), dim3(256), 0, s, net->wpart, splits, n_pad,
+                       net->wpart + (long long)splits * n_pad);
+    NCHK(hipGetLastError());
+    // compaction on the host side of the layout: copy rows of Ci out of Cip
+    NCHK(hipMemcpy2DAsync(net->g + l.woff, sizeof(float) * l.Ci, net->wpart + (long long)splits * n_pad,
+                          sizeof(float) * l.Cip, sizeof(float) * l.Ci, (size_t)l.Co * 9, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_bias_grad, dim3(l.Co), dim3(256), 0, s, dy, P, l.Co, ldy, net->g + l.boff);
+    NCHK(hipGetLastError());
+    return FLF_OK;
+}
+
+int refresh_bf16(flf_net* net, hipStream_t s) {
+    for (auto& l : net->L) {
+        hipLaunchKernelGGL(k_weights_bf16, dim3(blocks_for((long long)l.Co * 9 * l.Ci)), dim3(256), 0, s,
+                           net->w + l.woff, l.Co, l.Ci, l.Cip, l.Cop, l.Wb, l.Wt);
+        NCHK(hipGetLastError());
+    }
+    return FLF_OK;
+}
+
+int forward(flf_net* net, const float* img, int T, int Ly, int Lx, hipStream_t s) {
+    const int N = net->N, Wd = net->width;
+    const long long P = (long long)T * Ly * Lx;
+    int rc = ensure_acts(net, P);
+    if (rc) return rc;
+    if (T > 1024) return nfail(net, FLF_EINVAL, "too many frames for the network");
+    hipLaunchKernelGGL(k_frame_mean, dim3(T), dim3(256), 0, s, img, (long long)Ly * N * Lx * N, net->scale);
+    hipLaunchKernelGGL(k_unshuffle, dim3(blocks_for(P * net->L[0].Cip)), dim3(256), 0, s, img, T, Ly, Lx, N, net->scale,
+                       net->L[0].Cip, net->x);
+    NCHK(hipGetLastError());
+    if ((rc = conv_fwd(net, net->L[0], net->x, net->L[0].Cip, T, Ly, Lx, 1, nullptr, net->h[0], nullptr, Wd, s))) return rc;
+    for (int i = 1; i <= net->nblocks; ++i) {
+        const NetLayer& la = net->L[2 * i - 1];
+        const NetLayer& lb = net->L[2 * i];
+        if ((rc = conv_fwd(net, la, net->h[i - 1], Wd, T, Ly, Lx, 1, nullptr, net->t[i], nullptr, Wd, s))) return rc;
+        if ((rc = conv_fwd(net, lb, net->t[i], Wd, T, Ly, Lx, 0, net->h[i - 1], net->h[i], nullptr, Wd, s))) return rc;
+    }
+    const NetLayer& lh = net->L.back();
+    if ((rc = conv_fwd(net, lh, net->h[net->nblocks], Wd, T, Ly, Lx, 0, nullptr, nullptr, net->o, lh.Cop, s))) return rc;
+    return FLF_OK;
+}
+
+int backward(flf_net* net, const float* tgt, int T, int Ly, int Lx, hipStream_t s) {
+    const int Wd = net->width, nb = net->nblocks;
+    const long long P = (long long)T * Ly * Lx;
+    const NetLayer& lh = net->L.back();
+    const int nlb = 256;
+    hipLaunchKernelGGL(k_loss, dim3(nlb), dim3(256), 0, s, net->o, (int)P, lh.Cop, net->nz, Ly, Lx, net->scale, tgt,
+                       net->dO, lh.Cop, net->lpart);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1), 0, s, net->lpart, nlb, net->lout);
+    NCHK(hipGetLastError());
+    int rc;
+    // head
+    if ((rc = conv_wgrad(net, lh, net->h[nb], Wd, net->dO, lh.Cop, T, Ly, Lx, s))) return rc;
+    bf16* dh = net->dbuf0;
+    bf16* dt = net->dbuf1;
+    if ((rc = conv_dgrad(net, lh, net->dO, lh.Cop, T, Ly, Lx, nullptr, dh, Wd, s))) return rc;
+    for (int i = nb; i >= 1; --i) {
+        const NetLayer& la = net->L[2 * i - 1];
+        const NetLayer& lb = net->L[2 * i];
+        // h_i = h_{i-1} + conv_b(t_i)
+        if ((rc = conv_wgrad(net, lb, net->t[i], Wd, dh, Wd, T, Ly, Lx, s))) return rc;
+        if ((rc = conv_dgrad(net, lb, dh, Wd, T, Ly, Lx, nullptr, dt, Wd, s))) return rc;
+        hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(P * Wd)), dim3(256), 0, s, dt, net->t[i], P * Wd);
+        NCHK(hipGetLastError());
+        if ((rc = conv_wgrad(net, la, net->h[i - 1], Wd, dt, Wd, T, Ly, Lx, s))) return rc;
+        // dh_{i-1} = dh_i + dgrad_a(dt): accumulate into dt's buffer, then swap
+        if ((rc = conv_dgrad(net, la, dt, Wd, T, Ly, Lx, dh, dt, Wd, s))) return rc;
+        std::swap(dh, dt);
+    }
+    hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(P * Wd)), dim3(256), 0, s, dh, net->h[0], P * Wd);
+    NCHK(hipGetLastError());
+    if ((rc = conv_wgrad(net, net->L[0], net->x, net->L[0].Cip, dh, Wd, T, Ly, Lx, s))) return rc;
+    return FLF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int flf_net_create(int device, int nnum, int nz, int width, int nblocks, flf_net** out) {
+    if (!out) return FLF_EINVAL;
+    *out = nullptr;
+    if (nnum < 3 || nz < 1 || nz > 512 || width < 16 || width % 16 || nblocks < 0 || nblocks > 64) return FLF_EINVAL;
+    flf_net* net = new flf_net();
+    net->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete net; return FLF_EHIP; }
+    net->N = nnum; net->nz = nz; net->width = width; net->nblocks = nblocks;
+    auto add = [&](int Ci, int Co) {
+        NetLayer l;
+        l.Ci = Ci; l.Co = Co; l.Cip = pad16(Ci); l.Cop = pad16(Co);
+        l.woff = net->nparam;
+        net->nparam += (long long)Co * 9 * Ci;
+        l.boff = net->nparam;
+        net->nparam += Co;
+        net->L.push_back(l);
+    };
+    add(nnum * nnum, width);
+    for (int i = 0; i < nblocks; ++i) { add(width, width); add(width, width); }
+    add(width, nz);
+    std::vector<void*> keep;
+    int rc = 0;
+    rc |= nalloc(net, &net->w, (size_t)net->nparam, keep);
+    rc |= nalloc(net, &net->g, (size_t)net->nparam, keep);
+    rc |= nalloc(net, &net->m, (size_t)net->nparam, keep);
+    rc |= nalloc(net, &net->v, (size_t)net->nparam, keep);
+    for (auto& l : net->L) {
+        rc |= nalloc(net, &l.Wb, (size_t)l.Co * 9 * l.Cip, keep);
+        rc |= nalloc(net, &l.Wt, (size_t)l.Ci * 9 * l.Cop, keep);
+        if (!rc) {
+            (void)hipMemset(l.Wb, 0, (size_t)l.Co * 9 * l.Cip * sizeof(bf16));
+            (void)hipMemset(l.Wt, 0, (size_t)l.Ci * 9 * l.Cop * sizeof(bf16));
+        }
+    }
+    if (rc) {
+        for (void* p : keep) (void)hipFree(p);
+        delete net;
+        return FLF_ENOMEM;
+    }
+    (void)hipMemset(net->m, 0, net->nparam * sizeof(float));
+    (void)hipMemset(net->v, 0, net->nparam * sizeof(float));
+    (void)hipMemset(net->g, 0, net->nparam * sizeof(float));
+    (void)hipMemset(net->w, 0, net->nparam * sizeof(float));
+    net->acts.clear();
+    // keep the parameter allocations separate from the activation list
+    net->err.clear();
+    *out = net;
+    return FLF_OK;
+}
+
+void flf_net_destroy(flf_net* net) {
+    if (!net) return;
+    (void)hipSetDevice(net->device);
+    free_acts(net);
+    (void)hipFree(net->w); (void)hipFree(net->g); (void)hipFree(net->m); (void)hipFree(net->v);
+    for (auto& l : net->L) { (void)hipFree(l.Wb); (void)hipFree(l.Wt); }
+    delete net;
+}
+
+const char* flf_net_last_error(const flf_net* net) { return net ? net->err.c_str() : "null network"; }
+
+long long flf_net_num_params(const flf_net* net) { return net ? net->nparam : 0; }
+
+int flf_net_set_params(flf_net* net, const float* host) {
+    if (!net || !host) return FLF_EINVAL;
+    NCHK(hipMemcpy(net->w, host, net->nparam * sizeof(float), hipMemcpyHostToDevice));
+    (void)hipMemset(net->m, 0, net->nparam * sizeof(float));
+    (void)hipMemset(net->v, 0, net->nparam * sizeof(float));
+    net->step = 0;
+    int rc = refresh_bf16(net, 0);
+    if (rc) return rc;
+    NCHK(hipDeviceSynchronize());
+    return FLF_OK;
+}
+
+int flf_net_get_params(flf_net* net, float* host) {
+    if (!net || !host) return FLF_EINVAL;
+    NCHK(hipDeviceSynchronize());
+    NCHK(hipMemcpy(host, net->w, net->nparam * sizeof(float), hipMemcpyDeviceToHost));
+    return FLF_OK;
+}
+
+int flf_net_get_grads(flf_net* net, float* host) {
+    if (!net || !host) return FLF_EINVAL;
+    NCHK(hipDeviceSynchronize());
+    NCHK(hipMemcpy(host, net->g, net->nparam * sizeof(float), hipMemcpyDeviceToHost));
+    return FLF_OK;
+}
+
+int flf_net_forward(flf_net* net, const float* img, int T, int Ly, int Lx, float* out, void* stream) {
+    if (!net || !img || !out || T < 1 || Ly < 1 || Lx < 1) return nfail(net, FLF_EINVAL, "bad network input");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = forward(net, img, T, Ly, Lx, s);
+    if (rc) return rc;
+    const NetLayer& lh = net->L.back();
+    const long long P = (long long)T * Ly * Lx;
+    hipLaunchKernelGGL(k_head_out, dim3(blocks_for(P * net->nz)), dim3(256), 0, s, net->o, (int)P, lh.Cop, net->nz, Ly,
+                       Lx, net->scale, out);
+    NCHK(hipGetLastError());
+    return FLF_OK;
+}
+
+int flf_net_train_step(flf_net* net, const float* img, const float* target, int T, int Ly, int Lx, float lr,
+                       double* loss_host, void* stream) {
+    if (!net || !img || !target || T < 1 || Ly < 1 || Lx < 1) return nfail(net, FLF_EINVAL, "bad training input");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = forward(net, img, T, Ly, Lx, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fill0, dim3(blocks_for(net->nparam)), dim3(256), 0, s, net->g, net->nparam);
+    if ((rc = backward(net, target, T, Ly, Lx, s))) return rc;
+    if (lr > 0.f) {
+        net->step += 1;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float bc1 = 1.f - std::pow(b1, (float)net->step), bc2 = 1.f - std::pow(b2, (float)net->step);
+        hipLaunchKernelGGL(k_adam, dim3(blocks_for(net->nparam)), dim3(256), 0, s, net->w, net->g, net->m, net->v,
+                           net->nparam, lr, b1, b2, 1e-8f, bc1, bc2);
+        NCHK(hipGetLastError());
+        if ((rc = refresh_bf16(net, s))) return rc;
+    }
+    if (loss_host) {
+        NCHK(hipMemcpyAsync(loss_host, net->lout, sizeof(double), hipMemcpyDeviceToHost, s));
+        NCHK(hipStreamSynchronize(s));
+    }
+    return FLF_OK;
+}
+
+int flf_net_upsample(const float* g, int T, int nz, int Ly, int Lx, int nnum, float* vol, void* stream) {
+    if (!g || !vol || T < 1 || nz < 1 || Ly < 1 || Lx < 1 || nnum < 1) return FLF_EINVAL;
+    const long long total = (long long)nz * T * Ly * nnum * Lx * nnum;
+    hipLaunchKernelGGL(k_upsample, dim3(blocks_for(total)), dim3(256), 0, (hipStream_t)stream, g, T, nz, Ly, Lx, nnum,
+                       vol);
+    return hipGetLastError() == hipSuccess ? FLF_OK : FLF_EHIP;
+}
+
+// C = A * B^T for bf16 row-major A [M][K], B [N][K]; fp32 C [M][N].  Exposed
+// for the MFMA layout test.
+int flf_mfma_gemm_nt(const void* A, const void* B, float* C, int M, int N, int K, void* stream) {
+    if (!A || !B || !C || M < 1 || N < 1 || K < 16 || K % 16) return FLF_EINVAL;
+    GemmArgs a{};
+    a.A = (const bf16*)A; a.lda = K;
+    a.B = (const bf16*)B; a.ldb = K;
+    a.M = M; a.N = N; a.K = K; a.kchunk = K;
+    a.Cf = C; a.ldcf = N;
+    return gemm(a, 1, (hipStream_t)stream) == hipSuccess ? FLF_OK : FLF_EHIP;
+}
+
+}  // extern "C"

@@ -36,6 +36,21 @@ SIGNATURES = {
     "flf_datafit": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.POINTER(_c.c_double), _c.c_void_p]),
     "flf_fourier_len": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int]),
     "flf_grid_check": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.POINTER(_c.c_longlong)]),
+    "flf_net_create": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.POINTER(_c.c_void_p)]),
+    "flf_net_destroy": (None, [_c.c_void_p]),
+    "flf_net_last_error": (_c.c_char_p, [_c.c_void_p]),
+    "flf_net_num_params": (_c.c_longlong, [_c.c_void_p]),
+    "flf_net_set_params": (_c.c_int, [_c.c_void_p, _c.c_void_p]),
+    "flf_net_get_params": (_c.c_int, [_c.c_void_p, _c.c_void_p]),
+    "flf_net_get_grads": (_c.c_int, [_c.c_void_p, _c.c_void_p]),
+    "flf_net_forward": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_int, _c.c_int, _c.c_int, _c.c_void_p,
+                                   _c.c_void_p]),
+    "flf_net_train_step": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_int, _c.c_int, _c.c_int,
+                                      _c.c_float, _c.POINTER(_c.c_double), _c.c_void_p]),
+    "flf_net_upsample": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_void_p,
+                                    _c.c_void_p]),
+    "flf_mfma_gemm_nt": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_int, _c.c_int, _c.c_int,
+                                    _c.c_void_p]),
     "flf_profile_enable": (_c.c_int, [_c.c_void_p, _c.c_int]),
     "flf_profile_read": (_c.c_int, [_c.c_void_p, _c.POINTER(_c.c_double), _c.POINTER(_c.c_longlong),
                                     _c.POINTER(_c.c_double)]),

@@ -1,0 +1,214 @@
+"""Learned initialiser for Richardson-Lucy: a lenslet-aware conv net.
+
+The network (libflf_hip.so, flf_net_*) maps the rectified image to a volume
+guess at lenslet resolution (one value per plane and lenslet), which is then
+upsampled to full resolution and used as Xguess.  Training pairs are made by
+the method itself: seeded blob volumes are forward-projected through the same
+PSF set with Poisson noise and reconstructed by plain RL; the network learns
+the lenslet-block mean of that reconstruction.  The reference has no learned
+component; plain RL from a uniform start stays the fallback (the guard in
+``guess_or_fallback``).
+"""
+import ctypes
+import os
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class LensletNet(object):
+    def __init__(self, nnum, nz, width=128, nblocks=4, device=0, seed=0):
+        self.lib = _lib.load()
+        self.nnum, self.nz, self.width, self.nblocks = nnum, nz, width, nblocks
+        self.device = torch.device("cuda", device)
+        h = ctypes.c_void_p()
+        rc = self.lib.flf_net_create(device, nnum, nz, width, nblocks, ctypes.byref(h))
+        if rc != 0:
+            raise _lib.FLFError("flf_net_create failed (%d)" % rc)
+        self._h = h
+        self.nparam = int(self.lib.flf_net_num_params(h))
+        self.set_params(self.init_params(seed))
+
+    def __del__(self):
+        lib = getattr(self, "lib", None)
+        h = getattr(self, "_h", None)
+        if lib is not None and h:
+            lib.flf_net_destroy(h)
+            self._h = None
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.flf_net_last_error(self._h)
+            raise _lib.FLFError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    # ---- parameters: flat float32, per layer W[Co][9][Ci] then b[Co] ----
+    def layer_shapes(self):
+        shapes = [(self.nnum * self.nnum, self.width)]
+        shapes += [(self.width, self.width)] * (2 * self.nblocks)
+        shapes += [(self.width, self.nz)]
+        return shapes
+
+    def init_params(self, seed):
+        """He-normal weights; the second conv of each residual block and the
+        head start small so the initial map is close to identity/constant."""
+        rng = np.random.default_rng(seed)
+        out = []
+        shapes = self.layer_shapes()
+        for li, (ci, co) in enumerate(shapes):
+            std = np.sqrt(2.0 / (9 * ci))
+            if li > 0 and li < len(shapes) - 1 and li % 2 == 0:
+                std *= 0.1
+            if li == len(shapes) - 1:
+                std *= 0.1
+            out.append(rng.normal(0.0, std, size=co * 9 * ci).astype(np.float32))
+            out.append(np.zeros(co, np.float32))
+        p = np.concatenate(out)
+        assert p.size == self.nparam, (p.size, self.nparam)
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.nparam:
+            raise ValueError("expected %d parameters, got %d" % (self.nparam, p.size))
+        self._check(self.lib.flf_net_set_params(self._h, p.ctypes.data_as(ctypes.c_void_p)), "flf_net_set_params")
+
+    def get_params(self):
+        p = np.empty(self.nparam, np.float32)
+        self._check(self.lib.flf_net_get_params(self._h, p.ctypes.data_as(ctypes.c_void_p)), "flf_net_get_params")
+        return p
+
+    def get_grads(self):
+        p = np.empty(self.nparam, np.float32)
+        self._check(self.lib.flf_net_get_grads(self._h, p.ctypes.data_as(ctypes.c_void_p)), "flf_net_get_grads")
+        return p
+
+    def save(self, path, **meta):
+        np.savez(path, params=self.get_params(), nnum=self.nnum, nz=self.nz, width=self.width,
+                 nblocks=self.nblocks, **meta)
+
+    @classmethod
+    def load(cls, path, device=0):
+        d = np.load(path, allow_pickle=False)
+        net = cls(int(d["nnum"]), int(d["nz"]), int(d["width"]), int(d["nblocks"]), device=device)
+        net.set_params(d["params"])
+        return net
+
+    # ---- compute ----
+    def forward(self, img):
+        """img: (T, Ly*N, Lx*N) float32 cuda -> (T, nz, Ly, Lx) guess (volume units)."""
+        T, Y, X = img.shape
+        N = self.nnum
+        if Y % N or X % N:
+            raise ValueError("image size must be a multiple of Nnum")
+        Ly, Lx = Y // N, X // N
+        img = img.contiguous()
+        out = torch.empty((T, self.nz, Ly, Lx), dtype=torch.float32, device=self.device)
+        self._check(self.lib.flf_net_forward(self._h, _ptr(img), T, Ly, Lx, _ptr(out), _stream()), "flf_net_forward")
+        return out
+
+    def train_step(self, img, target, lr):
+        T, Y, X = img.shape
+        N = self.nnum
+        loss = ctypes.c_double()
+        self._check(self.lib.flf_net_train_step(self._h, _ptr(img.contiguous()), _ptr(target.contiguous()), T,
+                                                Y // N, X // N, float(lr), ctypes.byref(loss), _stream()),
+                    "flf_net_train_step")
+        return loss.value
+
+    def volume_guess(self, img):
+        """Full-resolution Xguess (nz, T, Y, X) from the image."""
+        g = self.forward(img)
+        T, nz, Ly, Lx = g.shape
+        N = self.nnum
+        vol = torch.empty((nz, T, Ly * N, Lx * N), dtype=torch.float32, device=self.device)
+        rc = self.lib.flf_net_upsample(_ptr(g), T, nz, Ly, Lx, N, _ptr(vol), _stream())
+        if rc != 0:
+            raise _lib.FLFError("flf_net_upsample failed (%d)" % rc)
+        return vol
+
+
+def lenslet_means(vol, nnum):
+    """(nz, T, Y, X) -> (T, nz, Ly, Lx): mean over each lenslet block."""
+    nz, T, Y, X = vol.shape
+    v = vol.reshape(nz, T, Y // nnum, nnum, X // nnum, nnum).mean(dim=(3, 5))
+    return v.permute(1, 0, 2, 3).contiguous()
+
+
+def blob_volumes_gpu(nz, Y, X, T, nblobs_range, seed, device, background=0.01, sigma=(1.0, 3.0), amp=(0.5, 2.0)):
+    """Seeded Gaussian-blob volumes (nz, T, Y, X) built on the GPU as sums of
+    separable Gaussians (same statistics as synthetic.blob_volume)."""
+    rng = np.random.default_rng(seed)
+    vol = torch.full((nz, T, Y, X), background, dtype=torch.float32, device=device)
+    zz = torch.arange(nz, device=device, dtype=torch.float32)
+    yy = torch.arange(Y, device=device, dtype=torch.float32)
+    xx = torch.arange(X, device=device, dtype=torch.float32)
+    for t in range(T):
+        nb = int(rng.integers(nblobs_range[0], nblobs_range[1] + 1))
+        zc = torch.tensor(rng.uniform(0, nz, nb), dtype=torch.float32, device=device)
+        yc = torch.tensor(rng.uniform(0, Y, nb), dtype=torch.float32, device=device)
+        xc = torch.tensor(rng.uniform(0, X, nb), dtype=torch.float32, device=device)
+        sg = torch.tensor(rng.uniform(sigma[0], sigma[1], nb), dtype=torch.float32, device=device)
+        am = torch.tensor(rng.uniform(amp[0], amp[1], nb), dtype=torch.float32, device=device)
+        gz = am[:, None] * torch.exp(-(zz[None] - zc[:, None]) ** 2 / (2 * sg[:, None] ** 2))   # (nb, nz)
+        gy = torch.exp(-(yy[None] - yc[:, None]) ** 2 / (2 * sg[:, None] ** 2))                # (nb, Y)
+        gx = torch.exp(-(xx[None] - xc[:, None]) ** 2 / (2 * sg[:, None] ** 2))                # (nb, X)
+        for z in range(nz):
+            # sum_i gz[i,z] gy[i,:]^T gx[i,:]  as one fp32 matmul
+            vol[z, t] += (gy * gz[:, z:z + 1]).t().contiguous() @ gx
+    return vol
+
+
+def make_pairs(proj, hMatrix, n, L, seed, peak=(500.0, 5000.0), iters=8, batch=16, nblobs=(50, 300)):
+    """n training pairs at L x L lenslets: (images (n, Y, X), targets
+    (n, nz, L, L)) on the GPU, targets = lenslet means of plain RL after
+    `iters` iterations from a uniform start."""
+    from .lfdeconv import deconv_rl_device
+    N = hMatrix.Nnum
+    Y = X = L * N
+    nz = hMatrix.numZ
+    dev = proj.device
+    imgs = torch.empty((n, Y, X), dtype=torch.float32, device=dev)
+    tgts = torch.empty((n, nz, L, L), dtype=torch.float32, device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    rng = np.random.default_rng(seed)
+    for b0 in range(0, n, batch):
+        T = min(batch, n - b0)
+        vol = blob_volumes_gpu(nz, Y, X, T, nblobs, seed * 100003 + b0, dev)
+        proj.prepare(hMatrix, T, Y, X)
+        clean = proj.forward_device(hMatrix, vol)
+        peaks = torch.tensor(rng.uniform(peak[0], peak[1], T), dtype=torch.float32, device=dev)
+        scale = peaks / clean.reshape(T, -1).amax(dim=1)
+        im = torch.poisson(clean.clamp_min(0) * scale[:, None, None], generator=gen)
+        Htf = proj.backward_device(hMatrix, im)
+        Xv = torch.ones_like(Htf)
+        deconv_rl_device(proj, hMatrix, Htf, iters, Xv, im=im)
+        imgs[b0:b0 + T] = im
+        tgts[b0:b0 + T] = lenslet_means(Xv, N)
+        del vol, clean, Htf, Xv
+    return imgs, tgts
+
+
+def train(net, imgs, tgts, steps, batch=32, lr=1e-3, seed=0, log_every=50, log=print):
+    """Adam on random mini-batches; returns the loss history."""
+    rng = np.random.default_rng(seed)
+    n = imgs.shape[0]
+    hist = []
+    for s in range(steps):
+        idx = torch.tensor(rng.choice(n, size=min(batch, n), replace=False), device=imgs.device)
+        lr_s = lr * 0.5 * (1 + np.cos(np.pi * s / max(steps, 1)))          # cosine decay
+        loss = net.train_step(imgs[idx], tgts[idx], lr_s)
+        hist.append(loss)
+        if log and (s % log_every == 0 or s == steps - 1):
+            log("step %d loss %.5f" % (s, loss))
+    return hist

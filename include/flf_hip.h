@@ -117,6 +117,34 @@ int flf_profile_read(flf_ctx* ctx, double* mac_ms, long long* mac_launches, doub
 int flf_fourier_len(int s1, int s2, int nnum);
 int flf_grid_check(int nnum, int fY, int fX, long long* stats);
 
+/* ---- Learned initialiser (no reference counterpart: the reference starts
+ * RL from Htf or a caller-supplied Xguess, lfdeconv.py:97-104) ----
+ * A lenslet-aware conv net on BF16 MFMA kernels with FP32 accumulation:
+ * pixel-unshuffled image (nnum^2 channels at lenslet resolution) -> 3x3 conv
+ * to `width` channels -> `nblocks` residual blocks -> 3x3 conv to nz planes ->
+ * softplus, scaled by the frame mean.  Parameters are one flat float32
+ * vector: per layer W[Co][9][Ci] then bias[Co], layers conv0, (a, b) per
+ * block, head. */
+typedef struct flf_net flf_net;
+int flf_net_create(int device, int nnum, int nz, int width, int nblocks, flf_net** out);
+void flf_net_destroy(flf_net* net);
+const char* flf_net_last_error(const flf_net* net);
+long long flf_net_num_params(const flf_net* net);
+int flf_net_set_params(flf_net* net, const float* host);   /* also resets the Adam state */
+int flf_net_get_params(flf_net* net, float* host);
+int flf_net_get_grads(flf_net* net, float* host);          /* gradient of the last train step */
+/* img: device [T][Ly*nnum][Lx*nnum]; out: device [T][nz][Ly][Lx] (volume units) */
+int flf_net_forward(flf_net* net, const float* img, int T, int Ly, int Lx, float* out, void* stream);
+/* One Adam step (lr > 0) or gradient only (lr == 0) on the MSE between the
+ * normalised output and target [T][nz][Ly][Lx]; loss_host may be NULL. */
+int flf_net_train_step(flf_net* net, const float* img, const float* target, int T, int Ly, int Lx, float lr,
+                       double* loss_host, void* stream);
+/* Nearest upsampling of [T][nz][Ly][Lx] to a volume [nz][T][Ly*nnum][Lx*nnum]. */
+int flf_net_upsample(const float* g, int T, int nz, int Ly, int Lx, int nnum, float* vol, void* stream);
+/* C[M][N] = A[M][K] * B[N][K]^T (bf16 in, fp32 out, K % 16 == 0): the MFMA
+ * GEMM the network uses, exposed for its layout test. */
+int flf_mfma_gemm_nt(const void* A, const void* B, float* C, int M, int N, int K, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
