@@ -622,6 +622,12 @@ int flf_backward_rl_update(flf_ctx* ctx, const float* img, float* X, const float
     return run_backward(ctx, img, X, Htf, trust, mode, planes, nplanes, (hipStream_t)stream);
 }
 
+int flf_rl_scale(const float* Htf, const float* v, float* X, long long n, void* stream) {
+    if (!Htf || !v || !X || n < 0) return FLF_EINVAL;
+    if (n == 0) return FLF_OK;
+    return flf::launch_rl_scale(X, Htf, v, n, (hipStream_t)stream) == hipSuccess ? FLF_OK : FLF_EHIP;
+}
+
 int flf_datafit(flf_ctx* ctx, const float* img, const float* im, double* out_host, void* stream) {
     if (!ctx) return FLF_EINVAL;
     if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");

@@ -138,6 +138,7 @@ hipError_t launch_ifft_rows_plane(const IfftRowPlaneArgs& a, int nplanes, int ma
 hipError_t launch_fit(const float* img, const float* im, long long n_per_frame, int T, double* part, int nblk,
                       double* out, hipStream_t s);
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t s);
+hipError_t launch_rl_scale(float* X, const float* Htf, const float* v, long long n, hipStream_t s);
 // Raise the dynamic-LDS limit of the FFT kernels to max_lds bytes.
 hipError_t configure_kernels(size_t max_lds);
 }  // namespace flf

@@ -676,6 +676,18 @@ __global__ __launch_bounds__(256) void k_fit_final(const double* part, int nblk,
     }
 }
 
+// Prevedel update with a precomputed denominator (lfdeconv.py:112-126):
+// X *= Htf / v, NaN -> 0, the same arithmetic as k_ifft_rows_plane mode 1.
+// Used for the first plain iterate from a uniform start, X1 = Htf / Ht(H 1),
+// whose denominator does not depend on the image.
+__global__ void k_rl_scale(float* X, const float* Htf, const float* v, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float r = Htf[i] / v[i];
+        const float xn = X[i] * r;
+        X[i] = __builtin_isnan(xn) ? 0.f : xn;
+    }
+}
+
 // Fill with a constant (used instead of memset so that work stays capturable).
 __global__ void k_fill(float* p, long long n, float v) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -762,6 +774,14 @@ hipError_t configure_kernels(size_t max_lds) {
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t launch_rl_scale(float* X, const float* Htf, const float* v, long long n, hipStream_t s) {
+    long long blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_rl_scale, dim3((unsigned)blocks), dim3(256), 0, s, X, Htf, v, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t s) {

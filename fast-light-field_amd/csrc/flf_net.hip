@@ -416,7 +416,7 @@ int ensure_acts(flf_net* net, long long P) {
     if (P <= net->cap_P) return FLF_OK;
     free_acts(net);
     const int Cin = net->L[0].Cip, Wd = net->width, Cop = net->L.back().Cop;
-    const int maxC = std::max(Cin, Wd);
+    const int maxC = std::max(std::max(Cin, Wd), Cop);
     const long long Pp = (P + 15) / 16 * 16;
     auto& A = net->acts;
     int rc;

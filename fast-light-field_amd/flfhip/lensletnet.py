@@ -212,3 +212,74 @@ def train(net, imgs, tgts, steps, batch=32, lr=1e-3, seed=0, log_every=50, log=p
         if log and (s % log_every == 0 or s == steps - 1):
             log("step %d loss %.5f" % (s, loss))
     return hist
+
+
+WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
+
+
+def default_weights_path(nnum, nz, tag="ring"):
+    return os.path.join(WEIGHTS_DIR, "lensletnet_%s_n%d_z%d.npz" % (tag, nnum, nz))
+
+
+def _fit(projector, HX, im):
+    from .lfdeconv import _fit_from_sums
+    return _fit_from_sums(projector.datafit_device(HX, im))
+
+
+class LearnedInit(object):
+    """Guarded learned starting volume for Prevedel RL on one projector/HMatrix.
+
+    The guess is kept for a frame only if its forward-model data fit is no
+    worse than that of one plain RL iteration from the uniform start; else
+    that plain iterate is used.  The plain iterate X1 = Htf / Ht(H 1) needs no
+    projection once the image-independent Ht(H 1) is cached (one forward and
+    one backward projection per HMatrix and frame shape)."""
+
+    def __init__(self, net, projector, hMatrix):
+        if net.nnum != hMatrix.Nnum or net.nz != hMatrix.numZ:
+            raise ValueError("network (Nnum %d, %d planes) does not match the HMatrix (Nnum %d, %d planes)"
+                             % (net.nnum, net.nz, hMatrix.Nnum, hMatrix.numZ))
+        self.net, self.projector, self.hMatrix = net, projector, hMatrix
+        self._den, self._den_key = None, None
+
+    def plain_denominator(self, T, Y, X):
+        key = (T, Y, X)
+        if self._den_key != key:
+            p, hm = self.projector, self.hMatrix
+            ones = torch.ones((hm.numZ, T, Y, X), dtype=torch.float32, device=p.device)
+            H1 = p.forward_device(hm, ones)
+            del ones
+            self._den = p.backward_device(hm, H1)
+            self._den_key = key
+        return self._den
+
+    def initial_volume(self, im, Htf):
+        """im (T, Y, X), Htf (nz, T, Y, X) on the device.  Returns (X, HX,
+        info): the starting volume, its forward projection and per-frame
+        guard results (accepted, fit_guess, fit_plain1)."""
+        p, hm = self.projector, self.hMatrix
+        T, Y, Xw = im.shape
+        den = self.plain_denominator(T, Y, Xw)
+        g = self.net.volume_guess(im)
+        Hg = p.forward_device(hm, g)
+        fit_g = _fit(p, Hg, im)
+        X1 = torch.ones_like(Htf)
+        p.rl_scale_device(X1, Htf, den)
+        HX1 = p.forward_device(hm, X1)
+        fit_1 = _fit(p, HX1, im)
+        accept = fit_g <= fit_1
+        for t in np.nonzero(~accept)[0]:
+            g[:, t] = X1[:, t]
+            Hg[t] = HX1[t]
+        return g, Hg, {"accepted": accept, "fit_guess": fit_g, "fit_plain1": fit_1}
+
+    def deconvolve(self, im, Htf, maxIter, targetFit=None, fitOut=None):
+        """Guarded learned start, then Prevedel RL until every frame's fit is
+        <= targetFit (or maxIter iterations).  Returns (X, iterations, info);
+        a rejected frame's plain iterate counts as one iteration."""
+        from .lfdeconv import deconv_rl_device
+        X, HX, info = self.initial_volume(im, Htf)
+        n = deconv_rl_device(self.projector, self.hMatrix, Htf, maxIter, X, im=im, targetFit=targetFit,
+                             fitOut=fitOut, HX0=HX)
+        info["iterations"] = n + (0 if info["accepted"].all() else 1)
+        return X, info["iterations"], info

@@ -161,6 +161,15 @@ class Projector_hip(object):
         self._check(self.lib.flf_backward_rl_update(self._ctx, _ptr(img.contiguous()), _ptr(X), hp, tp, int(mode),
                                                     arr, len(planes), _stream_handle()), "flf_backward_rl_update")
 
+    def rl_scale_device(self, X, Htf, v):
+        """X *= Htf / v in place, NaN -> 0 (the Prevedel update with a known
+        denominator v; all three tensors of one shape)."""
+        if X.shape != Htf.shape or X.shape != v.shape:
+            raise ValueError("X, Htf and v must have one shape")
+        if not (X.is_contiguous() and Htf.is_contiguous() and v.is_contiguous()):
+            raise ValueError("X, Htf and v must be contiguous")
+        self._check(self.lib.flf_rl_scale(_ptr(Htf), _ptr(v), _ptr(X), X.numel(), _stream_handle()), "flf_rl_scale")
+
     def datafit_device(self, img, im):
         """Per-frame FP64 (sum((img-im)^2), sum(im^2))."""
         T = img.shape[0]

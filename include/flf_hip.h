@@ -97,6 +97,13 @@ int flf_backward_rl_update(flf_ctx* ctx, const float* img, float* X, const float
  * the convergence measure of the build's RL driver. Synchronises `stream`. */
 int flf_datafit(flf_ctx* ctx, const float* img, const float* im, double* out_host, void* stream);
 
+/* X[i] *= Htf[i] / v[i], NaN -> 0, for n elements on `stream`: the Prevedel
+ * RL update of lfdeconv.py:112-126 with a denominator the caller already
+ * holds.  With X = 1 and v = Ht(H 1) (which does not depend on the image) it
+ * gives the first plain RL iterate from a uniform start without projecting;
+ * the learned initialiser's guard compares its guess against that iterate. */
+int flf_rl_scale(const float* Htf, const float* v, float* X, long long n, void* stream);
+
 /* Kernel timing for roofline reporting (no reference counterpart; the
  * reference times whole calls with time.time(), lf_performance_analysis.py:
  * 259-268).  While enabled, every multiply-accumulate launch is bracketed by
