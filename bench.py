@@ -6,10 +6,16 @@ image), 41 planes z=-20..20, seeded ring PSFs with odd sides growing from 15
 to 345 px, 400 seeded Gaussian blobs on a 0.01 background, Poisson counts at
 peak 2000, 8 plain RL iterations (Prevedel update) from a uniform start.
 
-One step = one full reconstruction of one volume (8 RL iterations, i.e. 8
-forward + 8 fused backward/update projections) with all inputs resident in
-HBM.  value = volumes/s.  The 8-iteration data fit is the fixed target, so
-plain RL reaches it in exactly 8 iterations.
+One step = one full reconstruction of one volume from its image with all
+inputs resident in HBM: the back projection Htf = Ht im, then RL.  Two legs
+reach the same data fit ||H X - im|| / ||im||:
+  plain    8 Prevedel RL iterations from a uniform start (the reference's
+           DeconvRL with a uniform Xguess); its final fit is the target;
+  learned  the guarded learned initialiser (network guess, kept only if its
+           fit beats one plain iteration), then RL until the fit is <= the
+           target.
+value = volumes/s of the learned leg (of the plain leg when no trained
+weights are packaged, or with --plain-only).
 
 Also reported:
   roofline      the multiply-accumulate kernel (dominant): algorithmic bytes
@@ -31,32 +37,13 @@ sys.path.insert(0, os.path.join(ROOT, "fast-light-field_amd"))
 
 import numpy as np  # noqa: E402
 
-PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
+from flfhip.workloads import WORKLOADS, setup  # noqa: E402
 
-WORKLOADS = {
-    # name: (Nnum, lenslets, zmax, s_min, s_max, blobs, peak, iters, seed)
-    "rl1455": (15, 97, 20, 15, 345, 400, 2000.0, 8, 2),
-    "rl300": (15, 20, 5, 15, 135, 200, 1000.0, 4, 1),
-}
+PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def setup(name):
-    from flfhip import synthetic
-    N, L, zmax, smin, smax, nblobs, peak, iters, seed = WORKLOADS[name]
-    zs = list(range(-zmax, zmax + 1))
-    sides = synthetic.psf_sides(zs, smin, smax)
-    t = time.time()
-    hm = synthetic.make_psf_set(N, zs, sides, seed=seed)
-    log("psf set: %d planes, sides %d..%d, %.1fs" % (len(zs), min(sides), max(sides), time.time() - t))
-    Y = X = N * L
-    t = time.time()
-    vol = synthetic.blob_volume(len(zs), Y, X, nblobs, seed=seed)
-    log("volume: %s, %.1fs" % (vol.shape, time.time() - t))
-    return dict(N=N, Y=Y, X=X, zs=zs, sides=sides, hm=hm, vol=vol, peak=peak, iters=iters, seed=seed)
 
 
 def cpu_baseline(w, target_s=15.0):
@@ -78,10 +65,10 @@ def cpu_baseline(w, target_s=15.0):
     for s2 in w["sides"]:
         f = oshapes.convolution_shape((Y, X), (s2, s2), N, "5smooth")[0]
         total += dt * bins(f) / bins(f_s)
-    per_volume = total * 2 * w["iters"]          # forward + backward per iteration
+    per_volume = total * (2 * w["iters"] + 1)    # Htf, then forward + backward per iteration
     return {"value": 1.0 / per_volume, "unit": "volumes/s", "cores": 1, "kind": "port",
             "sample": "FP64 numpy oracle (project_special), forward projection of plane z=%d (s2=%d) at %dx%d, "
-                      "%.1f s, scaled by Fourier bins to %d planes x 2 directions x %d iterations"
+                      "%.1f s, scaled by Fourier bins to %d planes x (2 x %d iterations + 1) projections"
                       % (w["zs"][z], w["sides"][z], Y, X, dt, len(w["zs"]), w["iters"])}
 
 
@@ -91,6 +78,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="rl1455", choices=sorted(WORKLOADS))
+    ap.add_argument("--weights", default=None, help="learned-initialiser weights (default: the packaged file)")
+    ap.add_argument("--plain-only", action="store_true", help="time plain RL only")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (reference kernel for HBM counters)")
@@ -104,6 +93,7 @@ def main():
     import torch
     from flfhip.lfdeconv import deconv_rl_device, _fit_from_sums
     from flfhip.projector import Projector_hip
+    from flfhip import lensletnet, synthetic
 
     w = setup(args.workload)
     hm, Y, X, iters = w["hm"], w["Y"], w["X"], w["iters"]
@@ -118,15 +108,16 @@ def main():
 
     vol = proj.asnative(w["vol"])
     clean = proj.forward_device(hm, vol)
-    from flfhip import synthetic
     im = proj.asnative(synthetic.poisson_image(proj.asnumpy(clean), w["peak"], w["seed"]))
     del vol, clean
-    Htf = proj.backward_device(hm, im)
+    Htf = torch.empty((nz, 1, Y, X), dtype=torch.float32, device=proj.device)
     Xv = torch.empty_like(Htf)
 
-    def step():
+    # One step = one reconstruction from the image: Htf = Ht im, then RL.
+    def plain_step():
+        proj.backward_device(hm, im, out=Htf)
         Xv.fill_(1.0)
-        deconv_rl_device(proj, hm, Htf, iters, Xv, im=im)
+        deconv_rl_device(proj, hm, Htf, iters, Xv)
 
     if args.calib_copy_mib:
         n = args.calib_copy_mib * (1 << 20) // 4
@@ -135,28 +126,63 @@ def main():
         b_.copy_(a_)
         torch.cuda.synchronize()
         del a_, b_
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    proj.profile(True)
+
     s = torch.cuda.current_stream()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.time()
-    e0.record(s)
-    for _ in range(args.steps):
-        step()
-    e1.record(s)
-    torch.cuda.synchronize()
-    wall = time.time() - t0
+
+    def timed(step, n):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(s)
+        for _ in range(n):
+            step()
+        e1.record(s)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    # plain RL: the reference's DeconvRL from a uniform Xguess, fixed iterations
+    for _ in range(args.warmup):
+        plain_step()
+    proj.profile(True)
+    plain_ms = timed(plain_step, args.steps)
     proj.profile(False)
     mac_ms, mac_n, mac_bytes = proj.profile_read()
-    ms_step = e0.elapsed_time(e1) / args.steps
-    log("wall %.3fs for %d steps; event %.2f ms/step" % (wall, args.steps, ms_step))
+    fit_plain = float(_fit_from_sums(proj.datafit_device(proj.forward_device(hm, Xv), im))[0])
+    X_plain = Xv.clone()
+    log("plain RL: %.2f ms/volume, data fit %.6f after %d iterations" % (plain_ms, fit_plain, iters))
 
-    # Data fit reached (the fixed target of the metric).
-    fit = _fit_from_sums(proj.datafit_device(proj.forward_device(hm, Xv), im))
+    learned = None
+    wpath = args.weights or lensletnet.default_weights_path(w["N"], nz, args.workload)
+    if not args.plain_only and not os.path.exists(wpath):
+        log("no learned-initialiser weights at %s: timing plain RL only" % wpath)
+    if not args.plain_only and os.path.exists(wpath):
+        net = lensletnet.LensletNet.load(wpath)
+        li = lensletnet.LearnedInit(net, proj, hm)
+        li.plain_denominator(1, Y, X)          # cached per PSF set and frame shape, like the spectra
+        out = {}
+
+        def learned_step():
+            proj.backward_device(hm, im, out=Htf)
+            Xl, n_it, info = li.deconvolve(im, Htf, 4 * iters, targetFit=fit_plain)
+            out.update(X=Xl, n=n_it, info=info)
+
+        for _ in range(args.warmup):
+            learned_step()
+        proj.profile(True)
+        learned_ms = timed(learned_step, args.steps)
+        proj.profile(False)
+        lm_ms, lm_n, lm_bytes = proj.profile_read()
+        mac_ms, mac_n, mac_bytes = mac_ms + lm_ms, mac_n + lm_n, mac_bytes + lm_bytes
+        info = out["info"]
+        fit_l = float(_fit_from_sums(proj.datafit_device(proj.forward_device(hm, out["X"]), im))[0])
+        rel = float(torch.linalg.vector_norm((out["X"] - X_plain).double()) / torch.linalg.vector_norm(X_plain.double()))
+        learned = {"ms": learned_ms, "iterations": int(out["n"]), "guess_accepted": bool(info["accepted"][0]),
+                   "fit_guess": float(info["fit_guess"][0]), "fit_plain1": float(info["fit_plain1"][0]),
+                   "datafit_reached": fit_l, "rel_l2_vs_plain": rel, "weights": os.path.relpath(wpath, ROOT)}
+        log("learned init: %.2f ms/volume, %d iterations, fit %.6f (guess %.5f vs one plain iteration %.5f)"
+            % (learned_ms, out["n"], fit_l, info["fit_guess"][0], info["fit_plain1"][0]))
+
+    ms_step = learned["ms"] if learned else plain_ms
     # Whole-iteration roofline: S_H + S_Ht + B*(4*nz+2)*Y*X*4 bytes.
     s_dir = 0.0
     for z in range(nz):
@@ -164,9 +190,12 @@ def main():
         u = 36 if fy == fx else 64
         s_dir += u * fy * (fx // 2 + 1) * 8
     it_bytes = 2 * s_dir + 1 * (4 * nz + 2) * Y * X * 4
-    it_s = ms_step / 1e3 / iters
+    it_s = (plain_ms / 1e3) / (iters + 0.5)      # Htf is half an iteration
     mac_avg_ms = mac_ms / max(mac_n, 1)
     achieved = (mac_bytes / max(mac_n, 1)) / (mac_avg_ms / 1e3)
+    desc = ("Nnum=15, %dx%d px, %d planes (z=-%d..%d), PSF %d..%d px, %d blobs, Poisson peak %g, 1 frame; "
+            "target = data fit of %d plain RL iterations from a uniform start"
+            % (Y, X, nz, w["zs"][-1], w["zs"][-1], min(w["sides"]), max(w["sides"]), w["nblobs"], w["peak"], iters))
     res = {
         "metric": "volumes/s at fixed RL data-fit: learned-init vs plain RL, 1 GPU",
         "value": 1e3 / ms_step,
@@ -180,11 +209,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "plain RL, Nnum=15, %dx%d px, %d planes (z=-%d..%d), PSF %d..%d px, %d blobs, "
-                               "Poisson peak %g, %d iterations from uniform start, 1 frame"
-                               % (Y, X, nz, w["zs"][-1], w["zs"][-1], min(w["sides"]), max(w["sides"]),
-                                  WORKLOADS[args.workload][5], w["peak"], iters),
-                   "init": "uniform (plain RL)", "iterations": iters, "datafit_reached": float(fit[0])},
+        "config": {"workload": desc, "init": "learned (guarded)" if learned else "uniform (plain RL)",
+                   "target_datafit": fit_plain,
+                   "plain": {"ms": plain_ms, "volumes_per_s": 1e3 / plain_ms, "iterations": iters},
+                   "learned": learned,
+                   "speedup_vs_plain": (plain_ms / learned["ms"]) if learned else None},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM, "traffic": None,
                      "kernel": "k_mac", "launch_avg_ms": mac_avg_ms, "launches": mac_n,

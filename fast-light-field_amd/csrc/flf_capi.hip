@@ -64,6 +64,7 @@ struct flf_ctx {
     double* d_fitout = nullptr;
     size_t max_lds = 65536;
     int ipt = 1;                      // items per MAC thread (FLF_ITEMS_PER_THREAD overrides)
+    int fb_max = 2;                   // frames per MAC workgroup at T >= 2 (FLF_MAC_FRAMES overrides)
     long long bytes = 0;
     std::vector<void*> allocs;
     // Optional timing of the multiply-accumulate launches (flf_profile_*).
@@ -211,7 +212,11 @@ int run_mac(flf_ctx* ctx, const MacArgs& m, const Group& g, int nplanes_grid, hi
         eb = take_event(ctx);
         if (ea && eb) HIPCHK(hipEventRecord(ea, st));
     }
-    HIPCHK(launch_mac(m, gt.slots, gt.ipt, (int)gt.chunks.size(), nplanes_grid, ctx->T, gt.max_res, st));
+    // Frames per workgroup: each spectrum value read serves fb frames.
+    int fb = 1;
+    while (fb * 2 <= ctx->fb_max && fb * 2 <= ctx->T && mac_lds(gt.slots, gt.nu, ctx->N, fb * 2) <= ctx->max_lds)
+        fb *= 2;
+    HIPCHK(launch_mac(m, gt.slots, gt.ipt, fb, (int)gt.chunks.size(), nplanes_grid, st));
     if (ctx->prof && ea && eb) {
         HIPCHK(hipEventRecord(eb, st));
         int np = 0;
@@ -276,6 +281,7 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
         m.A_tstride = (long long)gt.fY * gt.rfX;
         m.A_pstride = (long long)T * gt.fY * gt.rfX;
         m.N = N; m.fY = gt.fY; m.fX = gt.fX; m.rY = gt.rY; m.rX = gt.rX; m.rfX = gt.rfX; m.nu = gt.nu;
+        m.T = T;
         rc = run_mac(ctx, m, g, m.nP, st);
         if (rc) return rc;
         IfftColArgs ic{};
@@ -333,7 +339,11 @@ int flf_create(int device, flf_ctx** out) {
     if (configure_kernels(c->max_lds) != hipSuccess) c->max_lds = 65536;
     if (const char* e = getenv("FLF_ITEMS_PER_THREAD")) {
         int v = atoi(e);
-        if (v == 1 || v == 2 || v == 4) c->ipt = v;
+        if (v == 1 || v == 2) c->ipt = v;
+    }
+    if (const char* e = getenv("FLF_MAC_FRAMES")) {
+        int v = atoi(e);
+        if (v == 1 || v == 2 || v == 4) c->fb_max = v;
     }
     *out = c;
     return FLF_OK;
@@ -594,6 +604,7 @@ int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, i
         m.A_tstride = (long long)gt.fY * gt.rfX;
         m.A_pstride = 0;
         m.N = N; m.fY = gt.fY; m.fX = gt.fX; m.rY = gt.rY; m.rX = gt.rX; m.rfX = gt.rfX; m.nu = gt.nu;
+        m.T = T;
         rc = run_mac(ctx, m, g, 1, st);
         if (rc) return rc;
         IfftColArgs ic{};

@@ -251,25 +251,28 @@ __device__ __forceinline__ void load_g(float2 (&g)[4], const float2* p) {
 // j, source slot i and item a of the thread (the items share residues, so
 // each 16-byte LDS entry (P, M) serves all of them).
 // P = s_plain + s_conj, M = i (s_plain - s_conj).
-template <int SLOTS, int IPT>
-__device__ __forceinline__ void mac_spectrum(float2 (&acc)[IPT][4], const float2 (&g)[IPT][4],
-                                             const float4* __restrict__ pm, const int (&rb)[4], int ucol) {
+template <int SLOTS, int IPT, int FB>
+__device__ __forceinline__ void mac_spectrum(float2 (&acc)[FB][IPT][4], const float2 (&g)[IPT][4],
+                                             const float4* __restrict__ pm, int tab, const int (&rb)[4], int ucol) {
     constexpr int NPT = SLOTS == 4 ? 4 : 2;
 #pragma unroll
-    for (int j = 0; j < SLOTS; ++j) {
-        const float4* row = pm + (rb[j] + ucol) * NPT;
+    for (int f = 0; f < FB; ++f) {
 #pragma unroll
-        for (int i = 0; i < SLOTS; ++i) {
-            const int pt = SLOTS == 4 ? kQuadPT[j][i] : kPairPT[j][i];
-            const float sg = SLOTS == 4 ? (float)kQuadSG[j][i] : 1.0f;
-            const float4 q = row[pt];          // (P.x, P.y, M.x, M.y)
-            const float mx = sg * q.z, my = sg * q.w;
+        for (int j = 0; j < SLOTS; ++j) {
+            const float4* row = pm + f * tab + (rb[j] + ucol) * NPT;
 #pragma unroll
-            for (int a = 0; a < IPT; ++a) {
-                acc[a][j].x = fmaf(q.x, g[a][i].x, acc[a][j].x);
-                acc[a][j].y = fmaf(q.y, g[a][i].x, acc[a][j].y);
-                acc[a][j].x = fmaf(mx, g[a][i].y, acc[a][j].x);
-                acc[a][j].y = fmaf(my, g[a][i].y, acc[a][j].y);
+            for (int i = 0; i < SLOTS; ++i) {
+                const int pt = SLOTS == 4 ? kQuadPT[j][i] : kPairPT[j][i];
+                const float sg = SLOTS == 4 ? (float)kQuadSG[j][i] : 1.0f;
+                const float4 q = row[pt];          // (P.x, P.y, M.x, M.y)
+                const float mx = sg * q.z, my = sg * q.w;
+#pragma unroll
+                for (int a = 0; a < IPT; ++a) {
+                    acc[f][a][j].x = fmaf(q.x, g[a][i].x, acc[f][a][j].x);
+                    acc[f][a][j].y = fmaf(q.y, g[a][i].x, acc[f][a][j].y);
+                    acc[f][a][j].x = fmaf(mx, g[a][i].y, acc[f][a][j].x);
+                    acc[f][a][j].y = fmaf(my, g[a][i].y, acc[f][a][j].y);
+                }
             }
         }
     }
@@ -294,11 +297,12 @@ __device__ __forceinline__ void mac_spectrum(float2 (&acc)[IPT][4], const float2
 // This is the accumulation of py_light_field.cpp:303-341, 1029-1087 with the
 // mirror multipliers of projector.py:137-151 and the 'same' crop
 // (projector.py:792-796) folded into phases.  Each thread handles IPT
-// consecutive items that share their residues.
-template <int SLOTS, int IPT>
-__global__ __launch_bounds__(FLF_CHUNK_MAX) __attribute__((amdgpu_waves_per_eu(IPT >= 4 ? 2 : FLF_MAC_WAVES)))
+// consecutive items that share their residues, for FB frames: every stored
+// spectrum value read from HBM serves FB frames (grid z = frame blocks).
+template <int SLOTS, int IPT, int FB>
+__global__ __launch_bounds__(FLF_CHUNK_MAX) __attribute__((amdgpu_waves_per_eu(IPT * FB >= 4 ? 2 : (IPT * FB == 2 ? 3 : FLF_MAC_WAVES))))
 void k_mac(MacArgs a) {
-    extern __shared__ float4 pm_tab[];            // [NRES][nu][NPT], then staging
+    extern __shared__ float4 pm_tab[];            // [FB][NRES][nu][NPT], then staging
     constexpr int NUSE = SLOTS == 4 ? 8 : 4;
     constexpr int NPT = SLOTS == 4 ? 4 : 2;
     constexpr int NRES = SLOTS == 4 ? 8 : 4;      // residues per orbit (canonical list)
@@ -311,9 +315,11 @@ void k_mac(MacArgs a) {
     const bool valid = tid * IPT < n_items;
     const int tsafe = valid ? tid : 0;           // idle lanes read item 0 (in bounds) and never write
     const int N = a.N, NN = N * N, c = N / 2, fourN = 4 * N;
-    const int t = blockIdx.z;
+    const int t0 = blockIdx.z * FB;
+    const int nf = min(FB, a.T - t0);            // frames of this block (block-uniform)
     const int nu = a.nu;
-    float2* s_stage = reinterpret_cast<float2*>(pm_tab + NRES * nu * NPT);   // [NN][NH]
+    const int tab = NRES * nu * NPT;              // table entries per frame
+    float2* s_stage = reinterpret_cast<float2*>(pm_tab + FB * tab);   // [NN][NH]
     float2* psi = s_stage + NN * NH;                                        // [NRES][2]
     ItemDesc it[IPT];
 #pragma unroll
@@ -322,24 +328,26 @@ void k_mac(MacArgs a) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) rb[j] = it[0].res[j] * nu;
 
-    float2 tot[IPT][4];
+    float2 tot[FB][IPT][4];
 #pragma unroll
-    for (int q = 0; q < IPT; ++q)
+    for (int f = 0; f < FB; ++f)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) tot[q][j] = make_float2(0.f, 0.f);
+        for (int q = 0; q < IPT; ++q)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) tot[f][q][j] = make_float2(0.f, 0.f);
 
     const int pb = a.plane_loop ? 0 : blockIdx.y;
     const int pe = a.plane_loop ? a.nP : blockIdx.y + 1;
     const long long ustride = (long long)n_items * SLOTS;
     const int nb = (nu + UNR - 1) / UNR;          // spectrum batches per plane
 
-    // Per-plane set-up: stage the chunk's small-FFT block, flip factors and
-    // the P/M table (all threads; contains barriers).
+    // Per-plane set-up: flip factors, then per frame stage the chunk's
+    // small-FFT block and build its P/M table (all threads; contains
+    // barriers; frames past nf keep the tables of earlier planes and their
+    // accumulators are never stored).
     auto prologue = [&](int p) {
         const int s2 = a.s2[p];
         __syncthreads();
-        const float2* Sp = a.S + a.S_off[p] + (long long)t * a.S_tstride + (long long)blockIdx.x * NN * NH;
-        for (int i = tid; i < NN * NH; i += blockDim.x) s_stage[i] = Sp[i];
         // flip factors per residue: exp(-2 pi i (s2-1) kt / f)
         if (tid < 2 * NRES) {
             const int rho = tid >> 1;
@@ -354,8 +362,13 @@ void k_mac(MacArgs a) {
             sincospif(-2.0f * (float)m / (float)f, &sn, &cs);
             psi[tid] = make_float2(cs, sn);
         }
+        for (int f = 0; f < nf; ++f) {
+        if (f > 0) __syncthreads();
+        const float2* Sp = a.S + a.S_off[p] + (long long)(t0 + f) * a.S_tstride + (long long)blockIdx.x * NN * NH;
+        for (int i = tid; i < NN * NH; i += blockDim.x) s_stage[i] = Sp[i];
         __syncthreads();
-        for (int idx = tid; idx < NRES * nu * NPT; idx += blockDim.x) {
+        float4* pmf = pm_tab + f * tab;
+        for (int idx = tid; idx < tab; idx += blockDim.x) {
             const int pt = idx % NPT;
             const int ru = idx / NPT;
             const int u = ru % nu, rho = ru / nu;
@@ -376,7 +389,8 @@ void k_mac(MacArgs a) {
             }
             const float2 P = cadd(sv[0], sv[1]);
             const float2 d = csub(sv[0], sv[1]);
-            pm_tab[idx] = make_float4(P.x, P.y, -d.y, d.x);     // M = i * d
+            pmf[idx] = make_float4(P.x, P.y, -d.y, d.x);        // M = i * d
+        }
         }
         __syncthreads();
     };
@@ -395,7 +409,7 @@ void k_mac(MacArgs a) {
         }
     };
 
-    float2 acc[IPT][4];
+    float2 acc[FB][IPT][4];
     // Epilogue of plane p: outer phase and the 'same'-crop shift.
     auto epilogue = [&](int p) {
         const int s2 = a.s2[p];
@@ -412,7 +426,7 @@ void k_mac(MacArgs a) {
 #pragma unroll
             for (int j = 0; j < SLOTS; ++j) {
                 const int k0 = it[q].pos[j] / a.rfX, k1 = it[q].pos[j] - k0 * a.rfX;
-                float2 v = cmul(acc[q][j], phase_frac(outer[j], fourN));
+                const float2 po = phase_frac(outer[j], fourN);
                 // exp(+2 pi i c2 (k0/fY + k1/fX))
                 const int e0 = (int)(((long long)c2 * k0) % a.fY);
                 const int e1 = (int)(((long long)c2 * k1) % a.fX);
@@ -420,7 +434,9 @@ void k_mac(MacArgs a) {
                 fr -= rintf(fr);
                 float sn, cs;
                 sincospif(2.0f * fr, &sn, &cs);
-                tot[q][j] = cadd(tot[q][j], cmul(v, make_float2(cs, sn)));
+                const float2 pc = make_float2(cs, sn);
+#pragma unroll
+                for (int f = 0; f < FB; ++f) tot[f][q][j] = cadd(tot[f][q][j], cmul(cmul(acc[f][q][j], po), pc));
             }
         }
     };
@@ -438,9 +454,11 @@ void k_mac(MacArgs a) {
             if (b == 0) {
                 prologue(p);
 #pragma unroll
-                for (int q = 0; q < IPT; ++q)
+                for (int f = 0; f < FB; ++f)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) acc[q][j] = make_float2(0.f, 0.f);
+                    for (int q = 0; q < IPT; ++q)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) acc[f][q][j] = make_float2(0.f, 0.f);
             }
             int pn = p, bn = b + 1;
             if (bn == nb) {
@@ -451,7 +469,7 @@ void k_mac(MacArgs a) {
             if (pn < pe) load_batch(nxt, pn, bn);
 #pragma unroll
             for (int k = 0; k < UNR; ++k)
-                if (b * UNR + k < nu) mac_spectrum<SLOTS, IPT>(acc, cur[k], pm_tab, rb, b * UNR + k);
+                if (b * UNR + k < nu) mac_spectrum<SLOTS, IPT, FB>(acc, cur[k], pm_tab, tab, rb, b * UNR + k);
             if (b == nb - 1) epilogue(p);
             p = pn;
             b = bn;
@@ -462,12 +480,17 @@ void k_mac(MacArgs a) {
     }
 
     if (valid) {
-        float2* A = a.A + (long long)t * a.A_tstride + (a.plane_loop ? 0 : (long long)blockIdx.y * a.A_pstride);
 #pragma unroll
-        for (int q = 0; q < IPT; ++q)
+        for (int f = 0; f < FB; ++f) {
+            if (f >= nf) break;
+            float2* A = a.A + (long long)(t0 + f) * a.A_tstride +
+                        (a.plane_loop ? 0 : (long long)blockIdx.y * a.A_pstride);
 #pragma unroll
-            for (int j = 0; j < SLOTS; ++j)
-                if ((it[q].writemask >> j) & 1) A[it[q].pos[j]] = tot[q][j];
+            for (int q = 0; q < IPT; ++q)
+#pragma unroll
+                for (int j = 0; j < SLOTS; ++j)
+                    if ((it[q].writemask >> j) & 1) A[it[q].pos[j]] = tot[f][q][j];
+        }
     }
 }
 
@@ -721,22 +744,28 @@ hipError_t launch_spec_pack(const SpecPackArgs& a, int nchunks, hipStream_t s) {
     hipLaunchKernelGGL(k_spec_pack, dim3(nchunks, a.nu), dim3(FLF_CHUNK_MAX), 0, s, a);
     return hipGetLastError();
 }
-hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int nchunks, int nplanes_grid, int T, int max_res,
-                      hipStream_t s) {
+size_t mac_lds(int slots, int nu, int N, int fb) {
     const int nres = slots == 4 ? 8 : 4;
-    size_t lds = (size_t)nres * a.nu * (slots == 4 ? 4 : 2) * sizeof(float4) +
-                 ((size_t)a.N * a.N * (nres / 2) + 2 * nres) * sizeof(float2);
-    (void)max_res;
-    dim3 grid(nchunks, nplanes_grid, T);
+    return (size_t)fb * nres * nu * (slots == 4 ? 4 : 2) * sizeof(float4) +
+           ((size_t)N * N * (nres / 2) + 2 * nres) * sizeof(float2);
+}
+template <int SLOTS, int IPT>
+static void launch_mac_fb(const MacArgs& a, int fb, dim3 grid, dim3 block, size_t lds, hipStream_t s) {
+    if (fb == 1) hipLaunchKernelGGL((k_mac<SLOTS, IPT, 1>), grid, block, lds, s, a);
+    else if (fb == 2) hipLaunchKernelGGL((k_mac<SLOTS, IPT, 2>), grid, block, lds, s, a);
+    else hipLaunchKernelGGL((k_mac<SLOTS, IPT, 4>), grid, block, lds, s, a);
+}
+hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks, int nplanes_grid, hipStream_t s) {
+    if ((fb != 1 && fb != 2 && fb != 4) || (ipt != 1 && ipt != 2) || a.T < 1) return hipErrorInvalidValue;
+    const size_t lds = mac_lds(slots, a.nu, a.N, fb);
+    dim3 grid(nchunks, nplanes_grid, (a.T + fb - 1) / fb);
     dim3 block(FLF_CHUNK_MAX / ipt);
     if (slots == 4) {
-        if (ipt == 1) hipLaunchKernelGGL((k_mac<4, 1>), grid, block, lds, s, a);
-        else if (ipt == 2) hipLaunchKernelGGL((k_mac<4, 2>), grid, block, lds, s, a);
-        else hipLaunchKernelGGL((k_mac<4, 4>), grid, block, lds, s, a);
+        if (ipt == 1) launch_mac_fb<4, 1>(a, fb, grid, block, lds, s);
+        else launch_mac_fb<4, 2>(a, fb, grid, block, lds, s);
     } else {
-        if (ipt == 1) hipLaunchKernelGGL((k_mac<2, 1>), grid, block, lds, s, a);
-        else if (ipt == 2) hipLaunchKernelGGL((k_mac<2, 2>), grid, block, lds, s, a);
-        else hipLaunchKernelGGL((k_mac<2, 4>), grid, block, lds, s, a);
+        if (ipt == 1) launch_mac_fb<2, 1>(a, fb, grid, block, lds, s);
+        else launch_mac_fb<2, 2>(a, fb, grid, block, lds, s);
     }
     return hipGetLastError();
 }
@@ -767,8 +796,10 @@ hipError_t launch_fit(const float* img, const float* im, long long n_per_frame, 
 hipError_t configure_kernels(size_t max_lds) {
     const void* ks[] = {(const void*)k_small_fft, (const void*)k_psf_rows, (const void*)k_psf_cols,
                         (const void*)k_ifft_cols, (const void*)k_ifft_rows_sum, (const void*)k_ifft_rows_plane,
-                        (const void*)k_mac<4, 1>, (const void*)k_mac<4, 2>, (const void*)k_mac<4, 4>,
-                        (const void*)k_mac<2, 1>, (const void*)k_mac<2, 2>, (const void*)k_mac<2, 4>};
+                        (const void*)k_mac<4, 1, 1>, (const void*)k_mac<4, 1, 2>, (const void*)k_mac<4, 1, 4>,
+                        (const void*)k_mac<4, 2, 1>, (const void*)k_mac<4, 2, 2>, (const void*)k_mac<4, 2, 4>,
+                        (const void*)k_mac<2, 1, 1>, (const void*)k_mac<2, 1, 2>, (const void*)k_mac<2, 1, 4>,
+                        (const void*)k_mac<2, 2, 1>, (const void*)k_mac<2, 2, 2>, (const void*)k_mac<2, 2, 4>};
     for (const void* k : ks) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
         if (e != hipSuccess) return e;

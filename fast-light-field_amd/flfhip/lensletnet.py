@@ -168,11 +168,13 @@ def blob_volumes_gpu(nz, Y, X, T, nblobs_range, seed, device, background=0.01, s
     return vol
 
 
-def make_pairs(proj, hMatrix, n, L, seed, peak=(500.0, 5000.0), iters=8, batch=16, nblobs=(50, 300)):
+def make_pairs(proj, hMatrix, n, L, seed, peak=(500.0, 5000.0), iters=8, batch=16, nblobs=(50, 300), log=None):
     """n training pairs at L x L lenslets: (images (n, Y, X), targets
     (n, nz, L, L)) on the GPU, targets = lenslet means of plain RL after
     `iters` iterations from a uniform start."""
     from .lfdeconv import deconv_rl_device
+    if n % batch:
+        raise ValueError("the pair count must be a multiple of the batch (one projector plan per batch size)")
     N = hMatrix.Nnum
     Y = X = L * N
     nz = hMatrix.numZ
@@ -196,6 +198,8 @@ def make_pairs(proj, hMatrix, n, L, seed, peak=(500.0, 5000.0), iters=8, batch=1
         imgs[b0:b0 + T] = im
         tgts[b0:b0 + T] = lenslet_means(Xv, N)
         del vol, clean, Htf, Xv
+        if log and (b0 // batch) % 16 == 0:
+            log("pairs %d / %d" % (b0 + T, n))
     return imgs, tgts
 
 
@@ -217,7 +221,7 @@ def train(net, imgs, tgts, steps, batch=32, lr=1e-3, seed=0, log_every=50, log=p
 WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
 
 
-def default_weights_path(nnum, nz, tag="ring"):
+def default_weights_path(nnum, nz, tag):
     return os.path.join(WEIGHTS_DIR, "lensletnet_%s_n%d_z%d.npz" % (tag, nnum, nz))
 
 

@@ -1,0 +1,45 @@
+"""Synthetic workloads of the benchmark and of the learned initialiser's
+training: seeded ring PSF sets (synthetic.make_psf_set) and blob volumes.
+
+    name     Nnum  lenslets  planes      PSF sides   blobs  Poisson peak  RL iterations
+    rl1455   15    97x97     z=-20..20   15..345     400    2000          8
+    rl300    15    20x20     z=-5..5     15..135     200    1000          4
+"""
+import sys
+import time
+
+from . import synthetic
+
+WORKLOADS = {
+    # name: (Nnum, lenslets, zmax, s_min, s_max, blobs, peak, iters, seed)
+    "rl1455": (15, 97, 20, 15, 345, 400, 2000.0, 8, 2),
+    "rl300": (15, 20, 5, 15, 135, 200, 1000.0, 4, 1),
+}
+
+
+def _log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def psf_set(name):
+    """(HMatrix, zs, sides) of a workload."""
+    N, L, zmax, smin, smax, nblobs, peak, iters, seed = WORKLOADS[name]
+    zs = list(range(-zmax, zmax + 1))
+    sides = synthetic.psf_sides(zs, smin, smax)
+    t = time.time()
+    hm = synthetic.make_psf_set(N, zs, sides, seed=seed)
+    _log("psf set: %d planes, sides %d..%d, %.1fs" % (len(zs), min(sides), max(sides), time.time() - t))
+    return hm, zs, sides
+
+
+def setup(name, volume=True):
+    N, L, zmax, smin, smax, nblobs, peak, iters, seed = WORKLOADS[name]
+    hm, zs, sides = psf_set(name)
+    Y = X = N * L
+    vol = None
+    if volume:
+        t = time.time()
+        vol = synthetic.blob_volume(len(zs), Y, X, nblobs, seed=seed)
+        _log("volume: %s, %.1fs" % (vol.shape, time.time() - t))
+    return dict(name=name, N=N, Y=Y, X=X, zs=zs, sides=sides, hm=hm, vol=vol, peak=peak, iters=iters, seed=seed,
+                nblobs=nblobs)

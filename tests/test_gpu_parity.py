@@ -53,3 +53,22 @@ def test_small_direct(projector, N, Y, X, s2):
     got = projector.ForwardProjectACC(hm, x, [0])
     ref = oproj.project_direct(x[0], opsf.expand_quadrant(hm.Hcc(0, False).astype(np.float64)))
     assert rel_l2(got, ref) < 1e-5
+
+
+# Several frames per call: the multiply-accumulate kernel serves two frames per
+# workgroup from one read of the spectra; T = 3 leaves a ragged last block.
+@pytest.mark.parametrize("T,Y,X", [(3, 150, 150), (3, 150, 300), (2, 105, 120)])
+def test_multiframe(projector, T, Y, X):
+    hm = _hm_random(15, [31, 45], seed=T + Y)
+    rng = np.random.default_rng(5)
+    vol = rng.random((2, T, Y, X)).astype(np.float32)
+    img = rng.random((T, Y, X)).astype(np.float32)
+    print("multiframe", T, Y, X, flush=True)
+    fwd = projector.ForwardProjectACC(hm, vol, [0, 1])
+    bwd = projector.BackwardProjectACC(hm, img, [0, 1])
+    for t in range(T):
+        ref_f = sum(oproj.project_special(vol[z, t][None], hm.Hcc(z, False))[0] for z in range(2))
+        assert rel_l2(fwd[t], ref_f) < 1e-5, t
+        for z in range(2):
+            ref_b = oproj.project_special(img[t][None], hm.Hcc(z, True))[0]
+            assert rel_l2(bwd[z, t], ref_b) < 1e-5, (t, z)
