@@ -295,11 +295,11 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
         HIPCHK(launch_ifft_cols(ic, m.nP * T, st));
     }
     std::vector<int> ids;
-    int maxfX = 0;
+    int wbuf = 0;
     for (int z = 0; z < ctx->nz; ++z)
         if (sel[z]) {
             ids.push_back(z);
-            maxfX = std::max(maxfX, ctx->groups[ctx->group_of[z]].gt.fX);
+            wbuf = std::max(wbuf, c2r_wave_buffer(ctx->groups[ctx->group_of[z]].gt.fX));
         }
     if (ids.empty()) return FLF_OK;
     if (ids != ctx->plane_ids_host) {
@@ -313,8 +313,8 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
     r.Tbuf = ctx->d_T_bwd;
     r.vol = vol; r.Htf = Htf; r.trust = trust;
     r.mode = mode; r.T = T; r.Y = ctx->Y; r.X = ctx->X;
-    r.maxfX = maxfX;
-    HIPCHK(launch_ifft_rows_plane(r, (int)ids.size(), maxfX, st));
+    r.wbuf = wbuf;
+    HIPCHK(launch_ifft_rows_plane(r, (int)ids.size(), st));
     return FLF_OK;
 }
 

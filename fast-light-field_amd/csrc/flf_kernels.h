@@ -119,14 +119,15 @@ struct IfftRowPlaneArgs {
     const float* trust;          // mode 2 (optional, (nz, 1, Y, X))
     int mode;                    // 0 store, 1 Prevedel update, 2 standard update
     int T, Y, X;
-    int maxfX;                   // largest fX of the processed planes (LDS sizing)
+    int wbuf;                    // per-wave LDS buffer (float2): max over planes of fX/2 (even) or fX
 };
 
 namespace flf {
 size_t small_fft_lds(int rY, int rX);
 size_t ifft_cols_lds(int CB, int fY);
 size_t ifft_rows_sum_lds(int maxfX);
-size_t ifft_rows_plane_lds(int maxfX);
+size_t ifft_rows_plane_lds(int wbuf);
+int c2r_wave_buffer(int fX);
 hipError_t launch_small_fft(const SmallFFTArgs& a, int nsrc, int T, hipStream_t s);
 hipError_t launch_psf_rows(const PsfRowArgs& a, int nu, hipStream_t s);
 hipError_t launch_psf_cols(const PsfColArgs& a, int nu, hipStream_t s);
@@ -135,7 +136,7 @@ size_t mac_lds(int slots, int nu, int N, int fb);
 hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks, int nplanes_grid, hipStream_t s);
 hipError_t launch_ifft_cols(const IfftColArgs& a, int nbatch, hipStream_t s);
 hipError_t launch_ifft_rows_sum(const IfftRowSumArgs& a, int Y, int T, hipStream_t s);
-hipError_t launch_ifft_rows_plane(const IfftRowPlaneArgs& a, int nplanes, int maxfX, hipStream_t s);
+hipError_t launch_ifft_rows_plane(const IfftRowPlaneArgs& a, int nplanes, hipStream_t s);
 hipError_t launch_fit(const float* img, const float* im, long long n_per_frame, int T, double* part, int nblk,
                       double* out, hipStream_t s);
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t s);

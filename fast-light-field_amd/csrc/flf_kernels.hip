@@ -626,7 +626,7 @@ __global__ __launch_bounds__(256) void k_ifft_rows_plane(IfftRowPlaneArgs a) {
     const int z = a.plane_ids[blockIdx.z];
     const RowPlaneDesc& pd = a.pd[z];           // read in place (no private copy)
     const int L = pd.rp.L;
-    float2* buf = lds + wv * a.maxfX;
+    float2* buf = lds + wv * a.wbuf;
     if (y >= a.Y) return;                       // no block-level barrier below
     const float2* row = a.Tbuf + pd.T_off + (long long)t * pd.T_tstride + (long long)y * pd.rfX;
     wave_c2r(buf, row, L, pd.rph, pd.twh, pd.rp, pd.twf, lane);
@@ -781,9 +781,11 @@ hipError_t launch_ifft_rows_sum(const IfftRowSumArgs& a, int Y, int T, hipStream
     hipLaunchKernelGGL(k_ifft_rows_sum, dim3((Y + 3) / 4, T), dim3(256), lds, s, a);
     return hipGetLastError();
 }
-size_t ifft_rows_plane_lds(int maxfX) { return 4 * (size_t)maxfX * sizeof(float2); }
-hipError_t launch_ifft_rows_plane(const IfftRowPlaneArgs& a, int nplanes, int maxfX, hipStream_t s) {
-    size_t lds = ifft_rows_plane_lds(maxfX);
+// c2r of an even length runs as a half-length complex FFT in place.
+int c2r_wave_buffer(int fX) { return (fX & 1) ? fX : fX / 2; }
+size_t ifft_rows_plane_lds(int wbuf) { return 4 * (size_t)wbuf * sizeof(float2); }
+hipError_t launch_ifft_rows_plane(const IfftRowPlaneArgs& a, int nplanes, hipStream_t s) {
+    size_t lds = ifft_rows_plane_lds(a.wbuf);
     hipLaunchKernelGGL(k_ifft_rows_plane, dim3((a.Y + 3) / 4, a.T, nplanes), dim3(256), lds, s, a);
     return hipGetLastError();
 }
