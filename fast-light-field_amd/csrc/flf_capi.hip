@@ -282,6 +282,7 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
         m.A_pstride = (long long)T * gt.fY * gt.rfX;
         m.N = N; m.fY = gt.fY; m.fX = gt.fX; m.rY = gt.rY; m.rX = gt.rX; m.rfX = gt.rfX; m.nu = gt.nu;
         m.T = T;
+        m.rs = mac_row_stride(gt.slots, gt.nu);
         rc = run_mac(ctx, m, g, m.nP, st);
         if (rc) return rc;
         IfftColArgs ic{};
@@ -605,6 +606,7 @@ int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, i
         m.A_pstride = 0;
         m.N = N; m.fY = gt.fY; m.fX = gt.fX; m.rY = gt.rY; m.rX = gt.rX; m.rfX = gt.rfX; m.nu = gt.nu;
         m.T = T;
+        m.rs = mac_row_stride(gt.slots, gt.nu);
         rc = run_mac(ctx, m, g, 1, st);
         if (rc) return rc;
         IfftColArgs ic{};

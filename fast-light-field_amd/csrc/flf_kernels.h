@@ -66,6 +66,7 @@ struct MacArgs {
     long long A_tstride, A_pstride;
     int N, fY, fX, rY, rX, rfX, nu;
     int T;                       // frames (grid z covers them in blocks of FB)
+    int rs;                      // P/M table residue-row stride (mac_row_stride)
 };
 
 struct IfftColArgs {
@@ -132,6 +133,7 @@ hipError_t launch_small_fft(const SmallFFTArgs& a, int nsrc, int T, hipStream_t 
 hipError_t launch_psf_rows(const PsfRowArgs& a, int nu, hipStream_t s);
 hipError_t launch_psf_cols(const PsfColArgs& a, int nu, hipStream_t s);
 hipError_t launch_spec_pack(const SpecPackArgs& a, int nchunks, hipStream_t s);
+int mac_row_stride(int slots, int nu);
 size_t mac_lds(int slots, int nu, int N, int fb);
 hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks, int nplanes_grid, hipStream_t s);
 hipError_t launch_ifft_cols(const IfftColArgs& a, int nbatch, hipStream_t s);

@@ -253,13 +253,13 @@ __device__ __forceinline__ void load_g(float2 (&g)[4], const float2* p) {
 // P = s_plain + s_conj, M = i (s_plain - s_conj).
 template <int SLOTS, int IPT, int FB>
 __device__ __forceinline__ void mac_spectrum(float2 (&acc)[FB][IPT][4], const float2 (&g)[IPT][4],
-                                             const float4* __restrict__ pm, int tab, const int (&rb)[4], int ucol) {
+                                             const float4* __restrict__ pm, int tab, const int (&rb)[4], int u) {
     constexpr int NPT = SLOTS == 4 ? 4 : 2;
 #pragma unroll
     for (int f = 0; f < FB; ++f) {
 #pragma unroll
         for (int j = 0; j < SLOTS; ++j) {
-            const float4* row = pm + f * tab + (rb[j] + ucol) * NPT;
+            const float4* row = pm + f * tab + rb[j] + u * NPT;
 #pragma unroll
             for (int i = 0; i < SLOTS; ++i) {
                 const int pt = SLOTS == 4 ? kQuadPT[j][i] : kPairPT[j][i];
@@ -302,7 +302,7 @@ __device__ __forceinline__ void mac_spectrum(float2 (&acc)[FB][IPT][4], const fl
 template <int SLOTS, int IPT, int FB>
 __global__ __launch_bounds__(FLF_CHUNK_MAX) __attribute__((amdgpu_waves_per_eu(IPT * FB >= 4 ? 2 : (IPT * FB == 2 ? 3 : FLF_MAC_WAVES))))
 void k_mac(MacArgs a) {
-    extern __shared__ float4 pm_tab[];            // [FB][NRES][nu][NPT], then staging
+    extern __shared__ float4 pm_tab[];            // [FB][NRES][rs: nu*NPT + pad], then staging
     constexpr int NUSE = SLOTS == 4 ? 8 : 4;
     constexpr int NPT = SLOTS == 4 ? 4 : 2;
     constexpr int NRES = SLOTS == 4 ? 8 : 4;      // residues per orbit (canonical list)
@@ -318,7 +318,11 @@ void k_mac(MacArgs a) {
     const int t0 = blockIdx.z * FB;
     const int nf = min(FB, a.T - t0);            // frames of this block (block-uniform)
     const int nu = a.nu;
-    const int tab = NRES * nu * NPT;              // table entries per frame
+    // Residue rows are a.rs = nu*NPT + pad entries apart, rs = 1 (mod 16):
+    // threads of one wave read the same (u, pair type) entry of up to 8
+    // different residues, which then fall on distinct LDS banks.
+    const int rs = a.rs;
+    const int tab = NRES * rs;                    // table entries per frame
     float2* s_stage = reinterpret_cast<float2*>(pm_tab + FB * tab);   // [NN][NH]
     float2* psi = s_stage + NN * NH;                                        // [NRES][2]
     ItemDesc it[IPT];
@@ -326,7 +330,7 @@ void k_mac(MacArgs a) {
     for (int q = 0; q < IPT; ++q) it[q] = a.items[item_base + tsafe * IPT + q];
     int rb[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) rb[j] = it[0].res[j] * nu;
+    for (int j = 0; j < 4; ++j) rb[j] = it[0].res[j] * rs;
 
     float2 tot[FB][IPT][4];
 #pragma unroll
@@ -368,7 +372,7 @@ void k_mac(MacArgs a) {
         for (int i = tid; i < NN * NH; i += blockDim.x) s_stage[i] = Sp[i];
         __syncthreads();
         float4* pmf = pm_tab + f * tab;
-        for (int idx = tid; idx < tab; idx += blockDim.x) {
+        for (int idx = tid; idx < NRES * nu * NPT; idx += blockDim.x) {
             const int pt = idx % NPT;
             const int ru = idx / NPT;
             const int u = ru % nu, rho = ru / nu;
@@ -389,7 +393,7 @@ void k_mac(MacArgs a) {
             }
             const float2 P = cadd(sv[0], sv[1]);
             const float2 d = csub(sv[0], sv[1]);
-            pmf[idx] = make_float4(P.x, P.y, -d.y, d.x);        // M = i * d
+            pmf[rho * rs + u * NPT + pt] = make_float4(P.x, P.y, -d.y, d.x);   // M = i * d
         }
         }
         __syncthreads();
@@ -744,9 +748,13 @@ hipError_t launch_spec_pack(const SpecPackArgs& a, int nchunks, hipStream_t s) {
     hipLaunchKernelGGL(k_spec_pack, dim3(nchunks, a.nu), dim3(FLF_CHUNK_MAX), 0, s, a);
     return hipGetLastError();
 }
+int mac_row_stride(int slots, int nu) {
+    const int n = nu * (slots == 4 ? 4 : 2);
+    return n + ((1 - n % 16) + 16) % 16;
+}
 size_t mac_lds(int slots, int nu, int N, int fb) {
     const int nres = slots == 4 ? 8 : 4;
-    return (size_t)fb * nres * nu * (slots == 4 ? 4 : 2) * sizeof(float4) +
+    return (size_t)fb * nres * mac_row_stride(slots, nu) * sizeof(float4) +
            ((size_t)N * N * (nres / 2) + 2 * nres) * sizeof(float2);
 }
 template <int SLOTS, int IPT>
@@ -756,7 +764,8 @@ static void launch_mac_fb(const MacArgs& a, int fb, dim3 grid, dim3 block, size_
     else hipLaunchKernelGGL((k_mac<SLOTS, IPT, 4>), grid, block, lds, s, a);
 }
 hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks, int nplanes_grid, hipStream_t s) {
-    if ((fb != 1 && fb != 2 && fb != 4) || (ipt != 1 && ipt != 2) || a.T < 1) return hipErrorInvalidValue;
+    if ((fb != 1 && fb != 2 && fb != 4) || (ipt != 1 && ipt != 2) || a.T < 1 || a.rs != mac_row_stride(slots, a.nu))
+        return hipErrorInvalidValue;
     const size_t lds = mac_lds(slots, a.nu, a.N, fb);
     dim3 grid(nchunks, nplanes_grid, (a.T + fb - 1) / fb);
     dim3 block(FLF_CHUNK_MAX / ipt);

@@ -4,11 +4,14 @@
 // start (lfdeconv.py:97-99 with Xguess=None) stays the fallback.
 //
 // Architecture (all 3x3 convolutions, zero padding 1, at lenslet resolution):
-//   x   = pixel-unshuffle(image / mean(image))      N*N channels (padded to 16k)
+//   x   = pixel-unshuffle(sqrt(image / mean(image))) N*N channels (padded to 16k);
+//         the square root stabilises the Poisson variance and keeps x of order 1
 //   h0  = relu(conv0(x))                             width channels
 //   h_i = h_{i-1} + conv_b(relu(conv_a(h_{i-1})))    i = 1..nblocks
 //   o   = conv_head(h_n)                             nz channels (padded)
-//   y   = softplus(o) * mean(image)                  per plane, per lenslet
+//   y   = softplus(o) * mean(image) / nz             per plane, per lenslet
+// (a PSF plane sums to 1, so mean(image) / nz is the mean voxel value and the
+// normalised output is of order 1).
 // The volume guess is y upsampled (nearest) to full resolution.
 //
 // Arithmetic: one BF16 MFMA GEMM kernel (v_mfma_f32_32x32x16_bf16, FP32
@@ -178,7 +181,7 @@ __global__ void k_transpose(const bf16* x, long long P, int C, int ldx, long lon
 }
 
 // Pixel-unshuffle with per-frame normalisation:
-// x[(t*Ly + ly)*Lx + lx][b*N + a] = img[t][ly*N + b][lx*N + a] / scale[t].
+// x[(t*Ly + ly)*Lx + lx][b*N + a] = sqrt(max(img[t][ly*N + b][lx*N + a], 0) / scale[t]).
 __global__ void k_unshuffle(const float* img, int T, int Ly, int Lx, int N, const float* scale, int Cp, bf16* x) {
     const int NN = N * N;
     const long long total = (long long)T * Ly * Lx * Cp;
@@ -191,7 +194,8 @@ __global__ void k_unshuffle(const float* img, int T, int Ly, int Lx, int N, cons
         float v = 0.f;
         if (ch < NN) {
             const int b = ch / N, a = ch % N;
-            v = img[((long long)t * Ly * N + ly * N + b) * ((long long)Lx * N) + lx * N + a] / scale[t];
+            v = sqrtf(fmaxf(img[((long long)t * Ly * N + ly * N + b) * ((long long)Lx * N) + lx * N + a], 0.f) /
+                      scale[t]);
         }
         x[idx] = to_bf16(v);
     }
@@ -216,7 +220,8 @@ __device__ __forceinline__ float softplusf(float x) { return x > 20.f ? x : log1
 __device__ __forceinline__ float sigmoidf(float x) { return 1.f / (1.f + expf(-x)); }
 
 // Head output -> volume guess at lenslet resolution, out[t][z][ly][lx].
-__global__ void k_head_out(const float* o, int P, int ldo, int nz, int Ly, int Lx, const float* scale, float* out) {
+__global__ void k_head_out(const float* o, int P, int ldo, int nz, int Ly, int Lx, const float* scale, float oscale,
+                           float* out) {
     const long long total = (long long)P * nz;
     for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
          idx += (long long)gridDim.x * blockDim.x) {
@@ -224,14 +229,14 @@ __global__ void k_head_out(const float* o, int P, int ldo, int nz, int Ly, int L
         const long long p = idx / nz;
         const int lx = (int)(p % Lx), ly = (int)((p / Lx) % Ly);
         const int t = (int)(p / ((long long)Lx * Ly));
-        out[(((long long)t * nz + z) * Ly + ly) * Lx + lx] = softplusf(o[p * ldo + z]) * scale[t];
+        out[(((long long)t * nz + z) * Ly + ly) * Lx + lx] = softplusf(o[p * ldo + z]) * (scale[t] * oscale);
     }
 }
 
 // MSE loss on the normalised output and its gradient w.r.t. the head
 // pre-activation: target tgt[t][z][ly][lx] (volume units).
 __global__ __launch_bounds__(256) void k_loss(const float* o, int P, int ldo, int nz, int Ly, int Lx,
-                                              const float* scale, const float* tgt, bf16* dO, int ldd,
+                                              const float* scale, float oscale, const float* tgt, bf16* dO, int ldd,
                                               double* part) {
     __shared__ double sh[256];
     double acc = 0.0;
@@ -247,7 +252,7 @@ __global__ __launch_bounds__(256) void k_loss(const float* o, int P, int ldo, in
             const int t = (int)(p / ((long long)Lx * Ly));
             const float x = o[p * ldo + z];
             const float y = softplusf(x);
-            const float tv = tgt[(((long long)t * nz + z) * Ly + ly) * Lx + lx] / scale[t];
+            const float tv = tgt[(((long long)t * nz + z) * Ly + ly) * Lx + lx] / (scale[t] * oscale);
             const float d = y - tv;
             acc += (double)d * d;
             gsc = (float)(2.0 * inv) * d * sigmoidf(x);
@@ -555,8 +560,8 @@ int backward(flf_net* net, const float* tgt, int T, int Ly, int Lx, hipStream_t 
     const long long P = (long long)T * Ly * Lx;
     const NetLayer& lh = net->L.back();
     const int nlb = 256;
-    hipLaunchKernelGGL(k_loss, dim3(nlb), dim3(256), 0, s, net->o, (int)P, lh.Cop, net->nz, Ly, Lx, net->scale, tgt,
-                       net->dO, lh.Cop, net->lpart);
+    hipLaunchKernelGGL(k_loss, dim3(nlb), dim3(256), 0, s, net->o, (int)P, lh.Cop, net->nz, Ly, Lx, net->scale,
+                       1.0f / (float)net->nz, tgt, net->dO, lh.Cop, net->lpart);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1), 0, s, net->lpart, nlb, net->lout);
     NCHK(hipGetLastError());
     int rc;
@@ -685,7 +690,7 @@ int flf_net_forward(flf_net* net, const float* img, int T, int Ly, int Lx, float
     const NetLayer& lh = net->L.back();
     const long long P = (long long)T * Ly * Lx;
     hipLaunchKernelGGL(k_head_out, dim3(blocks_for(P * net->nz)), dim3(256), 0, s, net->o, (int)P, lh.Cop, net->nz, Ly,
-                       Lx, net->scale, out);
+                       Lx, net->scale, 1.0f / (float)net->nz, out);
     NCHK(hipGetLastError());
     return FLF_OK;
 }

@@ -60,7 +60,9 @@ class LensletNet(object):
 
     def init_params(self, seed):
         """He-normal weights; the second conv of each residual block and the
-        head start small so the initial map is close to identity/constant."""
+        head start small so the initial map is close to identity/constant,
+        and the head bias makes the initial output 1 (softplus(b) = 1), the
+        mean voxel value in the network's normalised units."""
         rng = np.random.default_rng(seed)
         out = []
         shapes = self.layer_shapes()
@@ -71,7 +73,8 @@ class LensletNet(object):
             if li == len(shapes) - 1:
                 std *= 0.1
             out.append(rng.normal(0.0, std, size=co * 9 * ci).astype(np.float32))
-            out.append(np.zeros(co, np.float32))
+            b0 = np.log(np.e - 1.0) if li == len(shapes) - 1 else 0.0
+            out.append(np.full(co, b0, np.float32))
         p = np.concatenate(out)
         assert p.size == self.nparam, (p.size, self.nparam)
         return p
@@ -203,15 +206,36 @@ def make_pairs(proj, hMatrix, n, L, seed, peak=(500.0, 5000.0), iters=8, batch=1
     return imgs, tgts
 
 
-def train(net, imgs, tgts, steps, batch=32, lr=1e-3, seed=0, log_every=50, log=print):
-    """Adam on random mini-batches; returns the loss history."""
+def dihedral(x, k):
+    """One of the 8 flips/transposes of the last two axes.  The PSF set is
+    symmetric under them jointly with the lenslet offsets (row/column mirror
+    and transpose, py_light_field.cpp:1514-1562), so transforming an image
+    and its RL volume alike gives another valid pair."""
+    if k & 1:
+        x = torch.flip(x, dims=(-2,))
+    if k & 2:
+        x = torch.flip(x, dims=(-1,))
+    if k & 4:
+        x = x.transpose(-1, -2)
+    return x.contiguous()
+
+
+def train(net, imgs, tgts, steps, batch=32, lr=1e-3, seed=0, log_every=50, log=print, warmup=200, augment=True):
+    """Adam on random mini-batches (linear warm-up, then cosine decay; each
+    batch under a random dihedral transform when augment and the frames are
+    square); returns the loss history."""
     rng = np.random.default_rng(seed)
     n = imgs.shape[0]
+    square = imgs.shape[-1] == imgs.shape[-2]
     hist = []
     for s in range(steps):
         idx = torch.tensor(rng.choice(n, size=min(batch, n), replace=False), device=imgs.device)
-        lr_s = lr * 0.5 * (1 + np.cos(np.pi * s / max(steps, 1)))          # cosine decay
-        loss = net.train_step(imgs[idx], tgts[idx], lr_s)
+        lr_s = lr * min(1.0, (s + 1) / max(warmup, 1)) * 0.5 * (1 + np.cos(np.pi * s / max(steps, 1)))
+        bi, bt = imgs[idx], tgts[idx]
+        if augment:
+            k = int(rng.integers(0, 8 if square else 4))
+            bi, bt = dihedral(bi, k), dihedral(bt, k)
+        loss = net.train_step(bi, bt, lr_s)
         hist.append(loss)
         if log and (s % log_every == 0 or s == steps - 1):
             log("step %d loss %.5f" % (s, loss))

@@ -7,12 +7,12 @@ that produce its training pairs are the ones restated in ``projector``/``rl``.
 
 Network, all 3x3 convolutions with zero padding 1 at lenslet resolution:
     s   = mean(image)                       per frame
-    x   = pixel_unshuffle(image / s, N)     N*N channels, channel b*N + a
+    x   = pixel_unshuffle(sqrt(max(image, 0) / s), N)   N*N channels, channel b*N + a
     h0  = relu(conv0(x))
     h_i = h_{i-1} + conv_b(relu(conv_a(h_{i-1})))      i = 1..nblocks
     o   = conv_head(h_n)
-    y   = softplus(o) * s                   volume guess at lenslet resolution
-    L   = mean over (frame, plane, lenslet) of (softplus(o) - target / s)^2
+    y   = softplus(o) * s / nz              volume guess at lenslet resolution
+    L   = mean over (frame, plane, lenslet) of (softplus(o) - target nz / s)^2
 Parameters are one flat float32 vector, per layer W[Co][tap][Ci] (tap =
 3 ky + kx, offset (ky - 1, kx - 1)) followed by b[Co].
 
@@ -60,7 +60,7 @@ def forward(p, img, nnum, nz, width, nblocks, bf16=False):
     """Returns (o, s): head pre-activation (T, nz, Ly, Lx) and frame means."""
     layers = split_params(p, layer_shapes(nnum, nz, width, nblocks))
     s = img.reshape(img.shape[0], -1).mean(dim=1).clamp_min(1e-12)
-    x = _r(unshuffle(img / s[:, None, None], nnum), bf16)
+    x = _r(unshuffle(torch.sqrt(img.clamp_min(0) / s[:, None, None]), nnum), bf16)
     w, b = layers[0]
     h = _r(F.relu(F.conv2d(x, _r(w, bf16), b, padding=1)), bf16)
     for i in range(nblocks):
@@ -75,11 +75,12 @@ def forward(p, img, nnum, nz, width, nblocks, bf16=False):
 
 def guess(p, img, nnum, nz, width, nblocks, bf16=False):
     o, s = forward(p, img, nnum, nz, width, nblocks, bf16)
-    return F.softplus(o) * s[:, None, None, None]
+    return F.softplus(o) * (s / nz)[:, None, None, None]
 
 
 def loss(o, s, target):
-    return ((F.softplus(o) - target / s[:, None, None, None]) ** 2).mean()
+    nz = o.shape[1]
+    return ((F.softplus(o) - target * nz / s[:, None, None, None]) ** 2).mean()
 
 
 def loss_and_grad(p_np, img_np, tgt_np, nnum, nz, width, nblocks, bf16=False, dtype=torch.float64):

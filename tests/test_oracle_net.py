@@ -96,7 +96,7 @@ def test_planted_sign_error_is_caught():
     o, s = onet.forward(p, img, N, nz, width, nb)
     layers = onet.split_params(p, shapes)
     F = torch.nn.functional
-    x = onet.unshuffle(img / s[:, None, None], N)
+    x = onet.unshuffle(torch.sqrt(img / s[:, None, None]), N)
     h = F.relu(F.conv2d(x, layers[0][0], layers[0][1], padding=1))
     t = F.relu(F.conv2d(h, layers[1][0], layers[1][1], padding=1))
     h_wrong = F.conv2d(t, layers[2][0], layers[2][1], padding=1)          # residual dropped
