@@ -1,13 +1,16 @@
 """Train the learned initialiser for one PSF set on the GPU.
 
-    python -m flfhip.train_init --workload rl1455 --pairs 2048 --steps 2000
+    python -m flfhip.train_init --workload rl1455 --pairs 2048 --steps 6000
 
 Pairs are made by the method itself (lensletnet.make_pairs): seeded blob
 volumes at L x L lenslets are forward-projected through the workload's PSF
 set, Poisson noise is drawn at a peak in [500, 5000] counts, and plain RL from
-a uniform start gives the target (its lenslet-block means).  Training runs the
-network's own BF16 MFMA forward/backward kernels with Adam.  The weights
-(with the training settings and losses) go to flfhip/weights/ by default.
+a uniform start gives the target (its lenslet-block means).  The target is
+taken after 16 iterations, so a good guess is ahead of the 8-iteration data
+fit that RL from the guess has to reach.  Training runs the network's own
+BF16 MFMA forward/backward kernels with Adam (warm-up, cosine decay, random
+dihedral transforms).  The weights (with the training settings and losses)
+go to flfhip/weights/ by default.
 """
 import argparse
 import os
@@ -23,10 +26,10 @@ def main(argv=None):
     ap.add_argument("--pairs", type=int, default=2048)
     ap.add_argument("--val-pairs", type=int, default=64)
     ap.add_argument("--lenslets", type=int, default=40, help="training frames are L x L lenslets")
-    ap.add_argument("--rl-iters", type=int, default=8, help="plain RL iterations that make a target")
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--rl-iters", type=int, default=16, help="plain RL iterations that make a target")
+    ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr", type=float, default=3e-4)
     ap.add_argument("--width", type=int, default=128)
     ap.add_argument("--blocks", type=int, default=4)
     ap.add_argument("--seed", type=int, default=1000)
@@ -54,7 +57,7 @@ def main(argv=None):
     log("pairs: %d + %d held out, %.1fs" % (args.pairs, args.val_pairs, t_pairs))
     net = LensletNet(hm.Nnum, hm.numZ, args.width, args.blocks, seed=0)
     t = time.time()
-    hist = train(net, imgs, tgts, args.steps, batch=args.batch, lr=args.lr, seed=args.seed, log=log)
+    hist = train(net, imgs, tgts, args.steps, batch=args.batch, lr=args.lr, seed=args.seed, log=log, log_every=500)
     torch.cuda.synchronize()
     t_train = time.time() - t
     vl = [net.train_step(vimgs[b:b + args.batch], vtgts[b:b + args.batch], 0.0)
