@@ -101,7 +101,9 @@ def main():
     proj = Projector_hip()
     torch.cuda.synchronize()
     t = time.time()
-    proj.prepare(hm, 1, Y, X)
+    # room for two frames per call: the learned leg projects its guess and the
+    # first plain iterate together
+    proj.prepare(hm, 1, Y, X, frames_capacity=2)
     torch.cuda.synchronize()
     t_prep = time.time() - t
     log("spectra precompute: %.2fs, device bytes %.2f GB" % (t_prep, proj.lib.flf_device_bytes(proj._ctx) / 1e9))

@@ -32,8 +32,10 @@ struct Group {
     unsigned char* d_enable = nullptr;
     std::vector<unsigned char> enable_host;
     float2* d_G[2] = {nullptr, nullptr};
-    long long* d_srcoff_fwd = nullptr;
+    long long* d_srcoff_fwd = nullptr;   // active frame count's tables (point into the *_T lists)
     long long* d_Soff_fwd = nullptr;
+    std::vector<long long*> srcoff_T, soff_T;    // [T - 1]: tables for every frame count <= Tcap
+    std::vector<long long> Tbwd_off_T;
     float2* d_S_fwd = nullptr;
     float2* d_S_bwd = nullptr;
     float2* d_A_fwd = nullptr;
@@ -53,6 +55,8 @@ struct flf_ctx {
     std::string err;
     bool planned = false;
     int N = 0, Y = 0, X = 0, T = 0, nz = 0;
+    int Tcap = 0;                     // frames the buffers hold; T <= Tcap is the active count
+    std::vector<RowPlaneDesc*> rowdesc_T;
     std::vector<int> s2, group_of, idx_in_group;
     std::vector<Group> groups;
     RowPlaneDesc* d_rowdesc = nullptr;
@@ -119,6 +123,8 @@ void free_all(flf_ctx* ctx) {
     for (void* p : ctx->allocs) (void)hipFree(p);
     ctx->allocs.clear();
     ctx->groups.clear();
+    ctx->rowdesc_T.clear();
+    ctx->Tcap = 0;
     ctx->bytes = 0;
     ctx->planned = false;
     ctx->d_rowdesc = nullptr;
@@ -370,7 +376,7 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
     if (Y < 1 || X < 1 || T < 1 || nz < 1 || !s2) return fail(ctx, FLF_EINVAL, "bad geometry");
     for (int z = 0; z < nz; ++z)
         if (s2[z] < 1 || s2[z] % 2 == 0) return fail(ctx, FLF_EINVAL, "PSF side must be odd and positive");
-    ctx->N = nnum; ctx->Y = Y; ctx->X = X; ctx->T = T; ctx->nz = nz;
+    ctx->N = nnum; ctx->Y = Y; ctx->X = X; ctx->T = T; ctx->Tcap = T; ctx->nz = nz;
     ctx->s2.assign(s2, s2 + nz);
     ctx->group_of.assign(nz, -1);
     ctx->idx_in_group.assign(nz, -1);
@@ -442,17 +448,25 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
         if ((rc = upload(ctx, &g.d_au, gt.u_au))) return rc;
         if ((rc = upload(ctx, &g.d_uoff, gt.uoff))) return rc;
         std::vector<int> s2g;
-        std::vector<long long> srcoff, soff;
-        const long long Sper = (long long)T * s_frame_stride(ctx, g);
-        for (int p = 0; p < nP; ++p) {
-            int z = g.planes[p];
-            s2g.push_back(s2[z]);
-            srcoff.push_back((long long)z * T * Y * X);
-            soff.push_back((long long)p * Sper);
-        }
+        for (int p = 0; p < nP; ++p) s2g.push_back(s2[g.planes[p]]);
         if ((rc = upload(ctx, &g.d_s2, s2g))) return rc;
-        if ((rc = upload(ctx, &g.d_srcoff_fwd, srcoff))) return rc;
-        if ((rc = upload(ctx, &g.d_Soff_fwd, soff))) return rc;
+        // Volume-plane and small-FFT offsets depend on the frame count: one
+        // table per count up to the capacity, selected by flf_set_frames.
+        const long long Sper = (long long)T * s_frame_stride(ctx, g);
+        for (int tf = 1; tf <= T; ++tf) {
+            std::vector<long long> srcoff, soff;
+            for (int p = 0; p < nP; ++p) {
+                srcoff.push_back((long long)g.planes[p] * tf * Y * X);
+                soff.push_back((long long)p * tf * s_frame_stride(ctx, g));
+            }
+            long long *ps = nullptr, *pso = nullptr;
+            if ((rc = upload(ctx, &ps, srcoff))) return rc;
+            if ((rc = upload(ctx, &pso, soff))) return rc;
+            g.srcoff_T.push_back(ps);
+            g.soff_T.push_back(pso);
+        }
+        g.d_srcoff_fwd = g.srcoff_T[T - 1];
+        g.d_Soff_fwd = g.soff_T[T - 1];
         g.enable_host.assign(nP, 1);
         if ((rc = upload(ctx, &g.d_enable, g.enable_host))) return rc;
         const long long items = (long long)gt.items.size();
@@ -465,10 +479,12 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
         if ((rc = dalloc(ctx, &g.d_A_fwd, (size_t)T * gt.fY * gt.rfX))) return rc;
         if ((rc = dalloc(ctx, &g.d_A_bwd, (size_t)nP * T * gt.fY * gt.rfX))) return rc;
         if ((rc = dalloc(ctx, &g.d_T_fwd, (size_t)T * Y * gt.rfX))) return rc;
+        // Offsets into the shared backward buffer scale with the frame count:
+        // unit offsets here, multiplied per count below.
         g.Tbwd_off = Tbwd_total;
         for (int p = 0; p < nP; ++p) {
             RowPlaneDesc& rd = rowdesc[g.planes[p]];
-            rd.T_off = Tbwd_total + (long long)p * T * Y * gt.rfX;
+            rd.T_off = Tbwd_total + (long long)p * Y * gt.rfX;
             rd.T_tstride = (long long)Y * gt.rfX;
             rd.rfX = gt.rfX;
             rd.scale = 1.0f / ((float)gt.fY * (float)gt.fX);
@@ -477,13 +493,24 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
             rd.twf = g.twX;
             rd.twh = g.twXh;
         }
-        Tbwd_total += (long long)nP * T * Y * gt.rfX;
+        Tbwd_total += (long long)nP * Y * gt.rfX;
         if (ifft_rows_sum_lds(gt.fX) > ctx->max_lds || X > 2048)
             return fail(ctx, FLF_EINVAL, "image row too long for the row transform");
     }
     int rc;
-    if ((rc = dalloc(ctx, &ctx->d_T_bwd, (size_t)Tbwd_total))) return rc;
-    if ((rc = upload(ctx, &ctx->d_rowdesc, rowdesc))) return rc;
+    if ((rc = dalloc(ctx, &ctx->d_T_bwd, (size_t)Tbwd_total * T))) return rc;
+    for (auto& g : ctx->groups) {
+        for (int tf = 1; tf <= T; ++tf) g.Tbwd_off_T.push_back(g.Tbwd_off * tf);
+        g.Tbwd_off = g.Tbwd_off_T[T - 1];
+    }
+    for (int tf = 1; tf <= T; ++tf) {
+        std::vector<RowPlaneDesc> rd = rowdesc;
+        for (auto& r : rd) r.T_off *= tf;
+        RowPlaneDesc* prd = nullptr;
+        if ((rc = upload(ctx, &prd, rd))) return rc;
+        ctx->rowdesc_T.push_back(prd);
+    }
+    ctx->d_rowdesc = ctx->rowdesc_T[T - 1];
     if ((rc = upload(ctx, &ctx->d_zeros, std::vector<long long>(maxP, 0)))) return rc;
     if ((rc = dalloc(ctx, &ctx->d_plane_ids, (size_t)nz))) return rc;
     ctx->plane_ids_host.clear();
@@ -492,6 +519,23 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
     ctx->planned = true;
     return FLF_OK;
 }
+
+int flf_set_frames(flf_ctx* ctx, int T) {
+    if (!ctx) return FLF_EINVAL;
+    if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");
+    if (T < 1 || T > ctx->Tcap)
+        return fail(ctx, FLF_EINVAL, "frame count " + std::to_string(T) + " outside 1.." + std::to_string(ctx->Tcap));
+    ctx->T = T;
+    for (auto& g : ctx->groups) {
+        g.d_srcoff_fwd = g.srcoff_T[T - 1];
+        g.d_Soff_fwd = g.soff_T[T - 1];
+        g.Tbwd_off = g.Tbwd_off_T[T - 1];
+    }
+    ctx->d_rowdesc = ctx->rowdesc_T[T - 1];
+    return FLF_OK;
+}
+
+int flf_frame_capacity(const flf_ctx* ctx) { return ctx && ctx->planned ? ctx->Tcap : 0; }
 
 int flf_plane_fshape(const flf_ctx* ctx, int z, int* fY, int* fX) {
     if (!ctx || !ctx->planned || z < 0 || z >= ctx->nz || !fY || !fX) return FLF_EINVAL;

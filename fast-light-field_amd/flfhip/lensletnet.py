@@ -284,17 +284,28 @@ class LearnedInit(object):
     def initial_volume(self, im, Htf):
         """im (T, Y, X), Htf (nz, T, Y, X) on the device.  Returns (X, HX,
         info): the starting volume, its forward projection and per-frame
-        guard results (accepted, fit_guess, fit_plain1)."""
+        guard results (accepted, fit_guess, fit_plain1).  When the projector
+        was planned for 2T frames (prepare(..., frames_capacity=2T)), the
+        guess and the plain iterate are projected in one call, so every
+        cached spectrum value read serves both."""
         p, hm = self.projector, self.hMatrix
         T, Y, Xw = im.shape
         den = self.plain_denominator(T, Y, Xw)
         g = self.net.volume_guess(im)
-        Hg = p.forward_device(hm, g)
-        fit_g = _fit(p, Hg, im)
         X1 = torch.ones_like(Htf)
         p.rl_scale_device(X1, Htf, den)
-        HX1 = p.forward_device(hm, X1)
-        fit_1 = _fit(p, HX1, im)
+        if getattr(p, "_tcap", 0) >= 2 * T:
+            both = torch.cat([g, X1], dim=1)                   # (nz, 2T, Y, X)
+            H2 = p.forward_device(hm, both)
+            fits = _fit(p, H2, torch.cat([im, im], dim=0))
+            fit_g, fit_1 = fits[:T], fits[T:]
+            Hg, HX1 = H2[:T].contiguous(), H2[T:].contiguous()
+            del both, H2
+        else:
+            Hg = p.forward_device(hm, g)
+            fit_g = _fit(p, Hg, im)
+            HX1 = p.forward_device(hm, X1)
+            fit_1 = _fit(p, HX1, im)
         accept = fit_g <= fit_1
         for t in np.nonzero(~accept)[0]:
             g[:, t] = X1[:, t]

@@ -38,6 +38,7 @@ class Projector_hip(object):
             raise _lib.FLFError("flf_create failed (%d)" % rc)
         self._ctx = ctx
         self._key = None
+        self._tcap = self._frames = 0
         self._loaded = set()
 
     def __del__(self):
@@ -77,9 +78,15 @@ class Projector_hip(object):
         _lib.check(self.lib, self._ctx, rc, what)
 
     def _ensure_plan(self, hMatrix, T, Y, X):
+        """Plan for (hMatrix, Y, X) with room for at least T frames; a call
+        with fewer frames than planned only selects the frame count (the
+        spectrum cache stays)."""
         key = (id(hMatrix), getattr(hMatrix, "HPathFormat", None), hMatrix.numZ, getattr(hMatrix, "zStart", 0),
-               hMatrix.Nnum, T, Y, X)
-        if key == self._key:
+               hMatrix.Nnum, Y, X)
+        if key == self._key and T <= self._tcap:
+            if T != self._frames:
+                self._check(self.lib.flf_set_frames(self._ctx, int(T)), "flf_set_frames")
+                self._frames = T
             return
         s2 = []
         for cc in range(hMatrix.numZ):
@@ -92,6 +99,7 @@ class Projector_hip(object):
         self._loaded = set()
         self._check(self.lib.flf_plan(self._ctx, int(hMatrix.Nnum), int(Y), int(X), int(T), len(s2), arr), "flf_plan")
         self._key = key
+        self._tcap = self._frames = int(T)
         self._hMatrix = hMatrix
 
     def _ensure_psf(self, hMatrix, planes, direction):
@@ -103,10 +111,14 @@ class Projector_hip(object):
                         "flf_set_psf")
             self._loaded.add((cc, direction))
 
-    def prepare(self, hMatrix, T, Y, X, planes=None, directions=(0, 1)):
+    def prepare(self, hMatrix, T, Y, X, planes=None, directions=(0, 1), frames_capacity=None):
         """Plan and compute all spectra up front (the first-iteration cost of
-        the reference's F(H) computation, done once)."""
+        the reference's F(H) computation, done once).  frames_capacity
+        (>= T) reserves buffers for that many frames per call."""
         planes = range(hMatrix.numZ) if planes is None else planes
+        cap = max(int(T), int(frames_capacity or 0))
+        if cap > T:
+            self._ensure_plan(hMatrix, cap, Y, X)
         self._ensure_plan(hMatrix, T, Y, X)
         for d in directions:
             self._ensure_psf(hMatrix, planes, d)
@@ -173,6 +185,11 @@ class Projector_hip(object):
     def datafit_device(self, img, im):
         """Per-frame FP64 (sum((img-im)^2), sum(im^2))."""
         T = img.shape[0]
+        if T != self._frames:
+            if not (1 <= T <= self._tcap):
+                raise ValueError("datafit of %d frames on a plan for %d" % (T, self._tcap))
+            self._check(self.lib.flf_set_frames(self._ctx, int(T)), "flf_set_frames")
+            self._frames = T
         out = (ctypes.c_double * (2 * T))()
         self._check(self.lib.flf_datafit(self._ctx, _ptr(img.contiguous()), _ptr(im.contiguous()), out,
                                          _stream_handle()), "flf_datafit")

@@ -56,6 +56,15 @@ const char* flf_last_error(const flf_ctx* ctx);
  * PSFs. */
 int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2);
 
+/* Frames per call.  flf_plan's T is the capacity the work buffers hold;
+ * flf_set_frames selects how many frames (1..capacity) the following calls
+ * process, without touching the spectrum cache.  The reference plans per
+ * call shape (ProjectorForZ_base.__init__, projector.py:106-151) and has no
+ * counterpart; callers that alternate between batch sizes (a two-frame
+ * check, then one-frame iterations) use it to keep one cache. */
+int flf_set_frames(flf_ctx* ctx, int T);
+int flf_frame_capacity(const flf_ctx* ctx);
+
 /* Fourier grid of plane z after flf_plan (fshape of projector.py:123). */
 int flf_plane_fshape(const flf_ctx* ctx, int z, int* fY, int* fX);
 

@@ -1,0 +1,76 @@
+"""The guarded learned start (LearnedInit) on the GPU: the guard falls back to
+plain RL exactly, and the packaged trained network saves RL iterations."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _problem(proj, name, lenslets, seed, peak):
+    from flfhip import synthetic, workloads
+    hm, zs, sides = workloads.psf_set(name)
+    N = hm.Nnum
+    Y = X = N * lenslets
+    proj.prepare(hm, 1, Y, X, frames_capacity=2)
+    vol = synthetic.blob_volume(hm.numZ, Y, X, 150, seed=seed)
+    clean = proj.forward_device(hm, proj.asnative(vol))
+    im = proj.asnative(synthetic.poisson_image(proj.asnumpy(clean), peak, seed))
+    Htf = proj.backward_device(hm, im)
+    return hm, im, Htf
+
+
+def _plain(proj, hm, Htf, im, iters):
+    from flfhip.lfdeconv import deconv_rl_device
+    X = torch.ones_like(Htf)
+    fits = []
+    deconv_rl_device(proj, hm, Htf, iters, X, im=im, fitOut=fits)
+    return X, [float(f[0]) for f in fits]
+
+
+def test_guard_rejects_bad_guess_and_matches_plain(projector):
+    """A network whose guess is ~0 (head bias -30) fails the guard; the run
+    then continues from the first plain iterate and equals plain RL."""
+    from flfhip.lensletnet import LearnedInit, LensletNet
+    from flfhip.lfdeconv import deconv_rl_device
+    hm, im, Htf = _problem(projector, "rl300", 20, seed=11, peak=1000.0)
+    net = LensletNet(hm.Nnum, hm.numZ, width=32, nblocks=1, seed=1)
+    p = net.init_params(1)
+    p[-hm.numZ:] = -30.0
+    net.set_params(p)
+    li = LearnedInit(net, projector, hm)
+    Xp, fits_p = _plain(projector, hm, Htf, im, 4)
+    Xl, n, info = li.deconvolve(im, Htf, 4, targetFit=fits_p[4])
+    print("guard", info, "plain fits", fits_p, flush=True)
+    assert not info["accepted"][0]
+    assert info["fit_guess"][0] > info["fit_plain1"][0]
+    assert abs(info["fit_plain1"][0] - fits_p[1]) <= 1e-9 * fits_p[1]
+    assert n == 4
+    assert rel_l2(Xl.cpu().numpy(), Xp.cpu().numpy()) < 1e-6
+
+
+def test_trained_network_saves_iterations(projector):
+    """The packaged network (trained on 600x600 pairs through the 41-plane
+    ring PSF set) on a held-out 600x600 frame: its guess passes the guard and
+    RL from it reaches the data fit of 8 plain iterations in at most 3."""
+    from flfhip.lensletnet import LearnedInit, LensletNet, default_weights_path
+    hm, im, Htf = _problem(projector, "rl1455", 40, seed=12345, peak=2000.0)
+    path = default_weights_path(hm.Nnum, hm.numZ, "rl1455")
+    assert os.path.exists(path), path
+    net = LensletNet.load(path)
+    Xp, fits_p = _plain(projector, hm, Htf, im, 8)
+    li = LearnedInit(net, projector, hm)
+    fits_l = []
+    Xl, n, info = li.deconvolve(im, Htf, 8, targetFit=fits_p[8], fitOut=fits_l)
+    print("plain", fits_p, "learned", [float(f[0]) for f in fits_l], info, flush=True)
+    assert info["accepted"][0]
+    assert n <= 3
+    assert float(fits_l[-1][0]) <= fits_p[8]

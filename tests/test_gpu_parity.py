@@ -72,3 +72,23 @@ def test_multiframe(projector, T, Y, X):
         for z in range(2):
             ref_b = oproj.project_special(img[t][None], hm.Hcc(z, True))[0]
             assert rel_l2(bwd[z, t], ref_b) < 1e-5, (t, z)
+
+
+def test_frame_capacity(projector):
+    """One plan with room for 3 frames serves calls with 3, 1 and 2 frames
+    (flf_set_frames); every result matches the oracle."""
+    hm = _hm_random(15, [31, 45], seed=9)
+    rng = np.random.default_rng(9)
+    Y, X = 120, 135
+    projector.prepare(hm, 1, Y, X, frames_capacity=3)
+    for T in (3, 1, 2, 1):
+        vol = rng.random((2, T, Y, X)).astype(np.float32)
+        img = rng.random((T, Y, X)).astype(np.float32)
+        print("frames", T, flush=True)
+        fwd = projector.ForwardProjectACC(hm, vol, [0, 1])
+        bwd = projector.BackwardProjectACC(hm, img, [0, 1])
+        for t in range(T):
+            ref_f = sum(oproj.project_special(vol[z, t][None], hm.Hcc(z, False))[0] for z in range(2))
+            assert rel_l2(fwd[t], ref_f) < 1e-5, (T, t)
+            ref_b = oproj.project_special(img[t][None], hm.Hcc(1, True))[0]
+            assert rel_l2(bwd[1, t], ref_b) < 1e-5, (T, t)
