@@ -78,6 +78,18 @@ def cpu_baseline(w):
                          len(w["zs"]), w["iters"])}
 
 
+def measured_traffic(workload):
+    """HBM bytes per k_mac launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs of this bench (calibrated by a 1 GiB copy), or None."""
+    path = os.path.join(ROOT, "profiles", "k_mac_%s_hbm_traffic.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_per_launch_bytes"):
+            return float(line.split()[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,7 +235,7 @@ def main():
                    "learned": learned,
                    "speedup_vs_plain": (plain_ms / learned["ms"]) if learned else None},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM, "traffic": None,
+                     "frac": achieved / PEAK_HBM, "traffic": measured_traffic(args.workload),
                      "kernel": "k_mac", "launch_avg_ms": mac_avg_ms, "launches": mac_n,
                      "bytes_per_launch": mac_bytes / max(mac_n, 1)},
         "rl_iteration": {"ms": it_s * 1e3, "algorithmic_bytes": it_bytes,

@@ -201,15 +201,29 @@ __global__ void k_unshuffle(const float* img, int T, int Ly, int Lx, int N, cons
     }
 }
 
-// Per-frame mean of the image (deterministic two-level reduction).
-__global__ __launch_bounds__(256) void k_frame_mean(const float* img, long long n, float* out) {
+// Per-frame mean of the image: FP64 partial sums over MEAN_BLOCKS blocks per
+// frame (grid (MEAN_BLOCKS, T)), then one block per frame adds them in a
+// fixed order (deterministic).
+constexpr int MEAN_BLOCKS = 128;
+__global__ __launch_bounds__(256) void k_frame_sum_partial(const float* img, long long n, double* part) {
     __shared__ double sh[256];
-    const float* a = img + (long long)blockIdx.x * n;
+    const float* a = img + (long long)blockIdx.y * n;
     double s = 0.0;
-    for (long long i = threadIdx.x; i < n; i += blockDim.x) s += a[i];
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        s += a[i];
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(long long)blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+}
+__global__ __launch_bounds__(MEAN_BLOCKS) void k_frame_mean_final(const double* part, long long n, float* out) {
+    __shared__ double sh[MEAN_BLOCKS];
+    sh[threadIdx.x] = part[(long long)blockIdx.x * MEAN_BLOCKS + threadIdx.x];
+    __syncthreads();
+    for (int k = MEAN_BLOCKS / 2; k > 0; k >>= 1) {
         if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
         __syncthreads();
     }
@@ -378,7 +392,7 @@ struct flf_net {
     bf16 *x = nullptr, *col = nullptr, *colt = nullptr, *dyt = nullptr, *dbuf0 = nullptr, *dbuf1 = nullptr;
     std::vector<bf16*> h, t;     // h[0..nblocks], t[1..nblocks] (t[0] unused)
     float *o = nullptr, *scale = nullptr, *wpart = nullptr;
-    double *lpart = nullptr, *lout = nullptr;
+    double *lpart = nullptr, *lout = nullptr, *mpart = nullptr;
     bf16* dO = nullptr;
     std::vector<void*> acts;
 };
@@ -441,6 +455,7 @@ int ensure_acts(flf_net* net, long long P) {
     if ((rc = nalloc(net, &net->dO, (size_t)P * Cop, A))) return rc;
     if ((rc = nalloc(net, &net->scale, (size_t)1024, A))) return rc;
     if ((rc = nalloc(net, &net->lpart, (size_t)1024, A))) return rc;
+    if ((rc = nalloc(net, &net->mpart, (size_t)MEAN_BLOCKS * 1024, A))) return rc;
     if ((rc = nalloc(net, &net->lout, (size_t)1, A))) return rc;
     // split-K partials for wgrad: splits x Co x 9*Ci
     if ((rc = nalloc(net, &net->wpart, (size_t)33 * 9 * maxC * std::max(Wd, Cop), A))) return rc;
@@ -539,7 +554,10 @@ int forward(flf_net* net, const float* img, int T, int Ly, int Lx, hipStream_t s
     int rc = ensure_acts(net, P);
     if (rc) return rc;
     if (T > 1024) return nfail(net, FLF_EINVAL, "too many frames for the network");
-    hipLaunchKernelGGL(k_frame_mean, dim3(T), dim3(256), 0, s, img, (long long)Ly * N * Lx * N, net->scale);
+    hipLaunchKernelGGL(k_frame_sum_partial, dim3(MEAN_BLOCKS, T), dim3(256), 0, s, img, (long long)Ly * N * Lx * N,
+                       net->mpart);
+    hipLaunchKernelGGL(k_frame_mean_final, dim3(T), dim3(MEAN_BLOCKS), 0, s, net->mpart, (long long)Ly * N * Lx * N,
+                       net->scale);
     hipLaunchKernelGGL(k_unshuffle, dim3(blocks_for(P * net->L[0].Cip)), dim3(256), 0, s, img, T, Ly, Lx, N, net->scale,
                        net->L[0].Cip, net->x);
     NCHK(hipGetLastError());
