@@ -69,6 +69,7 @@ struct flf_ctx {
     size_t max_lds = 65536;
     int ipt = 1;                      // items per MAC thread (FLF_ITEMS_PER_THREAD overrides)
     int fb_max = 2;                   // frames per MAC workgroup at T >= 2 (FLF_MAC_FRAMES overrides)
+    int fwd_split = 0;                // forward plane ranges, 0 = automatic (FLF_FWD_SPLIT overrides)
     long long bytes = 0;
     std::vector<void*> allocs;
     // Optional timing of the multiply-accumulate launches (flf_profile_*).
@@ -81,6 +82,7 @@ struct flf_ctx {
 namespace {
 
 const int FIT_BLOCKS = 256;
+const int FWD_SPLIT_MAX = 4;
 const int FFT_THREADS = 512;
 const int FFT_MAXB = 16;
 
@@ -207,6 +209,24 @@ hipEvent_t take_event(flf_ctx* ctx) {
     return e;
 }
 
+// Frames per MAC workgroup: each spectrum value read serves fb frames.
+int mac_frames(const flf_ctx* ctx, const GridTables& gt) {
+    int fb = 1;
+    while (fb * 2 <= ctx->fb_max && fb * 2 <= ctx->T && mac_lds(gt.slots, gt.nu, ctx->N, fb * 2) <= ctx->max_lds)
+        fb *= 2;
+    return fb;
+}
+
+// Plane ranges of the forward MAC: about 4096 workgroups (4 per slot of the
+// 256 CUs x 4 resident workgroups), at most FWD_SPLIT_MAX ranges and one
+// plane per range (FLF_FWD_SPLIT overrides).
+int fwd_split(const flf_ctx* ctx, const Group& g, int nP) {
+    const long long per = (long long)g.gt.chunks.size() * ((ctx->T + mac_frames(ctx, g.gt) - 1) / mac_frames(ctx, g.gt));
+    int sp = (int)std::max<long long>(1, (4096 + per - 1) / std::max<long long>(per, 1));
+    if (ctx->fwd_split > 0) sp = ctx->fwd_split;
+    return std::max(1, std::min(std::min(sp, FWD_SPLIT_MAX), nP));
+}
+
 // Launch the multiply-accumulate kernel; when profiling, bracket it with
 // events on the same stream and record its algorithmic bytes (the compulsory
 // read of the cached spectra of the planes it covers).
@@ -218,11 +238,7 @@ int run_mac(flf_ctx* ctx, const MacArgs& m, const Group& g, int nplanes_grid, hi
         eb = take_event(ctx);
         if (ea && eb) HIPCHK(hipEventRecord(ea, st));
     }
-    // Frames per workgroup: each spectrum value read serves fb frames.
-    int fb = 1;
-    while (fb * 2 <= ctx->fb_max && fb * 2 <= ctx->T && mac_lds(gt.slots, gt.nu, ctx->N, fb * 2) <= ctx->max_lds)
-        fb *= 2;
-    HIPCHK(launch_mac(m, gt.slots, gt.ipt, fb, (int)gt.chunks.size(), nplanes_grid, st));
+    HIPCHK(launch_mac(m, gt.slots, gt.ipt, mac_frames(ctx, gt), (int)gt.chunks.size(), nplanes_grid, st));
     if (ctx->prof && ea && eb) {
         HIPCHK(hipEventRecord(eb, st));
         int np = 0;
@@ -282,7 +298,6 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
         m.S_tstride = s_frame_stride(ctx, g);
         m.s2 = g.d_s2; m.enable = g.d_enable;
         m.nP = (int)g.planes.size();
-        m.plane_loop = 0;
         m.A = g.d_A_bwd;
         m.A_tstride = (long long)gt.fY * gt.rfX;
         m.A_pstride = (long long)T * gt.fY * gt.rfX;
@@ -294,6 +309,7 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
         IfftColArgs ic{};
         ic.A = g.d_A_bwd;
         ic.A_bstride = (long long)gt.fY * gt.rfX;
+        ic.nsum = 1;
         ic.rfX = gt.rfX; ic.CB = g.CBi; ic.Yout = ctx->Y;
         ic.rpY = g.rpY;
         ic.twY = g.twY;
@@ -347,6 +363,10 @@ int flf_create(int device, flf_ctx** out) {
     if (const char* e = getenv("FLF_ITEMS_PER_THREAD")) {
         int v = atoi(e);
         if (v == 1 || v == 2) c->ipt = v;
+    }
+    if (const char* e = getenv("FLF_FWD_SPLIT")) {
+        int v = atoi(e);
+        if (v >= 1 && v <= 4) c->fwd_split = v;
     }
     if (const char* e = getenv("FLF_MAC_FRAMES")) {
         int v = atoi(e);
@@ -476,7 +496,7 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
         }
         if ((rc = dalloc(ctx, &g.d_S_fwd, (size_t)(nP * Sper)))) return rc;
         if ((rc = dalloc(ctx, &g.d_S_bwd, (size_t)Sper))) return rc;
-        if ((rc = dalloc(ctx, &g.d_A_fwd, (size_t)T * gt.fY * gt.rfX))) return rc;
+        if ((rc = dalloc(ctx, &g.d_A_fwd, (size_t)FWD_SPLIT_MAX * T * gt.fY * gt.rfX))) return rc;
         if ((rc = dalloc(ctx, &g.d_A_bwd, (size_t)nP * T * gt.fY * gt.rfX))) return rc;
         if ((rc = dalloc(ctx, &g.d_T_fwd, (size_t)T * Y * gt.rfX))) return rc;
         // Offsets into the shared backward buffer scale with the frame count:
@@ -644,18 +664,23 @@ int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, i
         m.S = g.d_S_fwd; m.S_off = g.d_Soff_fwd; m.S_tstride = a.S_tstride;
         m.s2 = g.d_s2; m.enable = g.d_enable;
         m.nP = nP;
-        m.plane_loop = 1;
         m.A = g.d_A_fwd;
         m.A_tstride = (long long)gt.fY * gt.rfX;
-        m.A_pstride = 0;
+        m.A_pstride = (long long)T * gt.fY * gt.rfX;     // one partial accumulator per plane range
         m.N = N; m.fY = gt.fY; m.fX = gt.fX; m.rY = gt.rY; m.rX = gt.rX; m.rfX = gt.rfX; m.nu = gt.nu;
         m.T = T;
         m.rs = mac_row_stride(gt.slots, gt.nu);
-        rc = run_mac(ctx, m, g, 1, st);
+        // Split the plane sum into ranges so that the launch has enough
+        // workgroups to keep the tail short; the column transform adds the
+        // partial accumulators in a fixed order.
+        const int sp = fwd_split(ctx, g, nP);
+        rc = run_mac(ctx, m, g, sp, st);
         if (rc) return rc;
         IfftColArgs ic{};
         ic.A = g.d_A_fwd;
         ic.A_bstride = (long long)gt.fY * gt.rfX;
+        ic.nsum = sp;
+        ic.A_sstride = m.A_pstride;
         ic.rfX = gt.rfX; ic.CB = g.CBi; ic.Yout = ctx->Y;
         ic.rpY = g.rpY;
         ic.twY = g.twY;

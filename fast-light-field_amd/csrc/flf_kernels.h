@@ -61,7 +61,6 @@ struct MacArgs {
     const int* s2;               // per plane in group
     const unsigned char* enable; // per plane in group
     int nP;
-    int plane_loop;              // 1: sum all planes of the group (forward); 0: plane = blockIdx.y
     float2* A;
     long long A_tstride, A_pstride;
     int N, fY, fX, rY, rX, rfX, nu;
@@ -72,6 +71,8 @@ struct MacArgs {
 struct IfftColArgs {
     const float2* A;
     long long A_bstride;
+    int nsum;                    // partial accumulators summed on load (>= 1)
+    long long A_sstride;
     int rfX, CB, Yout;
     RadixPlan rpY;
     const float2* twY;           // exp(-2 pi i m / fY)

@@ -35,12 +35,15 @@ __device__ __forceinline__ int centred_res(int k, int r) {
 // ---------------------------------------------------------------------------
 // Small FFTs of the lenslet-subsampled source (py_light_field.cpp:1001-1027):
 // S_{b,a} = FFT_r(src[b::N, a::N] zero-padded to rY x rX), half spectrum.
-// grid (N*N, T, nsrc), block 256 (4 waves).  Rows are real, so two rows are
+// grid (N*N, T, nsrc), block 64 * SFFT_WAVES.  Rows are real, so two rows are
 // transformed as one complex sequence and separated afterwards; then each
-// wave transforms columns of the half spectrum.
+// wave transforms columns of the half spectrum.  The r x (r/2+1) spectrum
+// fills most of a workgroup's LDS, so a workgroup holds many waves (16) to
+// hide the latency of the wave-serial FFT passes.
 // Output S[z][t][off][k0][k1], k1 in [0, rX/2].
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_small_fft(SmallFFTArgs a) {
+constexpr int SFFT_WAVES = 16;
+__global__ __launch_bounds__(64 * SFFT_WAVES) void k_small_fft(SmallFFTArgs a) {
     extern __shared__ float2 lds[];
     const int off = blockIdx.x;
     const int t = blockIdx.y;
@@ -51,7 +54,7 @@ __global__ __launch_bounds__(256) void k_small_fft(SmallFFTArgs a) {
     float2* H = lds;                              // [rY][rXh]
     float2* twY = H + rY * rXh;                   // [rY]
     float2* twX = twY + rY;                       // [rX]
-    float2* scratch = twX + rX;                   // [4][rX]
+    float2* scratch = twX + rX;                   // [SFFT_WAVES][rX]
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < rY; i += blockDim.x) twY[i] = a.twY[i];
     for (int i = threadIdx.x; i < rX; i += blockDim.x) twX[i] = a.twX[i];
@@ -61,7 +64,7 @@ __global__ __launch_bounds__(256) void k_small_fft(SmallFFTArgs a) {
     for (int i = Ys * rXh + threadIdx.x; i < rY * rXh; i += blockDim.x) H[i] = make_float2(0.f, 0.f);
     __syncthreads();
     float2* sc = scratch + wv * rX;
-    for (int pr = wv; 2 * pr < Ys; pr += 4) {
+    for (int pr = wv; 2 * pr < Ys; pr += SFFT_WAVES) {
         const int y0 = 2 * pr, y1 = 2 * pr + 1;
         const float* r0 = src + (long long)(b + N * y0) * a.src_pitch + aa;
         const float* r1 = src + (long long)(b + N * y1) * a.src_pitch + aa;
@@ -85,7 +88,7 @@ __global__ __launch_bounds__(256) void k_small_fft(SmallFFTArgs a) {
         wave_sync();
     }
     __syncthreads();
-    for (int k = wv; k < rXh; k += 4) wave_fft<128>(H + k, rXh, a.rpY, twY, false, lane);
+    for (int k = wv; k < rXh; k += SFFT_WAVES) wave_fft<128>(H + k, rXh, a.rpY, twY, false, lane);
     // Sub-sampling phases of this offset: exp(-2 pi i b kt0 / fY) and
     // exp(-2 pi i a kt1 / fX), kt the centred residue (0 for r/2).
     float2* phY = scratch;                        // reuse: [rY]
@@ -307,7 +310,7 @@ void k_mac(MacArgs a) {
     constexpr int NPT = SLOTS == 4 ? 4 : 2;
     constexpr int NRES = SLOTS == 4 ? 8 : 4;      // residues per orbit (canonical list)
     constexpr int NH = NRES / 2;                  // residues stored by the small-FFT kernel
-    constexpr int UNR = IPT >= 4 ? 1 : FLF_MAC_UNR;   // spectra whose loads are issued together
+    constexpr int UNR = IPT >= 2 ? 1 : FLF_MAC_UNR;   // spectra whose loads are issued together
     const ChunkDesc* ch = a.chunks + blockIdx.x;
     const long long item_base = ch->item_base;
     const int n_items = ch->n_items;
@@ -340,8 +343,10 @@ void k_mac(MacArgs a) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) tot[f][q][j] = make_float2(0.f, 0.f);
 
-    const int pb = a.plane_loop ? 0 : blockIdx.y;
-    const int pe = a.plane_loop ? a.nP : blockIdx.y + 1;
+    // Planes of this workgroup: an even share of the group's nP planes for
+    // grid y = plane ranges (gridDim.y = nP gives one plane per workgroup).
+    const int pb = (int)(((long long)blockIdx.y * a.nP) / gridDim.y);
+    const int pe = (int)(((long long)(blockIdx.y + 1) * a.nP) / gridDim.y);
     const long long ustride = (long long)n_items * SLOTS;
     const int nb = (nu + UNR - 1) / UNR;          // spectrum batches per plane
 
@@ -487,8 +492,7 @@ void k_mac(MacArgs a) {
 #pragma unroll
         for (int f = 0; f < FB; ++f) {
             if (f >= nf) break;
-            float2* A = a.A + (long long)(t0 + f) * a.A_tstride +
-                        (a.plane_loop ? 0 : (long long)blockIdx.y * a.A_pstride);
+            float2* A = a.A + (long long)(t0 + f) * a.A_tstride + (long long)blockIdx.y * a.A_pstride;
 #pragma unroll
             for (int q = 0; q < IPT; ++q)
 #pragma unroll
@@ -500,8 +504,10 @@ void k_mac(MacArgs a) {
 
 // ---------------------------------------------------------------------------
 // Inverse transform, stage 1: column inverse FFTs (length fY) of the half
-// plane, keeping output rows y < Yout.  grid (ceil(rfX/CB), nbatch), one
-// wave per column (blockDim = 64*CB); LDS CB*fY + fY (twiddles).
+// plane, keeping output rows y < Yout.  The input is the sum of nsum partial
+// accumulators A_sstride apart (the forward projection's plane ranges).
+// grid (ceil(rfX/CB), nbatch), one wave per column (blockDim = 64*CB); LDS
+// CB*fY + fY (twiddles).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(512) void k_ifft_cols(IfftColArgs a) {
     extern __shared__ float2 lds[];
@@ -514,7 +520,13 @@ __global__ __launch_bounds__(512) void k_ifft_cols(IfftColArgs a) {
     for (int i = threadIdx.x; i < fY; i += blockDim.x) tw[i] = a.twY[i];
     for (int i = threadIdx.x; i < fY * CB; i += blockDim.x) {
         const int y = i / CB, cc = i - y * CB;
-        C[cc * fY + y] = cc < ncol ? in[(long long)y * rfX + c0 + cc] : make_float2(0.f, 0.f);
+        float2 v = make_float2(0.f, 0.f);
+        if (cc < ncol) {
+            const long long o = (long long)y * rfX + c0 + cc;
+            v = in[o];
+            for (int q = 1; q < a.nsum; ++q) v = cadd(v, in[o + q * a.A_sstride]);   // fixed order
+        }
+        C[cc * fY + y] = v;
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -727,11 +739,11 @@ __global__ void k_fill(float* p, long long n, float v) {
 namespace flf {
 
 size_t small_fft_lds(int rY, int rX) {
-    return ((size_t)rY * (rX / 2 + 1) + rY + rX + (size_t)std::max(4 * rX, rY + rX)) * sizeof(float2);
+    return ((size_t)rY * (rX / 2 + 1) + rY + rX + (size_t)std::max(SFFT_WAVES * rX, rY + rX)) * sizeof(float2);
 }
 hipError_t launch_small_fft(const SmallFFTArgs& a, int nsrc, int T, hipStream_t s) {
     size_t lds = small_fft_lds(a.rpY.L, a.rpX.L);
-    hipLaunchKernelGGL(k_small_fft, dim3(a.N * a.N, T, nsrc), dim3(256), lds, s, a);
+    hipLaunchKernelGGL(k_small_fft, dim3(a.N * a.N, T, nsrc), dim3(64 * SFFT_WAVES), lds, s, a);
     return hipGetLastError();
 }
 hipError_t launch_psf_rows(const PsfRowArgs& a, int nu, hipStream_t s) {
