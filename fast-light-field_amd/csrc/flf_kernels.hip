@@ -341,12 +341,22 @@ void k_mac(MacArgs a) {
     const int tab = NRES * rs;                    // table entries per frame
     float2* s_stage = reinterpret_cast<float2*>(pm_tab + FB * tab);   // [NN][NH]
     float2* psi = s_stage + NN * NH;                                        // [NRES][2]
-    ItemDesc it[IPT];
-#pragma unroll
-    for (int q = 0; q < IPT; ++q) it[q] = a.items[item_base + tsafe * IPT + q];
+    // The item descriptors are re-read (from the caches) where they are
+    // needed instead of living in registers across the spectrum loop.
+    const ItemDesc* itp = a.items + item_base + (long long)tsafe * IPT;
     int rb[4];
+    {
+        const ItemDesc i0 = itp[0];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) rb[j] = it[0].res[j] * rs;
+        for (int j = 0; j < 4; ++j) rb[j] = i0.res[j] * rs;
+    }
+    // An opaque copy of the descriptor pointer, so that the compiler does not
+    // hoist the epilogue's loads out of the plane loop.
+    auto items_now = [&]() {
+        const ItemDesc* ip = itp;
+        asm volatile("" : "+v"(ip));
+        return ip;
+    };
 
     float2 tot[FB][IPT][4];
 #pragma unroll
@@ -435,9 +445,11 @@ void k_mac(MacArgs a) {
         const int s2 = a.s2[p];
         const int sig = s2 - 2 + N;
         const int c2 = (s2 - 1) / 2;
+        const ItemDesc* ip = items_now();
 #pragma unroll
         for (int q = 0; q < IPT; ++q) {
-            const int m2 = it[q].m2[0], n2 = it[q].n2[0];
+            const ItemDesc itq = ip[q];
+            const int m2 = itq.m2[0], n2 = itq.n2[0];
             int outer[4];
             outer[0] = sig * (m2 + n2);
             outer[1] = sig * (n2 - m2);
@@ -445,7 +457,7 @@ void k_mac(MacArgs a) {
             outer[3] = sig * (m2 - n2);
 #pragma unroll
             for (int j = 0; j < SLOTS; ++j) {
-                const int k0 = it[q].pos[j] / a.rfX, k1 = it[q].pos[j] - k0 * a.rfX;
+                const int k0 = itq.pos[j] / a.rfX, k1 = itq.pos[j] - k0 * a.rfX;
                 const float2 po = phase_frac(outer[j], fourN);
                 // exp(+2 pi i c2 (k0/fY + k1/fX))
                 const int e0 = (int)(((long long)c2 * k0) % a.fY);
@@ -500,6 +512,7 @@ void k_mac(MacArgs a) {
     }
 
     if (valid) {
+        const ItemDesc* ip = items_now();
 #pragma unroll
         for (int f = 0; f < FB; ++f) {
             if (f >= nf) break;
@@ -508,7 +521,7 @@ void k_mac(MacArgs a) {
             for (int q = 0; q < IPT; ++q)
 #pragma unroll
                 for (int j = 0; j < SLOTS; ++j)
-                    if ((it[q].writemask >> j) & 1) A[it[q].pos[j]] = tot[f][q][j];
+                    if ((ip[q].writemask >> j) & 1) A[ip[q].pos[j]] = tot[f][q][j];
         }
     }
 }
