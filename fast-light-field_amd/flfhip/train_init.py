@@ -1,6 +1,6 @@
 """Train the learned initialiser for one PSF set on the GPU.
 
-    python -m flfhip.train_init --workload rl1455 --pairs 2048 --steps 6000
+    python -m flfhip.train_init --workload rl1455 --pairs 6144 --steps 16000
 
 Pairs are made by the method itself (lensletnet.make_pairs): seeded blob
 volumes at L x L lenslets are forward-projected through the workload's PSF
@@ -23,11 +23,11 @@ import numpy as np
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="rl1455")
-    ap.add_argument("--pairs", type=int, default=2048)
+    ap.add_argument("--pairs", type=int, default=6144)
     ap.add_argument("--val-pairs", type=int, default=64)
     ap.add_argument("--lenslets", type=int, default=40, help="training frames are L x L lenslets")
     ap.add_argument("--rl-iters", type=int, default=16, help="plain RL iterations that make a target")
-    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--steps", type=int, default=16000)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=3e-4)
     ap.add_argument("--width", type=int, default=128)
