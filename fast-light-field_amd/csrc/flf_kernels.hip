@@ -36,13 +36,17 @@ __device__ __forceinline__ int centred_res(int k, int r) {
 // Small FFTs of the lenslet-subsampled source (py_light_field.cpp:1001-1027):
 // S_{b,a} = FFT_r(src[b::N, a::N] zero-padded to rY x rX), half spectrum.
 // grid (N*N, T, nsrc), block 64 * SFFT_WAVES.  Rows are real, so two rows are
-// transformed as one complex sequence and separated afterwards; then each
-// wave transforms columns of the half spectrum.  The r x (r/2+1) spectrum
-// fills most of a workgroup's LDS, so a workgroup holds many waves (16) to
-// hide the latency of the wave-serial FFT passes.
+// transformed as one complex sequence, in place in those two rows' storage
+// (2 (rX/2+1) >= rX entries; one spare row at the end), and separated
+// afterwards; each wave transforms two such row pairs at once, then pairs of
+// columns of the half spectrum.  The r x (r/2+1) spectrum takes 60 KB of LDS
+// at r = 120; 8 waves per workgroup and two workgroups per CU measured best.
 // Output S[z][t][off][k0][k1], k1 in [0, rX/2].
 // ---------------------------------------------------------------------------
-constexpr int SFFT_WAVES = 16;
+#ifndef FLF_SFFT_WAVES
+#define FLF_SFFT_WAVES 8
+#endif
+constexpr int SFFT_WAVES = FLF_SFFT_WAVES;
 __global__ __launch_bounds__(64 * SFFT_WAVES) void k_small_fft(SmallFFTArgs a) {
     extern __shared__ float2 lds[];
     const int off = blockIdx.x;
@@ -51,10 +55,11 @@ __global__ __launch_bounds__(64 * SFFT_WAVES) void k_small_fft(SmallFFTArgs a) {
     const int N = a.N;
     const int b = off / N, aa = off % N;
     const int rY = a.rpY.L, rX = a.rpX.L, rXh = rX / 2 + 1;
-    float2* H = lds;                              // [rY][rXh]
-    float2* twY = H + rY * rXh;                   // [rY]
+    float2* H = lds;                              // [rY + 1][rXh]
+    float2* twY = H + (rY + 1) * rXh;             // [rY]
     float2* twX = twY + rY;                       // [rX]
-    float2* scratch = twX + rX;                   // [SFFT_WAVES][rX]
+    float2* phY = twX + rX;                       // [rY]
+    float2* phX = phY + rY;                       // [rX]
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < rY; i += blockDim.x) twY[i] = a.twY[i];
     for (int i = threadIdx.x; i < rX; i += blockDim.x) twX[i] = a.twX[i];
@@ -63,36 +68,60 @@ __global__ __launch_bounds__(64 * SFFT_WAVES) void k_small_fft(SmallFFTArgs a) {
     const int Xs = (a.X - aa + N - 1) / N;
     for (int i = Ys * rXh + threadIdx.x; i < rY * rXh; i += blockDim.x) H[i] = make_float2(0.f, 0.f);
     __syncthreads();
-    float2* sc = scratch + wv * rX;
-    for (int pr = wv; 2 * pr < Ys; pr += SFFT_WAVES) {
-        const int y0 = 2 * pr, y1 = 2 * pr + 1;
-        const float* r0 = src + (long long)(b + N * y0) * a.src_pitch + aa;
-        const float* r1 = src + (long long)(b + N * y1) * a.src_pitch + aa;
-        for (int x = lane; x < rX; x += 64) {
-            float v0 = 0.f, v1 = 0.f;
-            if (x < Xs) {
-                v0 = r0[N * x];
-                if (y1 < Ys) v1 = r1[N * x];
+    for (int q = wv; 4 * q < Ys; q += SFFT_WAVES) {
+        const int y0 = 4 * q;
+        const int np = (y0 + 2 < Ys) ? 2 : 1;      // row pairs in this group of four rows
+        float2* P = H + (long long)y0 * rXh;      // pair s at P + 2 s rXh
+        for (int sq = 0; sq < np; ++sq) {
+            const int ya = y0 + 2 * sq, yb = ya + 1;
+            const float* r0 = src + (long long)(b + N * ya) * a.src_pitch + aa;
+            const float* r1 = src + (long long)(b + N * yb) * a.src_pitch + aa;
+            for (int x = lane; x < rX; x += 64) {
+                float v0 = 0.f, v1 = 0.f;
+                if (x < Xs) {
+                    v0 = r0[N * x];
+                    if (yb < Ys) v1 = r1[N * x];
+                }
+                P[2 * sq * rXh + x] = make_float2(v0, v1);
             }
-            sc[x] = make_float2(v0, v1);
         }
         wave_sync();
-        wave_fft<128>(sc, 1, a.rpX, twX, false, lane);
-        for (int k = lane; k < rXh; k += 64) {
-            const float2 zk = sc[k];
-            const float2 zc = cconj(sc[(rX - k) % rX]);
-            // row y0: (Z + Zc)/2 ; row y1: (Z - Zc)/(2i)
-            H[y0 * rXh + k] = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
-            if (y1 < Ys) H[y1 * rXh + k] = make_float2(0.5f * (zk.y - zc.y), -0.5f * (zk.x - zc.x));
-        }
+        wave_fft_ns<128, 2>(P, 1, 2 * rXh, np, a.rpX, twX, false, lane);
+        // Separate: row ya = (Z + Zc)/2, row yb = (Z - Zc)/(2i), Zc = conj Z(-k);
+        // all reads before the in-place writes.
+        float2 ev[2][2], od[2][2];
+#pragma unroll
+        for (int sq = 0; sq < 2; ++sq)
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                const int k = lane + 64 * m;
+                if (sq < np && k < rXh) {
+                    const float2* Z = P + 2 * sq * rXh;
+                    const float2 zk = Z[k];
+                    const float2 zc = cconj(Z[(rX - k) % rX]);
+                    ev[sq][m] = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
+                    od[sq][m] = make_float2(0.5f * (zk.y - zc.y), -0.5f * (zk.x - zc.x));
+                }
+            }
+        wave_sync();
+#pragma unroll
+        for (int sq = 0; sq < 2; ++sq)
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                const int k = lane + 64 * m;
+                if (sq < np && k < rXh) {
+                    const int ya = y0 + 2 * sq;
+                    H[ya * rXh + k] = ev[sq][m];
+                    H[(ya + 1) * rXh + k] = (ya + 1 < Ys) ? od[sq][m] : make_float2(0.f, 0.f);
+                }
+            }
         wave_sync();
     }
     __syncthreads();
-    for (int k = wv; k < rXh; k += SFFT_WAVES) wave_fft<128>(H + k, rXh, a.rpY, twY, false, lane);
+    for (int c = 2 * wv; c < rXh; c += 2 * SFFT_WAVES)
+        wave_fft_ns<128, 2>(H + c, rXh, 1, min(2, rXh - c), a.rpY, twY, false, lane);
     // Sub-sampling phases of this offset: exp(-2 pi i b kt0 / fY) and
     // exp(-2 pi i a kt1 / fX), kt the centred residue (0 for r/2).
-    float2* phY = scratch;                        // reuse: [rY]
-    float2* phX = scratch + rY;                   // [rX]
     __syncthreads();
     for (int i = threadIdx.x; i < rY + rX; i += blockDim.x) {
         const bool isY = i < rY;
@@ -763,7 +792,7 @@ __global__ void k_fill(float* p, long long n, float v) {
 namespace flf {
 
 size_t small_fft_lds(int rY, int rX) {
-    return ((size_t)rY * (rX / 2 + 1) + rY + rX + (size_t)std::max(SFFT_WAVES * rX, rY + rX)) * sizeof(float2);
+    return ((size_t)(rY + 1) * (rX / 2 + 1) + 2 * (size_t)(rY + rX)) * sizeof(float2);
 }
 hipError_t launch_small_fft(const SmallFFTArgs& a, int nsrc, int T, hipStream_t s) {
     size_t lds = small_fft_lds(a.rpY.L, a.rpX.L);

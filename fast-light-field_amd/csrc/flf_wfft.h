@@ -80,4 +80,71 @@ __device__ void wave_fft(float2* buf, int es, const RadixPlan& rp, const float2*
     }
 }
 
+// NS sequences per call (sequence s at buf + s * ss, element stride es),
+// nseq <= NS of them present: the butterflies of all sequences share the
+// wave's lanes, which a single short sequence fills only partly.
+template <int R, int LMAX, int NS>
+__device__ __forceinline__ void wave_pass_ns(float2* buf, int es, int ss, int nseq, int L, int Ns,
+                                             const float2* __restrict__ tw, bool inv, int lane) {
+    constexpr int MAXQ = (NS * (LMAX / R) + 63) / 64;
+    const int M = L / R;
+    const int tot = nseq * M;
+    const int tstep = L / (Ns * R);
+    const float sign = inv ? 1.f : -1.f;
+    float2 v[MAXQ][R];
+#pragma unroll
+    for (int q = 0; q < MAXQ; ++q) {
+        const int j = lane + 64 * q;
+        if (j < tot) {
+            const int sq = j / M, jj = j - sq * M;
+            const int k = jj % Ns;
+            const float2* b = buf + sq * ss;
+            float2 a[5];
+#pragma unroll
+            for (int t = 0; t < R; ++t) a[t] = b[(jj + t * M) * es];
+            if (k != 0) {
+#pragma unroll
+                for (int t = 1; t < R; ++t) {
+                    float2 w = tw[t * k * tstep];
+                    if (inv) w.y = -w.y;
+                    a[t] = cmul(a[t], w);
+                }
+            }
+            dft_small<R>(a, sign);
+#pragma unroll
+            for (int t = 0; t < R; ++t) v[q][t] = a[t];
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < MAXQ; ++q) {
+        const int j = lane + 64 * q;
+        if (j < tot) {
+            const int sq = j / M, jj = j - sq * M;
+            const int k = jj % Ns;
+            const int o = (jj / Ns) * Ns * R + k;
+            float2* b = buf + sq * ss;
+#pragma unroll
+            for (int t = 0; t < R; ++t) b[(o + t * Ns) * es] = v[q][t];
+        }
+    }
+    wave_sync();
+}
+
+template <int LMAX, int NS>
+__device__ void wave_fft_ns(float2* buf, int es, int ss, int nseq, const RadixPlan& rp, const float2* __restrict__ tw,
+                            bool inv, int lane) {
+    int Ns = 1;
+    for (int p = 0; p < rp.n; ++p) {
+        const int R = rp.r[p];
+        switch (R) {
+            case 4: wave_pass_ns<4, LMAX, NS>(buf, es, ss, nseq, rp.L, Ns, tw, inv, lane); break;
+            case 2: wave_pass_ns<2, LMAX, NS>(buf, es, ss, nseq, rp.L, Ns, tw, inv, lane); break;
+            case 3: wave_pass_ns<3, LMAX, NS>(buf, es, ss, nseq, rp.L, Ns, tw, inv, lane); break;
+            default: wave_pass_ns<5, LMAX, NS>(buf, es, ss, nseq, rp.L, Ns, tw, inv, lane); break;
+        }
+        Ns *= R;
+    }
+}
+
 }  // namespace flf
