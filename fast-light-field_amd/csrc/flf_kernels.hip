@@ -571,15 +571,30 @@ __global__ __launch_bounds__(512) void k_ifft_cols(IfftColArgs a) {
     const int ncol = min(CB, rfX - c0);
     const float2* in = a.A + (long long)blockIdx.y * a.A_bstride;
     for (int i = threadIdx.x; i < fY; i += blockDim.x) tw[i] = a.twY[i];
-    for (int i = threadIdx.x; i < fY * CB; i += blockDim.x) {
-        const int y = i / CB, cc = i - y * CB;
-        float2 v = make_float2(0.f, 0.f);
-        if (cc < ncol) {
-            const long long o = (long long)y * rfX + c0 + cc;
-            v = in[o];
-            for (int q = 1; q < a.nsum; ++q) v = cadd(v, in[o + q * a.A_sstride]);   // fixed order
+    constexpr int B = 8;                        // loads in flight per thread
+    for (int i0 = threadIdx.x; i0 < fY * CB; i0 += blockDim.x * B) {
+        float2 v[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int i = i0 + u * blockDim.x;
+            const int y = i / CB, cc = i - y * CB;
+            v[u] = make_float2(0.f, 0.f);
+            if (i < fY * CB && cc < ncol) v[u] = in[(long long)y * rfX + c0 + cc];
         }
-        C[cc * fY + y] = v;
+        for (int q = 1; q < a.nsum; ++q) {     // partial accumulators, fixed order
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                const int i = i0 + u * blockDim.x;
+                const int y = i / CB, cc = i - y * CB;
+                if (i < fY * CB && cc < ncol) v[u] = cadd(v[u], in[(long long)y * rfX + c0 + cc + q * a.A_sstride]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int i = i0 + u * blockDim.x;
+            const int y = i / CB, cc = i - y * CB;
+            if (i < fY * CB) C[cc * fY + y] = v[u];
+        }
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -603,26 +618,56 @@ __global__ __launch_bounds__(512) void k_ifft_cols(IfftColArgs a) {
 __device__ __forceinline__ void wave_c2r(float2* buf, const float2* __restrict__ row, int L,
                                          const RadixPlan& rph, const float2* twh,
                                          const RadixPlan& rpf, const float2* twf, int lane) {
+    // Loads are issued C2R_BATCH per lane at a time, so that their latencies
+    // overlap instead of adding up.
+    constexpr int B = 8;
     if ((L & 1) == 0) {
         const int L2 = L / 2;
-        for (int k = lane; k < L2; k += 64) {
-            float2 x = row[k];
-            float2 xr = cconj(row[L2 - k]);
-            if (k == 0) { x.y = 0.f; xr.y = 0.f; }
-            const float2 E = cadd(x, xr);
-            const float2 d = csub(x, xr);
-            const float2 w = cconj(twf[k]);                  // exp(+2 pi i k / L)
-            const float2 O = cmul(d, w);
-            buf[k] = make_float2(E.x - O.y, E.y + O.x);      // E + i O
+        for (int k0 = lane; k0 < L2; k0 += 64 * B) {
+            float2 xs[B], xrs[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                const int k = k0 + 64 * u;
+                if (k < L2) {
+                    xs[u] = row[k];
+                    xrs[u] = row[L2 - k];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                const int k = k0 + 64 * u;
+                if (k < L2) {
+                    float2 x = xs[u];
+                    float2 xr = cconj(xrs[u]);
+                    if (k == 0) { x.y = 0.f; xr.y = 0.f; }
+                    const float2 E = cadd(x, xr);
+                    const float2 d = csub(x, xr);
+                    const float2 w = cconj(twf[k]);          // exp(+2 pi i k / L)
+                    const float2 O = cmul(d, w);
+                    buf[k] = make_float2(E.x - O.y, E.y + O.x);  // E + i O
+                }
+            }
         }
         wave_sync();
         wave_fft<1024>(buf, 1, rph, twh, true, lane);
     } else {
         const int rfX = L / 2 + 1;
-        for (int k = lane; k < L; k += 64) {
-            float2 v = k < rfX ? row[k] : cconj(row[L - k]);
-            if (k == 0) v.y = 0.f;
-            buf[k] = v;
+        for (int k0 = lane; k0 < L; k0 += 64 * B) {
+            float2 vs[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                const int k = k0 + 64 * u;
+                if (k < L) vs[u] = k < rfX ? row[k] : cconj(row[L - k]);
+            }
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                const int k = k0 + 64 * u;
+                if (k < L) {
+                    float2 v = vs[u];
+                    if (k == 0) v.y = 0.f;
+                    buf[k] = v;
+                }
+            }
         }
         wave_sync();
         wave_fft<2048>(buf, 1, rpf, twf, true, lane);
@@ -701,18 +746,35 @@ __global__ __launch_bounds__(256) void k_ifft_rows_plane(IfftRowPlaneArgs a) {
     wave_c2r(buf, row, L, pd.rph, pd.twh, pd.rp, pd.twf, lane);
     const float sc = pd.scale;
     const long long o = (((long long)z * a.T + t) * a.Y + y) * a.X;
-    for (int x = lane; x < a.X; x += 64) {
-        const float v = c2r_sample(buf, L, x) * sc;
-        if (a.mode == 0) {
-            a.vol[o + x] = v;
-        } else if (a.mode == 1) {
-            const float r = a.Htf[o + x] / v;
-            const float xn = a.vol[o + x] * r;
-            a.vol[o + x] = __builtin_isnan(xn) ? 0.f : xn;
-        } else {
-            float xn = a.vol[o + x] * v;
-            if (a.trust) xn = xn / a.trust[((long long)z * a.Y + y) * a.X + x];
-            a.vol[o + x] = __builtin_isnan(xn) ? 0.f : xn;
+    constexpr int B = 8;                        // loads in flight per lane
+    for (int x0 = lane; x0 < a.X; x0 += 64 * B) {
+        float v[B], xo[B], hd[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int x = x0 + 64 * u;
+            if (x < a.X) {
+                v[u] = c2r_sample(buf, L, x) * sc;
+                if (a.mode != 0) xo[u] = a.vol[o + x];
+                if (a.mode == 1) hd[u] = a.Htf[o + x];
+                else if (a.mode == 2) hd[u] = a.trust ? a.trust[((long long)z * a.Y + y) * a.X + x] : 1.f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int x = x0 + 64 * u;
+            if (x < a.X) {
+                if (a.mode == 0) {
+                    a.vol[o + x] = v[u];
+                } else if (a.mode == 1) {
+                    const float r = hd[u] / v[u];
+                    const float xn = xo[u] * r;
+                    a.vol[o + x] = __builtin_isnan(xn) ? 0.f : xn;
+                } else {
+                    float xn = xo[u] * v[u];
+                    if (a.trust) xn = xn / hd[u];
+                    a.vol[o + x] = __builtin_isnan(xn) ? 0.f : xn;
+                }
+            }
         }
     }
 }
