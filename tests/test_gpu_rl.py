@@ -71,3 +71,31 @@ def test_fit_trace_matches_oracle(projector, problem):
     print("fits", got.tolist(), ref.tolist(), flush=True)
     assert got.shape == ref.shape
     assert np.max(np.abs(got - ref) / ref) < 1e-5
+
+
+def test_cfg1_full_rl_vs_oracle(projector):
+    """The CPU-runnable reference case at full size: 20 x 20 lenslets (300 x
+    300), 11 planes z = -5..5 with ring PSFs 15..135 px, 200 blobs, Poisson
+    peak 1000, 4 Prevedel iterations from a uniform start, against the FP64
+    oracle (volume within 1e-4 relative L2, every iterate's data fit within
+    1e-5)."""
+    from flfhip import lfdeconv, workloads
+    w = workloads.setup("rl300")
+    hm = w["hm"]
+    nz = hm.numZ
+    Qf = [hm.Hcc(z, False) for z in range(nz)]
+    Qb = [hm.Hcc(z, True) for z in range(nz)]
+    fwd = lambda v: oproj.forward_project(v, Qf)
+    bwd = lambda i: oproj.backward_project(i, Qb)
+    clean = projector.ForwardProjectACC(hm, w["vol"], range(nz))
+    im = synthetic.poisson_image(clean, w["peak"], w["seed"])
+    X0 = np.ones((nz,) + im.shape, np.float32)
+    fits = []
+    X = lfdeconv.DeconvRL(hm, None, w["iters"], X0, logPrint=False, projector=projector, im=im, fitOut=fits)
+    Xr, rec = orl.deconv_rl(fwd, bwd, None, w["iters"], Xguess=X0, im=im, record_fit=True)
+    err = rel_l2(X, Xr)
+    got = np.array([np.asarray(f, np.float64) for f in fits[:w["iters"]]])
+    print("cfg1 RL(4) rel-L2 %.3e; fits gpu %s oracle %s" % (err, got.ravel().tolist(), np.ravel(rec).tolist()),
+          flush=True)
+    assert err < 1e-4
+    assert np.max(np.abs(got - np.array(rec)) / np.array(rec)) < 1e-5
