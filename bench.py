@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="rl1455", choices=sorted(WORKLOADS))
     ap.add_argument("--weights", default=None, help="learned-initialiser weights (default: the packaged file)")
+    ap.add_argument("--frames", type=int, default=1,
+                    help="frames reconstructed together per step (value = frames x steps / time)")
     ap.add_argument("--plain-only", action="store_true", help="time plain RL only")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -113,7 +115,8 @@ def main():
     from flfhip.projector import Projector_hip
     from flfhip import lensletnet, synthetic
 
-    w = setup(args.workload)
+    T = args.frames
+    w = setup(args.workload, frames=T)
     hm, Y, X, iters = w["hm"], w["Y"], w["X"], w["iters"]
     nz = len(w["zs"])
     proj = Projector_hip()
@@ -121,16 +124,17 @@ def main():
     t = time.time()
     # room for two frames per call: the learned leg projects its guess and the
     # first plain iterate together
-    proj.prepare(hm, 1, Y, X, frames_capacity=2)
+    proj.prepare(hm, T, Y, X, frames_capacity=2 * T)
     torch.cuda.synchronize()
     t_prep = time.time() - t
     log("spectra precompute: %.2fs, device bytes %.2f GB" % (t_prep, proj.lib.flf_device_bytes(proj._ctx) / 1e9))
 
     vol = proj.asnative(w["vol"])
-    clean = proj.forward_device(hm, vol)
-    im = proj.asnative(synthetic.poisson_image(proj.asnumpy(clean), w["peak"], w["seed"]))
+    clean = proj.asnumpy(proj.forward_device(hm, vol))
+    im = proj.asnative(np.concatenate([synthetic.poisson_image(clean[t:t + 1], w["peak"], w["seed"] + t)
+                                       for t in range(T)]))
     del vol, clean
-    Htf = torch.empty((nz, 1, Y, X), dtype=torch.float32, device=proj.device)
+    Htf = torch.empty((nz, T, Y, X), dtype=torch.float32, device=proj.device)
     Xv = torch.empty_like(Htf)
 
     # One step = one reconstruction from the image: Htf = Ht im, then RL.
@@ -167,9 +171,11 @@ def main():
     plain_ms = timed(plain_step, args.steps)
     proj.profile(False)
     mac_ms, mac_n, mac_bytes = proj.profile_read()
-    fit_plain = float(_fit_from_sums(proj.datafit_device(proj.forward_device(hm, Xv), im))[0])
+    fit_plain_t = _fit_from_sums(proj.datafit_device(proj.forward_device(hm, Xv), im))
+    fit_plain = float(fit_plain_t[0]) if T == 1 else [float(f) for f in fit_plain_t]
     X_plain = Xv.clone()
-    log("plain RL: %.2f ms/volume, data fit %.6f after %d iterations" % (plain_ms, fit_plain, iters))
+    log("plain RL: %.2f ms per step of %d frame(s), data fit %s after %d iterations"
+        % (plain_ms, T, fit_plain, iters))
 
     learned = None
     wpath = args.weights or lensletnet.default_weights_path(w["N"], nz, args.workload)
@@ -178,12 +184,12 @@ def main():
     if not args.plain_only and os.path.exists(wpath):
         net = lensletnet.LensletNet.load(wpath)
         li = lensletnet.LearnedInit(net, proj, hm)
-        li.plain_denominator(1, Y, X)          # cached per PSF set and frame shape, like the spectra
+        li.plain_denominator(T, Y, X)          # cached per PSF set and frame shape, like the spectra
         out = {}
 
         def learned_step():
             proj.backward_device(hm, im, out=Htf)
-            Xl, n_it, info = li.deconvolve(im, Htf, 4 * iters, targetFit=fit_plain)
+            Xl, n_it, info = li.deconvolve(im, Htf, 4 * iters, targetFit=fit_plain_t)
             out.update(X=Xl, n=n_it, info=info)
 
         for _ in range(args.warmup):
@@ -194,13 +200,14 @@ def main():
         lm_ms, lm_n, lm_bytes = proj.profile_read()
         mac_ms, mac_n, mac_bytes = mac_ms + lm_ms, mac_n + lm_n, mac_bytes + lm_bytes
         info = out["info"]
-        fit_l = float(_fit_from_sums(proj.datafit_device(proj.forward_device(hm, out["X"]), im))[0])
+        fit_l_t = _fit_from_sums(proj.datafit_device(proj.forward_device(hm, out["X"]), im))
         rel = float(torch.linalg.vector_norm((out["X"] - X_plain).double()) / torch.linalg.vector_norm(X_plain.double()))
-        learned = {"ms": learned_ms, "iterations": int(out["n"]), "guess_accepted": bool(info["accepted"][0]),
-                   "fit_guess": float(info["fit_guess"][0]), "fit_plain1": float(info["fit_plain1"][0]),
-                   "datafit_reached": fit_l, "rel_l2_vs_plain": rel, "weights": os.path.relpath(wpath, ROOT)}
-        log("learned init: %.2f ms/volume, %d iterations, fit %.6f (guess %.5f vs one plain iteration %.5f)"
-            % (learned_ms, out["n"], fit_l, info["fit_guess"][0], info["fit_plain1"][0]))
+        one = lambda a, f=float: f(a[0]) if T == 1 else [f(x) for x in a]
+        learned = {"ms": learned_ms, "iterations": int(out["n"]), "guess_accepted": one(info["accepted"], bool),
+                   "fit_guess": one(info["fit_guess"]), "fit_plain1": one(info["fit_plain1"]),
+                   "datafit_reached": one(fit_l_t), "rel_l2_vs_plain": rel, "weights": os.path.relpath(wpath, ROOT)}
+        log("learned init: %.2f ms per step, %d iterations, fit %s (guess %s vs one plain iteration %s)"
+            % (learned_ms, out["n"], learned["datafit_reached"], learned["fit_guess"], learned["fit_plain1"]))
 
     ms_step = learned["ms"] if learned else plain_ms
     # Whole-iteration roofline: S_H + S_Ht + B*(4*nz+2)*Y*X*4 bytes.
@@ -209,16 +216,17 @@ def main():
         fy, fx = proj.fshape(z)
         u = 36 if fy == fx else 64
         s_dir += u * fy * (fx // 2 + 1) * 8
-    it_bytes = 2 * s_dir + 1 * (4 * nz + 2) * Y * X * 4
-    it_s = (plain_ms / 1e3) / (iters + 0.5)      # Htf is half an iteration
+    it_bytes = 2 * s_dir + T * (4 * nz + 2) * Y * X * 4
+    it_s = (plain_ms / 1e3) / (iters + 0.5)      # Htf is half an iteration (all frames of a step)
     mac_avg_ms = mac_ms / max(mac_n, 1)
     achieved = (mac_bytes / max(mac_n, 1)) / (mac_avg_ms / 1e3)
-    desc = ("Nnum=15, %dx%d px, %d planes (z=-%d..%d), PSF %d..%d px, %d blobs, Poisson peak %g, 1 frame; "
-            "target = data fit of %d plain RL iterations from a uniform start"
-            % (Y, X, nz, w["zs"][-1], w["zs"][-1], min(w["sides"]), max(w["sides"]), w["nblobs"], w["peak"], iters))
+    desc = ("Nnum=15, %dx%d px, %d planes (z=-%d..%d), PSF %d..%d px, %d blobs, Poisson peak %g, %d frame%s per "
+            "step; target = data fit of %d plain RL iterations from a uniform start"
+            % (Y, X, nz, w["zs"][-1], w["zs"][-1], min(w["sides"]), max(w["sides"]), w["nblobs"], w["peak"], T,
+               "" if T == 1 else "s", iters))
     res = {
         "metric": "volumes/s at fixed RL data-fit: learned-init vs plain RL, 1 GPU",
-        "value": 1e3 / ms_step,
+        "value": T * 1e3 / ms_step,
         "unit": "volumes/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -231,7 +239,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "init": "learned (guarded)" if learned else "uniform (plain RL)",
                    "target_datafit": fit_plain,
-                   "plain": {"ms": plain_ms, "volumes_per_s": 1e3 / plain_ms, "iterations": iters},
+                   "frames": T,
+                   "plain": {"ms": plain_ms, "volumes_per_s": T * 1e3 / plain_ms, "iterations": iters},
                    "learned": learned,
                    "speedup_vs_plain": (plain_ms / learned["ms"]) if learned else None},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",

@@ -32,14 +32,16 @@ def psf_set(name):
     return hm, zs, sides
 
 
-def setup(name, volume=True):
+def setup(name, volume=True, frames=1):
+    """Workload dictionary; with frames > 1 the volume holds that many seeded
+    frames (frame 0 is the single-frame volume)."""
     N, L, zmax, smin, smax, nblobs, peak, iters, seed = WORKLOADS[name]
     hm, zs, sides = psf_set(name)
     Y = X = N * L
     vol = None
     if volume:
         t = time.time()
-        vol = synthetic.blob_volume(len(zs), Y, X, nblobs, seed=seed)
+        vol = synthetic.blob_volume(len(zs), Y, X, nblobs, seed=seed, T=frames)
         _log("volume: %s, %.1fs" % (vol.shape, time.time() - t))
     return dict(name=name, N=N, Y=Y, X=X, zs=zs, sides=sides, hm=hm, vol=vol, peak=peak, iters=iters, seed=seed,
                 nblobs=nblobs)
