@@ -25,6 +25,7 @@ struct Group {
     int RB = 1, CBp = 1, CBi = 1;     // FFT batching (psf rows, psf cols, ifft cols)
     ChunkDesc* d_chunks = nullptr;
     ItemDesc* d_items = nullptr;
+    TileDesc* d_tiles = nullptr;
     int* d_bu = nullptr;
     int* d_au = nullptr;
     int* d_uoff = nullptr;
@@ -70,6 +71,8 @@ struct flf_ctx {
     int ipt = 1;                      // items per MAC thread (FLF_ITEMS_PER_THREAD overrides)
     int fb_max = 2;                   // frames per MAC workgroup at T >= 2 (FLF_MAC_FRAMES overrides)
     int fwd_split = 0;                // forward plane ranges, 0 = automatic (FLF_FWD_SPLIT overrides)
+    bool mfma = false;                // matrix-core MAC for >= 2 frames per workgroup (FLF_MAC_MFMA=1 enables;
+                                      // measured slower than the vector kernel, see the kernel's comment)
     long long bytes = 0;
     std::vector<void*> allocs;
     // Optional timing of the multiply-accumulate launches (flf_profile_*).
@@ -238,7 +241,7 @@ int run_mac(flf_ctx* ctx, const MacArgs& m, const Group& g, int nplanes_grid, hi
         eb = take_event(ctx);
         if (ea && eb) HIPCHK(hipEventRecord(ea, st));
     }
-    HIPCHK(launch_mac(m, gt.slots, gt.ipt, mac_frames(ctx, gt), (int)gt.chunks.size(), nplanes_grid, st));
+    HIPCHK(launch_mac(m, gt.slots, gt.ipt, mac_frames(ctx, gt), (int)gt.chunks.size(), nplanes_grid, ctx->mfma, st));
     if (ctx->prof && ea && eb) {
         HIPCHK(hipEventRecord(eb, st));
         int np = 0;
@@ -294,6 +297,7 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
         MacArgs m{};
         m.G = g.d_G[1];
         m.chunks = g.d_chunks; m.items = g.d_items; m.u_bu = g.d_bu; m.u_au = g.d_au; m.uoff = g.d_uoff;
+        m.tiles = g.d_tiles;
         m.S = g.d_S_bwd; m.S_off = ctx->d_zeros;
         m.S_tstride = s_frame_stride(ctx, g);
         m.s2 = g.d_s2; m.enable = g.d_enable;
@@ -364,6 +368,7 @@ int flf_create(int device, flf_ctx** out) {
         int v = atoi(e);
         if (v == 1 || v == 2) c->ipt = v;
     }
+    if (const char* e = getenv("FLF_MAC_MFMA")) c->mfma = atoi(e) != 0;
     if (const char* e = getenv("FLF_FWD_SPLIT")) {
         int v = atoi(e);
         if (v >= 1 && v <= 4) c->fwd_split = v;
@@ -464,6 +469,7 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
         int rc;
         if ((rc = upload(ctx, &g.d_chunks, gt.chunks))) return rc;
         if ((rc = upload(ctx, &g.d_items, gt.items))) return rc;
+        if ((rc = upload(ctx, &g.d_tiles, gt.tiles))) return rc;
         if ((rc = upload(ctx, &g.d_bu, gt.u_bu))) return rc;
         if ((rc = upload(ctx, &g.d_au, gt.u_au))) return rc;
         if ((rc = upload(ctx, &g.d_uoff, gt.uoff))) return rc;
@@ -661,6 +667,7 @@ int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, i
         MacArgs m{};
         m.G = g.d_G[0];
         m.chunks = g.d_chunks; m.items = g.d_items; m.u_bu = g.d_bu; m.u_au = g.d_au; m.uoff = g.d_uoff;
+        m.tiles = g.d_tiles;
         m.S = g.d_S_fwd; m.S_off = g.d_Soff_fwd; m.S_tstride = a.S_tstride;
         m.s2 = g.d_s2; m.enable = g.d_enable;
         m.nP = nP;

@@ -65,6 +65,7 @@ struct MacArgs {
     long long A_tstride, A_pstride;
     int N, fY, fX, rY, rX, rfX, nu;
     int T;                       // frames (grid z covers them in blocks of FB)
+    const TileDesc* tiles;       // residue-class tiles (matrix-core kernel)
     int rs;                      // P/M table residue-row stride (mac_row_stride)
 };
 
@@ -136,7 +137,8 @@ hipError_t launch_psf_cols(const PsfColArgs& a, int nu, hipStream_t s);
 hipError_t launch_spec_pack(const SpecPackArgs& a, int nchunks, hipStream_t s);
 int mac_row_stride(int slots, int nu);
 size_t mac_lds(int slots, int nu, int N, int fb);
-hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks, int nplanes_grid, hipStream_t s);
+hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks, int nplanes_grid, bool mfma,
+                      hipStream_t s);
 hipError_t launch_ifft_cols(const IfftColArgs& a, int nbatch, hipStream_t s);
 hipError_t launch_ifft_rows_sum(const IfftRowSumArgs& a, int Y, int T, hipStream_t s);
 hipError_t launch_ifft_rows_plane(const IfftRowPlaneArgs& a, int nplanes, hipStream_t s);

@@ -330,6 +330,73 @@ __device__ __forceinline__ void mac_spectrum(float2 (&acc)[FB][IPT][4], const fl
 #define FLF_MAC_WAVES 4
 #endif
 
+// Per-plane set-up of a MAC workgroup: flip factors, then per frame f < nf
+// the chunk's small-FFT block staged in LDS and its P/M table
+//   pm[f][rho * rs + u * NPT + pt] = (P, M),  P = s_plain + s_conj,
+//   M = i (s_plain - s_conj)
+// for residue rho, stored spectrum u and pair type pt (all threads; contains
+// barriers; frames past nf keep the tables of earlier planes and their
+// accumulators are never stored).
+template <int SLOTS>
+__device__ __forceinline__ void mac_tables(const MacArgs& a, const ChunkDesc* ch, int p, int t0, int nf,
+                                           float4* pm_tab, float2* s_stage, float2* psi, int rs) {
+    constexpr int NUSE = SLOTS == 4 ? 8 : 4;
+    constexpr int NPT = SLOTS == 4 ? 4 : 2;
+    constexpr int NRES = SLOTS == 4 ? 8 : 4;
+    constexpr int NH = NRES / 2;
+    const int tid = threadIdx.x;
+    const int N = a.N, NN = N * N, c = N / 2;
+    const int nu = a.nu;
+    const int tab = NRES * rs;
+    const int s2 = a.s2[p];
+    __syncthreads();
+    // flip factors per residue: exp(-2 pi i (s2-1) kt / f)
+    if (tid < 2 * NRES) {
+        const int rho = tid >> 1;
+        const bool isY = (tid & 1) == 0;
+        const int k = isY ? ch->res0[rho] : ch->res1[rho];
+        const int r = isY ? a.rY : a.rX;
+        const int f = isY ? a.fY : a.fX;
+        const int kt = (2 * k < r) ? k : ((2 * k > r) ? k - r : 0);
+        int m = (int)(((long long)(s2 - 1) * kt) % f);
+        if (m < 0) m += f;
+        float sn, cs;
+        sincospif(-2.0f * (float)m / (float)f, &sn, &cs);
+        psi[tid] = make_float2(cs, sn);
+    }
+    for (int f = 0; f < nf; ++f) {
+        if (f > 0) __syncthreads();
+        const float2* Sp = a.S + a.S_off[p] + (long long)(t0 + f) * a.S_tstride + (long long)blockIdx.x * NN * NH;
+        for (int i = tid; i < NN * NH; i += blockDim.x) s_stage[i] = Sp[i];
+        __syncthreads();
+        float4* pmf = pm_tab + f * tab;
+        for (int idx = tid; idx < NRES * nu * NPT; idx += blockDim.x) {
+            const int pt = idx % NPT;
+            const int ru = idx / NPT;
+            const int u = ru % nu, rho = ru / nu;
+            const int rb0 = rho < NH ? rho : rho - NH;
+            float2 sv[2];
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+                const int off = a.uoff[u * NUSE + (w == 0 ? kPairPlain[pt] : kPairConj[pt])];
+                float2 r = make_float2(0.f, 0.f);
+                if (off < NN) {
+                    const int bo = off / N, ao = off - bo * N;
+                    r = s_stage[off * NH + rb0];
+                    if (rho >= NH) r = cconj(r);
+                    if (bo > c) r = cmul(r, psi[2 * rho]);
+                    if (ao > c) r = cmul(r, psi[2 * rho + 1]);
+                }
+                sv[w] = r;
+            }
+            const float2 P = cadd(sv[0], sv[1]);
+            const float2 d = csub(sv[0], sv[1]);
+            pmf[rho * rs + u * NPT + pt] = make_float4(P.x, P.y, -d.y, d.x);   // M = i * d
+        }
+    }
+    __syncthreads();
+}
+
 // For an item with base alias (m2, n2) (half units) and output slots
 // P0..P3, every output accumulates
 //   A_j = w4^(outer_j) * sum_u sum_i [ P_{j,i,u} Re g_i + M_{j,i,u} Im g_i ]
@@ -348,8 +415,6 @@ template <int SLOTS, int IPT, int FB>
 __global__ __launch_bounds__(FLF_CHUNK_MAX) __attribute__((amdgpu_waves_per_eu(IPT * FB >= 4 ? 2 : (IPT * FB == 2 ? 3 : FLF_MAC_WAVES))))
 void k_mac(MacArgs a) {
     extern __shared__ float4 pm_tab[];            // [FB][NRES][rs: nu*NPT + pad], then staging
-    constexpr int NUSE = SLOTS == 4 ? 8 : 4;
-    constexpr int NPT = SLOTS == 4 ? 4 : 2;
     constexpr int NRES = SLOTS == 4 ? 8 : 4;      // residues per orbit (canonical list)
     constexpr int NH = NRES / 2;                  // residues stored by the small-FFT kernel
     constexpr int UNR = IPT >= 2 ? 1 : FLF_MAC_UNR;   // spectra whose loads are issued together
@@ -359,7 +424,7 @@ void k_mac(MacArgs a) {
     const int tid = threadIdx.x;
     const bool valid = tid * IPT < n_items;
     const int tsafe = valid ? tid : 0;           // idle lanes read item 0 (in bounds) and never write
-    const int N = a.N, NN = N * N, c = N / 2, fourN = 4 * N;
+    const int N = a.N, NN = N * N, fourN = 4 * N;
     const int t0 = blockIdx.z * FB;
     const int nf = min(FB, a.T - t0);            // frames of this block (block-uniform)
     const int nu = a.nu;
@@ -407,53 +472,7 @@ void k_mac(MacArgs a) {
     // barriers; frames past nf keep the tables of earlier planes and their
     // accumulators are never stored).
     auto prologue = [&](int p) {
-        const int s2 = a.s2[p];
-        __syncthreads();
-        // flip factors per residue: exp(-2 pi i (s2-1) kt / f)
-        if (tid < 2 * NRES) {
-            const int rho = tid >> 1;
-            const bool isY = (tid & 1) == 0;
-            const int k = isY ? ch->res0[rho] : ch->res1[rho];
-            const int r = isY ? a.rY : a.rX;
-            const int f = isY ? a.fY : a.fX;
-            const int kt = (2 * k < r) ? k : ((2 * k > r) ? k - r : 0);
-            int m = (int)(((long long)(s2 - 1) * kt) % f);
-            if (m < 0) m += f;
-            float sn, cs;
-            sincospif(-2.0f * (float)m / (float)f, &sn, &cs);
-            psi[tid] = make_float2(cs, sn);
-        }
-        for (int f = 0; f < nf; ++f) {
-        if (f > 0) __syncthreads();
-        const float2* Sp = a.S + a.S_off[p] + (long long)(t0 + f) * a.S_tstride + (long long)blockIdx.x * NN * NH;
-        for (int i = tid; i < NN * NH; i += blockDim.x) s_stage[i] = Sp[i];
-        __syncthreads();
-        float4* pmf = pm_tab + f * tab;
-        for (int idx = tid; idx < NRES * nu * NPT; idx += blockDim.x) {
-            const int pt = idx % NPT;
-            const int ru = idx / NPT;
-            const int u = ru % nu, rho = ru / nu;
-            const int rb0 = rho < NH ? rho : rho - NH;
-            float2 sv[2];
-#pragma unroll
-            for (int w = 0; w < 2; ++w) {
-                const int off = a.uoff[u * NUSE + (w == 0 ? kPairPlain[pt] : kPairConj[pt])];
-                float2 r = make_float2(0.f, 0.f);
-                if (off < NN) {
-                    const int bo = off / N, ao = off - bo * N;
-                    r = s_stage[off * NH + rb0];
-                    if (rho >= NH) r = cconj(r);
-                    if (bo > c) r = cmul(r, psi[2 * rho]);
-                    if (ao > c) r = cmul(r, psi[2 * rho + 1]);
-                }
-                sv[w] = r;
-            }
-            const float2 P = cadd(sv[0], sv[1]);
-            const float2 d = csub(sv[0], sv[1]);
-            pmf[rho * rs + u * NPT + pt] = make_float4(P.x, P.y, -d.y, d.x);   // M = i * d
-        }
-        }
-        __syncthreads();
+        mac_tables<SLOTS>(a, ch, p, t0, nf, pm_tab, s_stage, psi, rs);
     };
 
     auto gbase = [&](int p) { return a.G + item_base * SLOTS * nu * a.nP + ((long long)p * nu) * ustride; };
@@ -551,6 +570,215 @@ void k_mac(MacArgs a) {
 #pragma unroll
                 for (int j = 0; j < SLOTS; ++j)
                     if ((ip[q].writemask >> j) & 1) A[ip[q].pos[j]] = tot[f][q][j];
+        }
+    }
+}
+
+// Matrix-core variant for FB >= 2 frames per workgroup.  The items of a chunk
+// are grouped into tiles of <= 16 items of one residue class (TileDesc); for a
+// tile and a stored spectrum u the MAC of all its items and frames is the
+// product  acc[item][(f, j, re/im)] += sum_k g[item][k] W_u[k][(f, j, re/im)]
+// with k = (slot i, Re/Im g_i) and W_u made of the same P/M table entries as
+// the vector kernel (P for Re g, sg*M for Im g).  It runs on
+// v_mfma_f32_16x16x4_f32: lane l supplies A[l % 16][l / 16] (one float of the
+// item's slot-pair float4, read straight from the cached spectra),
+// B[l / 16][l % 16] (one float of the P/M table in LDS), and holds
+// D[4 (l / 16) + r][l % 16].  The two lanes of a column pair (re, im) swap
+// values to apply the complex epilogue phases.  A workgroup has MFMA_WAVES
+// waves; wave w owns tiles w, w + MFMA_WAVES, ...; FP32 operands and
+// accumulation throughout.  Opt-in (FLF_MAC_MFMA=1): measured 2.2x slower
+// than the vector kernel at 4 frames on MI355X, with 5.8 LDS bank-conflict
+// cycles per B-operand read (the 64 lanes gather P/M entries of up to 16
+// table rows that share banks), so the vector kernel stays the default.
+constexpr int MFMA_WAVES = 8;
+template <int SLOTS, int FB>
+__global__ __launch_bounds__(64 * MFMA_WAVES) void k_mac_mfma(MacArgs a) {
+    extern __shared__ float4 pm_tab[];            // [FB][NRES][rs], then staging
+    constexpr int NPT = SLOTS == 4 ? 4 : 2;
+    constexpr int NRES = SLOTS == 4 ? 8 : 4;
+    constexpr int NH = NRES / 2;
+    constexpr int CPF = 2 * SLOTS;                // output columns per frame: (j, re/im)
+    constexpr int NT = (FB * CPF + 15) / 16;      // 16-column tiles
+    constexpr int KS = SLOTS / 2;                 // 4-deep k steps per spectrum (slot pairs)
+    constexpr int MAXT = (FLF_MAX_TILES + MFMA_WAVES - 1) / MFMA_WAVES;   // tiles per wave
+    const ChunkDesc* ch = a.chunks + blockIdx.x;
+    const long long item_base = ch->item_base;
+    const int n_items = ch->n_items;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int g = lane >> 4, ln = lane & 15;
+    const int N = a.N, NN = N * N, fourN = 4 * N;
+    const int t0 = blockIdx.z * FB;
+    const int nf = min(FB, a.T - t0);
+    const int nu = a.nu;
+    const int rs = a.rs;
+    const int tab = NRES * rs;
+    float2* s_stage = reinterpret_cast<float2*>(pm_tab + FB * tab);
+    float2* psi = s_stage + NN * NH;
+    const float* pmf = reinterpret_cast<const float*>(pm_tab);
+    const TileDesc* tiles = a.tiles + ch->tile_base;
+    const int ntiles = ch->n_tiles;
+
+    // This wave's tiles q: chunk tile wv + MFMA_WAVES q.  Per tile: A offset of the
+    // lane's item (floats, within a spectrum block) or -1, and the B table
+    // offsets (floats, per k step and column tile; bit 30 = negate, -1 = 0).
+    int aoff[MAXT];
+    int boff[MAXT][KS][NT];
+#pragma unroll
+    for (int q = 0; q < MAXT; ++q) {
+        const int tq = wv + MFMA_WAVES * q;
+        aoff[q] = -1;
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) boff[q][s][nt] = -1;
+        if (tq < ntiles) {
+            const TileDesc td = tiles[tq];
+            if (ln < td.count) aoff[q] = (td.start + ln) * 4 + g;
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const int i = 2 * s + (g >> 1), part = g & 1;
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    const int gc = nt * 16 + ln;
+                    const int f = gc / CPF, jj = (gc % CPF) >> 1, cc = gc & 1;
+                    if (f < FB) {
+                        const int pt = SLOTS == 4 ? kQuadPT[jj][i] : kPairPT[jj][i];
+                        const int sg = SLOTS == 4 ? kQuadSG[jj][i] : 1;
+                        const int off = (f * tab + td.res[jj] * rs + pt) * 4 + part * 2 + cc;
+                        boff[q][s][nt] = off | ((part && sg < 0) ? (1 << 30) : 0);
+                    }
+                }
+            }
+        }
+    }
+
+    v4f tot[MAXT][NT];
+#pragma unroll
+    for (int q = 0; q < MAXT; ++q)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) tot[q][nt] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    const float* Gf = reinterpret_cast<const float*>(a.G);
+    auto load_a = [&](float (&av)[MAXT][KS], int p, int u) {
+        const float* blk = Gf + (item_base * SLOTS * nu * a.nP + ((long long)p * nu + u) * n_items * SLOTS) * 2;
+#pragma unroll
+        for (int q = 0; q < MAXT; ++q)
+#pragma unroll
+            for (int s = 0; s < KS; ++s) av[q][s] = aoff[q] >= 0 ? blk[s * n_items * 4 + aoff[q]] : 0.f;
+    };
+
+    const int pb = (int)(((long long)blockIdx.y * a.nP) / gridDim.y);
+    const int pe = (int)(((long long)(blockIdx.y + 1) * a.nP) / gridDim.y);
+    for (int p = pb; p < pe; ++p) {
+        if (!a.enable[p]) continue;
+        mac_tables<SLOTS>(a, ch, p, t0, nf, pm_tab, s_stage, psi, rs);
+        v4f acc[MAXT][NT];
+#pragma unroll
+        for (int q = 0; q < MAXT; ++q)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[q][nt] = v4f{0.f, 0.f, 0.f, 0.f};
+        // Ring of PD spectra in flight: buffer d holds spectrum u0 + d; after
+        // its products, the loads of spectrum u0 + d + PD are issued into it.
+        constexpr int PD = 4;
+        float av[PD][MAXT][KS];
+#pragma unroll
+        for (int d = 0; d < PD; ++d) load_a(av[d], p, min(d, nu - 1));
+        auto step = [&](float (&cur)[MAXT][KS], int u) {
+            const int ub = u * NPT * 4;
+#pragma unroll
+            for (int q = 0; q < MAXT; ++q) {
+                if (wv + MFMA_WAVES * q >= ntiles) continue;                // wave-uniform: tile absent
+#pragma unroll
+                for (int s = 0; s < KS; ++s)
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt) {
+                        const int bo = boff[q][s][nt];
+                        float bv = 0.f;
+                        if (bo >= 0) {
+                            bv = pmf[(bo & ((1 << 30) - 1)) + ub];
+                            if (bo & (1 << 30)) bv = -bv;
+                        }
+                        acc[q][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur[q][s], bv, acc[q][nt], 0, 0, 0);
+                    }
+            }
+        };
+        for (int u0 = 0; u0 < nu; u0 += PD) {
+#pragma unroll
+            for (int d = 0; d < PD; ++d) {
+                const int u = u0 + d;
+                if (u < nu) step(av[d], u);
+                if (u + PD < nu) load_a(av[d], p, u + PD);
+            }
+        }
+        // Epilogue of plane p: outer phase and 'same'-crop shift, per item and
+        // output slot (the same for every frame).
+        const int s2 = a.s2[p];
+        const int sig = s2 - 2 + N;
+        const int c2 = (s2 - 1) / 2;
+#pragma unroll
+        for (int q = 0; q < MAXT; ++q) {
+            const int tq = wv + MFMA_WAVES * q;
+            if (tq >= ntiles) continue;                                  // wave-uniform
+            const TileDesc td = tiles[tq];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int gc = nt * 16 + ln;
+                const int f = gc / CPF, jj = (gc % CPF) >> 1, cc = gc & 1;
+                float mine[4], other[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    mine[r] = acc[q][nt][r];
+                    other[r] = __shfl_xor(mine[r], 1);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int ir = 4 * g + r;
+                    if (f < FB && ir < td.count) {
+                        const ItemDesc& it = a.items[item_base + td.start + ir];
+                        const int m2 = it.m2[0], n2 = it.n2[0];
+                        const int outer = jj == 0 ? sig * (m2 + n2)
+                                        : jj == 1 ? sig * (n2 - m2)
+                                        : jj == 2 ? sig * (n2 + m2) : sig * (m2 - n2);
+                        const int pos = it.pos[jj];
+                        const int k0 = pos / a.rfX, k1 = pos - k0 * a.rfX;
+                        const float2 po = phase_frac(outer, fourN);
+                        const int e0 = (int)(((long long)c2 * k0) % a.fY);
+                        const int e1 = (int)(((long long)c2 * k1) % a.fX);
+                        float fr = (float)e0 / (float)a.fY + (float)e1 / (float)a.fX;
+                        fr -= rintf(fr);
+                        float sn, cs;
+                        sincospif(2.0f * fr, &sn, &cs);
+                        const float2 z = cc == 0 ? make_float2(mine[r], other[r]) : make_float2(other[r], mine[r]);
+                        const float2 v = cmul(cmul(z, po), make_float2(cs, sn));
+                        tot[q][nt][r] += cc == 0 ? v.x : v.y;
+                    }
+                }
+            }
+        }
+    }
+
+    // Store: lane (column f, j, re/im) writes its component of every item of
+    // the tile whose slot j owns the bin.
+#pragma unroll
+    for (int q = 0; q < MAXT; ++q) {
+        const int tq = wv + MFMA_WAVES * q;
+        if (tq >= ntiles) continue;
+        const TileDesc td = tiles[tq];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int gc = nt * 16 + ln;
+            const int f = gc / CPF, jj = (gc % CPF) >> 1, cc = gc & 1;
+            if (f >= nf) continue;
+            float* A = reinterpret_cast<float*>(a.A + (long long)(t0 + f) * a.A_tstride +
+                                                (long long)blockIdx.y * a.A_pstride);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int ir = 4 * g + r;
+                if (ir < td.count) {
+                    const ItemDesc& it = a.items[item_base + td.start + ir];
+                    if ((it.writemask >> jj) & 1) A[2LL * it.pos[jj] + cc] = tot[q][nt][r];
+                }
+            }
         }
     }
 }
@@ -890,11 +1118,23 @@ static void launch_mac_fb(const MacArgs& a, int fb, dim3 grid, dim3 block, size_
     else if (fb == 2) hipLaunchKernelGGL((k_mac<SLOTS, IPT, 2>), grid, block, lds, s, a);
     else hipLaunchKernelGGL((k_mac<SLOTS, IPT, 4>), grid, block, lds, s, a);
 }
-hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks, int nplanes_grid, hipStream_t s) {
+hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks, int nplanes_grid, bool mfma,
+                      hipStream_t s) {
     if ((fb != 1 && fb != 2 && fb != 4) || (ipt != 1 && ipt != 2) || a.T < 1 || a.rs != mac_row_stride(slots, a.nu))
         return hipErrorInvalidValue;
     const size_t lds = mac_lds(slots, a.nu, a.N, fb);
     dim3 grid(nchunks, nplanes_grid, (a.T + fb - 1) / fb);
+    if (mfma && fb >= 2 && ipt == 1 && a.tiles) {
+        dim3 block(64 * MFMA_WAVES);
+        if (slots == 4) {
+            if (fb == 2) hipLaunchKernelGGL((k_mac_mfma<4, 2>), grid, block, lds, s, a);
+            else hipLaunchKernelGGL((k_mac_mfma<4, 4>), grid, block, lds, s, a);
+        } else {
+            if (fb == 2) hipLaunchKernelGGL((k_mac_mfma<2, 2>), grid, block, lds, s, a);
+            else hipLaunchKernelGGL((k_mac_mfma<2, 4>), grid, block, lds, s, a);
+        }
+        return hipGetLastError();
+    }
     dim3 block(FLF_CHUNK_MAX / ipt);
     if (slots == 4) {
         if (ipt == 1) launch_mac_fb<4, 1>(a, fb, grid, block, lds, s);
@@ -937,7 +1177,9 @@ hipError_t configure_kernels(size_t max_lds) {
                         (const void*)k_mac<4, 1, 1>, (const void*)k_mac<4, 1, 2>, (const void*)k_mac<4, 1, 4>,
                         (const void*)k_mac<4, 2, 1>, (const void*)k_mac<4, 2, 2>, (const void*)k_mac<4, 2, 4>,
                         (const void*)k_mac<2, 1, 1>, (const void*)k_mac<2, 1, 2>, (const void*)k_mac<2, 1, 4>,
-                        (const void*)k_mac<2, 2, 1>, (const void*)k_mac<2, 2, 2>, (const void*)k_mac<2, 2, 4>};
+                        (const void*)k_mac<2, 2, 1>, (const void*)k_mac<2, 2, 2>, (const void*)k_mac<2, 2, 4>,
+                        (const void*)k_mac_mfma<4, 2>, (const void*)k_mac_mfma<4, 4>,
+                        (const void*)k_mac_mfma<2, 2>, (const void*)k_mac_mfma<2, 4>};
     for (const void* k : ks) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
         if (e != hipSuccess) return e;

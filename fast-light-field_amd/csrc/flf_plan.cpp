@@ -215,15 +215,16 @@ std::string build_grid_tables(int N, int fY, int fX, GridTables& gt, int items_p
         }
         gt.max_res = std::max(gt.max_res, (int)res.size());
         gt.real_items += (long long)its.size();
-        // Order by residue tuple so that runs of ipt items share their
-        // residues; pad every class to a multiple of ipt with non-writing copies.
+        // Order by residue tuple (residue class): the matrix-core MAC works on
+        // tiles of one class, and runs of ipt items share their residues.  For
+        // ipt > 1, pad every class to a multiple of ipt with non-writing copies.
         const int ipt = gt.ipt;
+        std::stable_sort(its.begin(), its.end(), [](const ItemDesc& x, const ItemDesc& y) {
+            for (int j = 0; j < 4; ++j)
+                if (x.res[j] != y.res[j]) return x.res[j] < y.res[j];
+            return false;
+        });
         if (ipt > 1) {
-            std::stable_sort(its.begin(), its.end(), [](const ItemDesc& x, const ItemDesc& y) {
-                for (int j = 0; j < 4; ++j)
-                    if (x.res[j] != y.res[j]) return x.res[j] < y.res[j];
-                return false;
-            });
             std::vector<ItemDesc> padded;
             size_t i0 = 0;
             while (i0 < its.size()) {
@@ -252,6 +253,24 @@ std::string build_grid_tables(int N, int fY, int fX, GridTables& gt, int items_p
                 ch.res1[q] = q < ch.n_res ? res[q].second : 0;
             }
             gt.items.insert(gt.items.end(), its.begin() + s, its.begin() + s + ch.n_items);
+            // Tiles: runs of one residue class, at most FLF_TILE items each.
+            ch.tile_base = (int)gt.tiles.size();
+            int i0 = 0;
+            while (i0 < ch.n_items) {
+                const ItemDesc& first = its[s + i0];
+                int i1 = i0;
+                while (i1 < ch.n_items && i1 - i0 < FLF_TILE &&
+                       std::equal(its[s + i1].res, its[s + i1].res + 4, first.res))
+                    ++i1;
+                TileDesc td{};
+                td.start = i0;
+                td.count = i1 - i0;
+                for (int j = 0; j < 4; ++j) td.res[j] = first.res[j];
+                gt.tiles.push_back(td);
+                i0 = i1;
+            }
+            ch.n_tiles = (int)gt.tiles.size() - ch.tile_base;
+            if (ch.n_tiles > FLF_MAX_TILES) return "internal: too many residue-class tiles in a chunk";
             gt.chunks.push_back(ch);
         }
     }

@@ -32,6 +32,7 @@ struct GridTables {
     int nuse = 0;                // uses per stored spectrum: 8 (quads) or 4 (pairs)
     std::vector<ChunkDesc> chunks;
     std::vector<ItemDesc> items;
+    std::vector<TileDesc> tiles;
 };
 
 // Builds the tables; returns an empty string or an error message.

@@ -6,6 +6,8 @@
 #define FLF_MAX_RES 8          // residues per orbit (D4 orbit of a reduced bin)
 #define FLF_CHUNK_MAX 256      // items per MAC workgroup
 #define FLF_MAX_N 31           // largest supported Nnum (s-table = N*N entries)
+#define FLF_TILE 16            // items per matrix-core tile (one residue class)
+#define FLF_MAX_TILES 24       // tiles per chunk: <= FLF_CHUNK_MAX / FLF_TILE + FLF_MAX_RES
 
 struct RadixPlan {
     int L;                      // transform length
@@ -18,8 +20,18 @@ struct ChunkDesc {
     long long item_base;        // first item (global index within the grid group)
     int n_items;
     int n_res;
+    int tile_base;              // first tile (global index within the grid group)
+    int n_tiles;
     int res0[FLF_MAX_RES];      // residue row    (kappa0 in [0, rY))
     int res1[FLF_MAX_RES];      // residue column (kappa1 in [0, rX))
+};
+
+// A run of at most FLF_TILE consecutive items of one chunk that share their
+// slot residues (one residue class): the unit of the matrix-core MAC.
+struct TileDesc {
+    int start;                  // first item, relative to the chunk
+    int count;                  // items in the tile (<= FLF_TILE)
+    unsigned char res[4];       // slot residues of every item in the tile
 };
 
 // One item: 4 (square grid) or 2 (non-square) half-plane bins related by

@@ -38,7 +38,7 @@ def _plain(proj, hm, Htf, im, iters):
 
 def test_guard_rejects_bad_guess_and_matches_plain(projector):
     """A network whose guess is ~0 (head bias -30) fails the guard; the run
-    then continues from the first plain iterate and equals plain RL."""
+    then continues from the first plain iterate and reproduces plain RL."""
     from flfhip.lensletnet import LearnedInit, LensletNet
     from flfhip.lfdeconv import deconv_rl_device
     hm, im, Htf = _problem(projector, "rl300", 20, seed=11, peak=1000.0)
@@ -48,13 +48,16 @@ def test_guard_rejects_bad_guess_and_matches_plain(projector):
     net.set_params(p)
     li = LearnedInit(net, projector, hm)
     Xp, fits_p = _plain(projector, hm, Htf, im, 4)
-    Xl, n, info = li.deconvolve(im, Htf, 4, targetFit=fits_p[4])
+    # The guard projects guess and plain iterate as one two-frame call (the
+    # matrix-core MAC), the plain run one frame at a time: the same values up
+    # to FP32 summation order, hence the 1e-6 margins.
+    Xl, n, info = li.deconvolve(im, Htf, 4, targetFit=fits_p[4] * (1 + 1e-6))
     print("guard", info, "plain fits", fits_p, flush=True)
     assert not info["accepted"][0]
     assert info["fit_guess"][0] > info["fit_plain1"][0]
-    assert abs(info["fit_plain1"][0] - fits_p[1]) <= 1e-9 * fits_p[1]
+    assert abs(info["fit_plain1"][0] - fits_p[1]) <= 1e-6 * fits_p[1]
     assert n == 4
-    assert rel_l2(Xl.cpu().numpy(), Xp.cpu().numpy()) < 1e-6
+    assert rel_l2(Xl.cpu().numpy(), Xp.cpu().numpy()) < 1e-5
 
 
 def test_trained_network_saves_iterations(projector):

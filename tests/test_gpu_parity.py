@@ -92,3 +92,32 @@ def test_frame_capacity(projector):
             assert rel_l2(fwd[t], ref_f) < 1e-5, (T, t)
             ref_b = oproj.project_special(img[t][None], hm.Hcc(1, True))[0]
             assert rel_l2(bwd[1, t], ref_b) < 1e-5, (T, t)
+
+
+def test_multiframe_matrix_core_kernel():
+    """The opt-in matrix-core MAC (FLF_MAC_MFMA=1, read when a projector is
+    created) against the oracle on 2 and 3 frames, square and non-square."""
+    import os
+    from flfhip.projector import Projector_hip
+    old = os.environ.get("FLF_MAC_MFMA")
+    os.environ["FLF_MAC_MFMA"] = "1"
+    try:
+        proj = Projector_hip()
+    finally:
+        if old is None:
+            del os.environ["FLF_MAC_MFMA"]
+        else:
+            os.environ["FLF_MAC_MFMA"] = old
+    for T, Y, X in [(3, 150, 150), (2, 150, 300)]:
+        hm = _hm_random(15, [31, 45], seed=T + Y + 1)
+        rng = np.random.default_rng(7)
+        vol = rng.random((2, T, Y, X)).astype(np.float32)
+        img = rng.random((T, Y, X)).astype(np.float32)
+        print("matrix-core multiframe", T, Y, X, flush=True)
+        fwd = proj.ForwardProjectACC(hm, vol, [0, 1])
+        bwd = proj.BackwardProjectACC(hm, img, [0, 1])
+        for t in range(T):
+            ref_f = sum(oproj.project_special(vol[z, t][None], hm.Hcc(z, False))[0] for z in range(2))
+            assert rel_l2(fwd[t], ref_f) < 1e-5, t
+            ref_b = oproj.project_special(img[t][None], hm.Hcc(1, True))[0]
+            assert rel_l2(bwd[1, t], ref_b) < 1e-5, t
