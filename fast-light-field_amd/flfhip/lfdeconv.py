@@ -100,6 +100,40 @@ def deconv_rl_device(projector, hMatrix, Htf, maxIter, X, im=None, prevedel=True
     return it
 
 
+class RLGraph(object):
+    """One Prevedel RL iteration (forward projection + fused back projection
+    and update of X in place) captured as a HIP graph on the current device,
+    replayed once per iteration: the launches of an iteration go to the GPU
+    as one graph instead of one call each.  X and Htf are the static buffers
+    the graph reads and writes (copy new data into them, do not rebind).
+    The capture happens on first use, after one eager warm-up iteration on a
+    scratch copy, so every lazy allocation and table upload is done before
+    capture; the captured work contains no memset or host-memory copy."""
+
+    def __init__(self, projector, hMatrix, Htf, X, planes=None):
+        self.projector, self.hMatrix, self.Htf, self.X, self.planes = projector, hMatrix, Htf, X, planes
+        self.graph = None
+
+    def _capture(self):
+        p, hm = self.projector, self.hMatrix
+        Xw = self.X.clone()
+        HX = p.forward_device(hm, Xw, self.planes)
+        p.rl_update_device(hm, HX, Xw, Htf=self.Htf, mode=1, planes=self.planes)
+        torch.cuda.synchronize()
+        del Xw, HX
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            HX = p.forward_device(hm, self.X, self.planes)
+            p.rl_update_device(hm, HX, self.X, Htf=self.Htf, mode=1, planes=self.planes)
+        self._hx = HX
+
+    def run(self, iterations):
+        if self.graph is None:
+            self._capture()
+        for _ in range(iterations):
+            self.graph.replay()
+
+
 def DeconvRL(hMatrix, Htf, maxIter, Xguess, logPrint=True, numjobs=1, projector=None, im=None, progress=None,
              prevedel=True, trustMask=None, targetFit=None, fitOut=None):
     """lfdeconv.py:72-202 (Prevedel update by default; standard RL with an
