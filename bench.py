@@ -101,6 +101,7 @@ def main():
                     help="frames reconstructed together per step (value = frames x steps / time)")
     ap.add_argument("--plain-only", action="store_true", help="time plain RL only")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -128,6 +129,12 @@ def main():
     torch.cuda.synchronize()
     t_prep = time.time() - t
     log("spectra precompute: %.2fs, device bytes %.2f GB" % (t_prep, proj.lib.flf_device_bytes(proj._ctx) / 1e9))
+    tuned = None
+    if not args.no_autotune:
+        from flfhip import autotune
+        t = time.time()
+        tuned, _ = autotune.tune(proj, hm, T, Y, X)
+        log("autotune %.1fs: %s" % (time.time() - t, tuned))
 
     vol = proj.asnative(w["vol"])
     clean = proj.asnumpy(proj.forward_device(hm, vol))
@@ -239,7 +246,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "init": "learned (guarded)" if learned else "uniform (plain RL)",
                    "target_datafit": fit_plain,
-                   "frames": T,
+                   "frames": T, "launch_options": tuned,
                    "plain": {"ms": plain_ms, "volumes_per_s": T * 1e3 / plain_ms, "iterations": iters},
                    "learned": learned,
                    "speedup_vs_plain": (plain_ms / learned["ms"]) if learned else None},

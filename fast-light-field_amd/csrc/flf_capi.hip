@@ -563,6 +563,41 @@ int flf_set_frames(flf_ctx* ctx, int T) {
 
 int flf_frame_capacity(const flf_ctx* ctx) { return ctx && ctx->planned ? ctx->Tcap : 0; }
 
+int flf_set_option(flf_ctx* ctx, const char* name, int value) {
+    if (!ctx || !name) return FLF_EINVAL;
+    const std::string n(name);
+    if (n == "fwd_split") {
+        if (value < 0 || value > FWD_SPLIT_MAX) return fail(ctx, FLF_EINVAL, "fwd_split must be 0..4");
+        ctx->fwd_split = value;
+    } else if (n == "mac_frames") {
+        if (value != 1 && value != 2 && value != 4) return fail(ctx, FLF_EINVAL, "mac_frames must be 1, 2 or 4");
+        ctx->fb_max = value;
+    } else if (n == "mac_mfma") {
+        ctx->mfma = value != 0;
+    } else if (n == "ifft_col_batch") {
+        if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");
+        if (value < 1 || value > 8) return fail(ctx, FLF_EINVAL, "ifft_col_batch must be 1..8");
+        for (auto& g : ctx->groups)
+            if (ifft_cols_lds(value, g.gt.fY) > ctx->max_lds)
+                return fail(ctx, FLF_EINVAL, "ifft_col_batch too large for the LDS at this Fourier grid");
+        for (auto& g : ctx->groups) g.CBi = value;
+    } else {
+        return fail(ctx, FLF_EINVAL, "unknown option " + n);
+    }
+    return FLF_OK;
+}
+
+int flf_get_option(const flf_ctx* ctx, const char* name, int* value) {
+    if (!ctx || !name || !value) return FLF_EINVAL;
+    const std::string n(name);
+    if (n == "fwd_split") *value = ctx->fwd_split;
+    else if (n == "mac_frames") *value = ctx->fb_max;
+    else if (n == "mac_mfma") *value = ctx->mfma ? 1 : 0;
+    else if (n == "ifft_col_batch") *value = ctx->groups.empty() ? 0 : ctx->groups[0].CBi;
+    else return FLF_EINVAL;
+    return FLF_OK;
+}
+
 int flf_plane_fshape(const flf_ctx* ctx, int z, int* fY, int* fX) {
     if (!ctx || !ctx->planned || z < 0 || z >= ctx->nz || !fY || !fX) return FLF_EINVAL;
     const Group& g = ctx->groups[ctx->group_of[z]];

@@ -38,6 +38,8 @@ SIGNATURES = {
     "flf_grid_check": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.POINTER(_c.c_longlong)]),
     "flf_set_frames": (_c.c_int, [_c.c_void_p, _c.c_int]),
     "flf_frame_capacity": (_c.c_int, [_c.c_void_p]),
+    "flf_set_option": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.c_int]),
+    "flf_get_option": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.POINTER(_c.c_int)]),
     "flf_rl_scale": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_longlong, _c.c_void_p]),
     "flf_net_create": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.POINTER(_c.c_void_p)]),
     "flf_net_destroy": (None, [_c.c_void_p]),

@@ -123,6 +123,15 @@ class Projector_hip(object):
         for d in directions:
             self._ensure_psf(hMatrix, planes, d)
 
+    def set_option(self, name, value):
+        """Launch-shape option of the library (flf_set_option)."""
+        self._check(self.lib.flf_set_option(self._ctx, name.encode(), int(value)), "flf_set_option(%s)" % name)
+
+    def get_option(self, name):
+        v = ctypes.c_int()
+        self._check(self.lib.flf_get_option(self._ctx, name.encode(), ctypes.byref(v)), "flf_get_option(%s)" % name)
+        return v.value
+
     def fshape(self, cc):
         fy, fx = ctypes.c_int(), ctypes.c_int()
         self._check(self.lib.flf_plane_fshape(self._ctx, int(cc), ctypes.byref(fy), ctypes.byref(fx)), "fshape")

@@ -65,6 +65,18 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
 int flf_set_frames(flf_ctx* ctx, int T);
 int flf_frame_capacity(const flf_ctx* ctx);
 
+/* Launch-shape options (no reference counterpart; the reference's
+ * SetNumThreadsToUse / SetPlanningMode, py_light_field.cpp:216-241, are its
+ * tuning knobs).  Names and values:
+ *   "fwd_split"      0 (automatic) or 1..4 plane ranges of the forward MAC
+ *   "mac_frames"     1, 2 or 4 frames per MAC workgroup (upper bound)
+ *   "mac_mfma"       0/1: matrix-core MAC for >= 2 frames per workgroup
+ *   "ifft_col_batch" 1..8 columns per column-transform workgroup (after plan)
+ * Results do not depend on them beyond FP32 rounding.  flfhip.autotune picks
+ * them by timing. */
+int flf_set_option(flf_ctx* ctx, const char* name, int value);
+int flf_get_option(const flf_ctx* ctx, const char* name, int* value);
+
 /* Fourier grid of plane z after flf_plan (fshape of projector.py:123). */
 int flf_plane_fshape(const flf_ctx* ctx, int z, int* fY, int* fX);
 

@@ -142,3 +142,10 @@ def test_synthetic_psf_normalised_and_symmetric():
     # that offset's lenslet centre (checked on the centre offset and a corner).
     np.testing.assert_allclose(H[7, 7], H[7, 7][::-1, :], atol=1e-7)
     np.testing.assert_allclose(H[0, 0], H[0, 0].T, atol=1e-7)
+
+
+def test_option_entry_points_reject_null():
+    lib = _lib.load()
+    assert lib.flf_set_option(None, b"fwd_split", 1) == -1
+    v = ctypes.c_int()
+    assert lib.flf_get_option(None, b"fwd_split", ctypes.byref(v)) == -1

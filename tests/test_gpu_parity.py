@@ -121,3 +121,28 @@ def test_multiframe_matrix_core_kernel():
             assert rel_l2(fwd[t], ref_f) < 1e-5, t
             ref_b = oproj.project_special(img[t][None], hm.Hcc(1, True))[0]
             assert rel_l2(bwd[1, t], ref_b) < 1e-5, t
+
+
+def test_autotune_and_options_keep_parity(projector):
+    """The autotuner's choices, and every single option value, leave the
+    projections within the parity bar."""
+    from flfhip import autotune
+    hm = _hm_random(15, [31, 45, 61], seed=31)
+    rng = np.random.default_rng(8)
+    T, Y, X = 2, 120, 150
+    vol = rng.random((3, T, Y, X)).astype(np.float32)
+    ref = [sum(oproj.project_special(vol[z, t][None], hm.Hcc(z, False))[0] for z in range(3)) for t in range(T)]
+    chosen, timings = autotune.tune(projector, hm, T, Y, X, reps=1)
+    print("autotune", chosen, flush=True)
+    assert set(chosen) >= {"fwd_split", "ifft_col_batch", "mac_frames"}
+    settings = [dict(chosen)] + [{"fwd_split": v} for v in (1, 4)] + [{"ifft_col_batch": 2}, {"mac_frames": 1}]
+    for opts in settings:
+        for k, v in opts.items():
+            projector.set_option(k, v)
+        fwd = projector.ForwardProjectACC(hm, vol, [0, 1, 2])
+        for t in range(T):
+            assert rel_l2(fwd[t], ref[t]) < 1e-5, (opts, t)
+    for k, v in {"fwd_split": 0, "mac_frames": 2, "ifft_col_batch": 8}.items():
+        projector.set_option(k, v)
+    with pytest.raises(Exception):
+        projector.set_option("no_such_option", 1)
