@@ -61,6 +61,11 @@ struct flf_ctx {
     std::vector<int> s2, group_of, idx_in_group;
     std::vector<Group> groups;
     RowPlaneDesc* d_rowdesc = nullptr;
+    // Ragged batches: per active frame the valid (Y, X) inside the planned
+    // canvas (flf_set_frame_dims); null pointer in the kernels = uniform.
+    int2* d_fdims = nullptr;
+    std::vector<int2> fdims_host;
+    bool ragged = false;
     float2* d_T_bwd = nullptr;
     long long* d_zeros = nullptr;     // zero offsets (length max planes per group)
     int* d_plane_ids = nullptr;
@@ -133,6 +138,8 @@ void free_all(flf_ctx* ctx) {
     ctx->bytes = 0;
     ctx->planned = false;
     ctx->d_rowdesc = nullptr;
+    ctx->d_fdims = nullptr;
+    ctx->ragged = false;
     ctx->d_T_bwd = nullptr;
     ctx->d_zeros = nullptr;
     ctx->d_plane_ids = nullptr;
@@ -258,6 +265,8 @@ long long s_frame_stride(const flf_ctx* ctx, const Group& g) {
     return (long long)g.gt.chunks.size() * ctx->N * ctx->N * g.gt.slots;
 }
 
+const int2* frame_dims(const flf_ctx* ctx) { return ctx->ragged ? ctx->d_fdims : nullptr; }
+
 // Small FFTs of the backward source (the image) for one group.
 int run_small_fft_image(flf_ctx* ctx, Group& g, const float* img, hipStream_t st) {
     SmallFFTArgs a{};
@@ -273,6 +282,7 @@ int run_small_fft_image(flf_ctx* ctx, Group& g, const float* img, hipStream_t st
     a.chunks = g.d_chunks; a.nchunks = (int)g.gt.chunks.size(); a.NH = g.gt.slots;
     a.fY = g.gt.fY; a.fX = g.gt.fX;
     a.S_tstride = s_frame_stride(ctx, g);
+    a.fdims = frame_dims(ctx);
     HIPCHK(launch_small_fft(a, 1, ctx->T, st));
     return FLF_OK;
 }
@@ -341,6 +351,7 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
     r.vol = vol; r.Htf = Htf; r.trust = trust;
     r.mode = mode; r.T = T; r.Y = ctx->Y; r.X = ctx->X;
     r.wbuf = wbuf;
+    r.fdims = frame_dims(ctx);
     HIPCHK(launch_ifft_rows_plane(r, (int)ids.size(), st));
     return FLF_OK;
 }
@@ -542,6 +553,9 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
     ctx->plane_ids_host.clear();
     if ((rc = dalloc(ctx, &ctx->d_fitpart, (size_t)2 * FIT_BLOCKS * T))) return rc;
     if ((rc = dalloc(ctx, &ctx->d_fitout, (size_t)2 * T))) return rc;
+    if ((rc = dalloc(ctx, &ctx->d_fdims, (size_t)T))) return rc;
+    ctx->fdims_host.assign(T, make_int2(Y, X));
+    ctx->ragged = false;
     ctx->planned = true;
     return FLF_OK;
 }
@@ -558,6 +572,41 @@ int flf_set_frames(flf_ctx* ctx, int T) {
         g.Tbwd_off = g.Tbwd_off_T[T - 1];
     }
     ctx->d_rowdesc = ctx->rowdesc_T[T - 1];
+    ctx->ragged = false;                  // frame sizes apply to one frame count
+    return FLF_OK;
+}
+
+int flf_set_frame_dims(flf_ctx* ctx, const int* dims, int T, void* stream) {
+    if (!ctx) return FLF_EINVAL;
+    if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");
+    if (T != ctx->T)
+        return fail(ctx, FLF_EINVAL, "frame sizes for " + std::to_string(T) + " frames, " + std::to_string(ctx->T) +
+                                         " active");
+    if (!dims) {
+        ctx->ragged = false;
+        return FLF_OK;
+    }
+    std::vector<int2> d(T);
+    bool full = true;
+    for (int t = 0; t < T; ++t) {
+        const int y = dims[2 * t], x = dims[2 * t + 1];
+        if (y < 1 || y > ctx->Y || x < 1 || x > ctx->X)
+            return fail(ctx, FLF_EINVAL, "frame " + std::to_string(t) + " size " + std::to_string(y) + " x " +
+                                             std::to_string(x) + " outside 1.." + std::to_string(ctx->Y) + " x 1.." +
+                                             std::to_string(ctx->X));
+        d[t] = make_int2(y, x);
+        full = full && y == ctx->Y && x == ctx->X;
+    }
+    if (full) {
+        ctx->ragged = false;
+        return FLF_OK;
+    }
+    // The host copy outlives the asynchronous upload (it is only rewritten
+    // by the next call, which is ordered after this one on the caller's stream).
+    ctx->fdims_host.assign(d.begin(), d.end());
+    HIPCHK(hipMemcpyAsync(ctx->d_fdims, ctx->fdims_host.data(), T * sizeof(int2), hipMemcpyHostToDevice,
+                          (hipStream_t)stream));
+    ctx->ragged = true;
     return FLF_OK;
 }
 
@@ -666,6 +715,7 @@ int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, i
     rs.Y = ctx->Y;
     rs.img = img;
     rs.img_tstride = (long long)ctx->Y * ctx->X;
+    rs.fdims = frame_dims(ctx);
     rs.maxfX = 1;
     for (size_t gi = 0; gi < ctx->groups.size(); ++gi) {
         Group& g = ctx->groups[gi];
@@ -698,6 +748,7 @@ int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, i
         a.chunks = g.d_chunks; a.nchunks = (int)gt.chunks.size(); a.NH = gt.slots;
         a.fY = gt.fY; a.fX = gt.fX;
         a.S_tstride = s_frame_stride(ctx, g);
+        a.fdims = frame_dims(ctx);
         HIPCHK(launch_small_fft(a, nP, T, st));
         MacArgs m{};
         m.G = g.d_G[0];

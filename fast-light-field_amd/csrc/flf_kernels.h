@@ -19,6 +19,7 @@ struct SmallFFTArgs {
     float2* S;                   // [chunk][N*N][NH] per source plane and frame
     const long long* S_off;      // per source plane z: element offset into S (device array)
     long long S_tstride;         // per frame
+    const int2* fdims;           // per frame valid (Y, X) <= (Y, X), or null (all full size)
 };
 
 struct PsfRowArgs {
@@ -100,6 +101,7 @@ struct IfftRowSumArgs {
     int X, Y;
     float* img;
     long long img_tstride;
+    const int2* fdims;           // per frame valid size; outside it the image is 0
 };
 
 struct RowPlaneDesc {
@@ -123,6 +125,7 @@ struct IfftRowPlaneArgs {
     int mode;                    // 0 store, 1 Prevedel update, 2 standard update
     int T, Y, X;
     int wbuf;                    // per-wave LDS buffer (float2): max over planes of fX/2 (even) or fX
+    const int2* fdims;           // per frame valid size; outside it the volume is set to 0
 };
 
 namespace flf {

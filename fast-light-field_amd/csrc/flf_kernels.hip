@@ -64,8 +64,10 @@ __global__ __launch_bounds__(64 * SFFT_WAVES) void k_small_fft(SmallFFTArgs a) {
     for (int i = threadIdx.x; i < rY; i += blockDim.x) twY[i] = a.twY[i];
     for (int i = threadIdx.x; i < rX; i += blockDim.x) twX[i] = a.twX[i];
     const float* src = a.src + a.src_off[z] + (long long)t * a.src_tstride;
-    const int Ys = (a.Y - b + N - 1) / N;
-    const int Xs = (a.X - aa + N - 1) / N;
+    // frame t's valid region (frames of one batch may be smaller than the plan)
+    const int Yv = a.fdims ? a.fdims[t].x : a.Y, Xv = a.fdims ? a.fdims[t].y : a.X;
+    const int Ys = max(0, (Yv - b + N - 1) / N);
+    const int Xs = max(0, (Xv - aa + N - 1) / N);
     for (int i = Ys * rXh + threadIdx.x; i < rY * rXh; i += blockDim.x) H[i] = make_float2(0.f, 0.f);
     __syncthreads();
     for (int q = wv; 4 * q < Ys; q += SFFT_WAVES) {
@@ -946,11 +948,12 @@ __global__ __launch_bounds__(256) void k_ifft_rows_sum(IfftRowSumArgs a) {
         }
     }
     if (y < a.Y) {
+        const int Yv = a.fdims ? a.fdims[t].x : a.Y, Xv = a.fdims ? a.fdims[t].y : a.X;
         float* out = a.img + (long long)t * a.img_tstride + (long long)y * a.X;
 #pragma unroll
         for (int q = 0; q < XPL; ++q) {
             const int x = lane + 64 * q;
-            if (x < a.X) out[x] = acc[q];
+            if (x < a.X) out[x] = (y < Yv && x < Xv) ? acc[q] : 0.f;
         }
     }
 }
@@ -974,6 +977,7 @@ __global__ __launch_bounds__(256) void k_ifft_rows_plane(IfftRowPlaneArgs a) {
     wave_c2r(buf, row, L, pd.rph, pd.twh, pd.rp, pd.twf, lane);
     const float sc = pd.scale;
     const long long o = (((long long)z * a.T + t) * a.Y + y) * a.X;
+    const int Yv = a.fdims ? a.fdims[t].x : a.Y, Xv = a.fdims ? a.fdims[t].y : a.X;
     constexpr int B = 8;                        // loads in flight per lane
     for (int x0 = lane; x0 < a.X; x0 += 64 * B) {
         float v[B], xo[B], hd[B];
@@ -990,7 +994,9 @@ __global__ __launch_bounds__(256) void k_ifft_rows_plane(IfftRowPlaneArgs a) {
 #pragma unroll
         for (int u = 0; u < B; ++u) {
             const int x = x0 + 64 * u;
-            if (x < a.X) {
+            if (x < a.X && (y >= Yv || x >= Xv)) {
+                a.vol[o + x] = 0.f;                 // outside this frame's valid region
+            } else if (x < a.X) {
                 if (a.mode == 0) {
                     a.vol[o + x] = v[u];
                 } else if (a.mode == 1) {

@@ -123,6 +123,22 @@ class Projector_hip(object):
         for d in directions:
             self._ensure_psf(hMatrix, planes, d)
 
+    def set_frame_dims(self, dims):
+        """Ragged batch: dims = [(Y_t, X_t)] per active frame, each within
+        the planned Y x X (frame t in the top-left corner of its slot), or
+        None for full-size frames (flf_set_frame_dims).  Outputs are zero
+        outside each frame's region; each frame's result equals the
+        projection at its own size."""
+        if dims is None:
+            self._check(self.lib.flf_set_frame_dims(self._ctx, None, int(self._frames), _stream_handle()),
+                        "flf_set_frame_dims")
+            return
+        flat = [int(v) for d in dims for v in d]
+        if len(flat) != 2 * len(dims):
+            raise ValueError("dims must be (Y, X) pairs")
+        arr = (ctypes.c_int * len(flat))(*flat)
+        self._check(self.lib.flf_set_frame_dims(self._ctx, arr, len(dims), _stream_handle()), "flf_set_frame_dims")
+
     def set_option(self, name, value):
         """Launch-shape option of the library (flf_set_option)."""
         self._check(self.lib.flf_set_option(self._ctx, name.encode(), int(value)), "flf_set_option(%s)" % name)

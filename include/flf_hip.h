@@ -65,6 +65,21 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
 int flf_set_frames(flf_ctx* ctx, int T);
 int flf_frame_capacity(const flf_ctx* ctx);
 
+/* Ragged batches: dims[2t], dims[2t+1] = valid rows and columns of frame t
+ * (1..Y, 1..X) for the T active frames (T must equal the active count);
+ * NULL returns to full-size frames.  Frame t occupies the top-left corner
+ * of its Y x X slot: forward projection reads only that region of every
+ * plane and writes zeros outside it, back projection reads only that region
+ * of the image and writes (or, in the update modes, sets) the volume to
+ * zero outside it.  Each frame's result equals the projection at its own
+ * size, because zero padding of a linear convolution does not change the
+ * cropped output.  The reference refuses mixed shapes in one call
+ * (deconvolve_time_series.py:21-23) and plans one convolution shape per
+ * image size (special_fftconvolve.py:142-174); here frames of uneven
+ * lenslet grids share one plan and one spectrum cache.  flf_set_frames
+ * resets to full-size frames.  The upload is ordered on stream. */
+int flf_set_frame_dims(flf_ctx* ctx, const int* dims, int T, void* stream);
+
 /* Launch-shape options (no reference counterpart; the reference's
  * SetNumThreadsToUse / SetPlanningMode, py_light_field.cpp:216-241, are its
  * tuning knobs).  Names and values:
