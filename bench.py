@@ -25,6 +25,11 @@ Also reported:
                 B*(4*nz+2)*Y*X*4 bytes / iteration time.
   cpu_baseline  the FP64 numpy oracle (single thread) timed on one plane's
                 projection at full size and scaled to a volume.
+
+--workload ragged64 measures the time series of uneven regions instead
+(64 frames of 300..900 px square, 31 planes, PSF 15..255 px, 8 Prevedel RL
+iterations from Htf per frame, all frames of a canvas bucket in one call per
+kernel): value = frames reconstructed per second.
 """
 import argparse
 import json
@@ -37,9 +42,10 @@ sys.path.insert(0, os.path.join(ROOT, "fast-light-field_amd"))
 
 import numpy as np  # noqa: E402
 
-from flfhip.workloads import WORKLOADS, setup  # noqa: E402
+from flfhip.workloads import RAGGED, WORKLOADS, setup  # noqa: E402
 
 PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
+PEAK_FP32 = 157.3e12  # MI355X FP32, flop/s
 
 
 def log(*a):
@@ -90,12 +96,172 @@ def measured_traffic(workload):
     return None
 
 
+def ragged_cpu_baseline(w):
+    """FP64 oracle (one thread) on the median-size frame: forward and backward
+    projection of the largest-PSF plane and forward of the median plane,
+    scaled by Fourier bins (each frame planned at its own size, as the
+    reference does) to every frame's Htf + 2 projections per iteration."""
+    sys.path.insert(0, ROOT)
+    from oracle import projector as oproj
+    from oracle import shapes as oshapes
+    hm, N, shapes = w["hm"], w["N"], w["shapes"]
+    bins = lambda sh, s2: (lambda f: f[0] * (f[1] // 2 + 1))(oshapes.convolution_shape(sh, (s2, s2), N, "5smooth")[0])
+    tmed = int(np.argsort([a * b for a, b in shapes])[len(shapes) // 2])
+    sh = shapes[tmed]
+    zmax = int(np.argmax(w["sides"]))
+    zmed = int(np.argsort(w["sides"])[len(w["sides"]) // 2])
+    t_total, b_total = 0.0, 0
+    for z, back in [(zmax, False), (zmax, True), (zmed, False)]:
+        src = w["vols"][tmed][z][None]
+        t = time.time()
+        oproj.project_special(src, hm.Hcc(z, back), pad="5smooth")
+        t_total += time.time() - t
+        b_total += bins(sh, w["sides"][z])
+    per_bin = t_total / b_total
+    all_bins = sum(bins(s, s2) for s in shapes for s2 in w["sides"])
+    per_series = per_bin * all_bins * (2 * w["iters"] + 1)
+    return {"value": len(shapes) / per_series, "unit": "volumes/s", "cores": 1, "kind": "port",
+            "sample": "FP64 numpy oracle (project_special), one thread: forward + backward projection of plane z=%d "
+                      "(PSF %d px) and forward of plane z=%d (PSF %d px) of the median frame (%dx%d), %.1f s, "
+                      "scaled by Fourier bins to %d frames x %d planes x (2 x %d iterations + 1) projections"
+                      % (w["zs"][zmax], w["sides"][zmax], w["zs"][zmed], w["sides"][zmed], sh[0], sh[1], t_total,
+                         len(shapes), len(w["zs"]), w["iters"])}
+
+
+def ragged_main(args):
+    import torch
+    from flfhip import autotune, synthetic
+    from flfhip.lfdeconv import _fit_from_sums
+    from flfhip.ragged import RaggedBatch
+    from flfhip.workloads import setup_ragged
+
+    w = setup_ragged(args.workload)
+    hm, shapes, iters, N = w["hm"], w["shapes"], w["iters"], w["N"]
+    nz, T = len(w["zs"]), len(shapes)
+    rb = RaggedBatch(hm, shapes, nbuckets=args.buckets)
+    log("buckets: %s" % [(c, len(i)) for c, i in rb.buckets])
+    torch.cuda.synchronize()
+    t = time.time()
+    for k, ((yc, xc), idx) in enumerate(rb.buckets):
+        rb.projectors[k].prepare(hm, len(idx), yc, xc)
+    torch.cuda.synchronize()
+    t_prep = time.time() - t
+    dev_bytes = sum(p.lib.flf_device_bytes(p._ctx) for p in rb.projectors)
+    log("spectra precompute: %.2fs, device bytes %.2f GB" % (t_prep, dev_bytes / 1e9))
+    tuned = []
+    for k, ((yc, xc), idx) in enumerate(rb.buckets):
+        if args.no_autotune:
+            tuned.append(None)
+            continue
+        t = time.time()
+        opt, _ = autotune.tune(rb.projectors[k], hm, len(idx), yc, xc, reps=1)
+        tuned.append(opt)
+        log("autotune bucket %d (%dx%d, %d frames) %.1fs: %s" % (k, yc, xc, len(idx), time.time() - t, opt))
+    clean = rb.forward(w["vols"])
+    ims = [synthetic.poisson_image(c[None], w["peak"], w["seed"] + t)[0] for t, c in enumerate(clean)]
+    del clean
+    staged = rb.stage(ims)
+    bufs = [(torch.empty((nz, len(idx), yc, xc), dtype=torch.float32, device=rb.projectors[k].device),
+             torch.empty((nz, len(idx), yc, xc), dtype=torch.float32, device=rb.projectors[k].device))
+            for k, ((yc, xc), idx) in enumerate(rb.buckets)]
+
+    def step():
+        rb.deconv_staged(staged, iters, out=bufs, graph=args.graph)
+
+    for _ in range(args.warmup):
+        step()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for p in rb.projectors:
+        p.profile(True)
+    torch.cuda.synchronize()
+    e0.record(s)
+    for _ in range(args.steps):
+        step()
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms_step = e0.elapsed_time(e1) / args.steps
+    for p in rb.projectors:
+        p.profile(False)
+    prof_steps = args.steps
+    if args.graph:
+        # graph replays record no per-launch events: take the kernel figures
+        # from one eager step
+        for p in rb.projectors:
+            p.profile_read()
+            p.profile(True)
+        rb.deconv_staged(staged, iters, out=bufs, graph=False)
+        torch.cuda.synchronize()
+        for p in rb.projectors:
+            p.profile(False)
+        prof_steps = 1
+    mac_ms = mac_n = mac_bytes = 0
+    for p in rb.projectors:
+        a, b, c = p.profile_read()
+        mac_ms, mac_n, mac_bytes = mac_ms + a, mac_n + b, mac_bytes + c
+    # data fit of every frame's reconstruction
+    fits = np.zeros(T)
+    for k, ((yc, xc), idx) in enumerate(rb.buckets):
+        with rb._bucket(k) as (p, _, _, _):
+            f = _fit_from_sums(p.datafit_device(p.forward_device(hm, bufs[k][1]), staged[k]))
+        fits[idx] = f
+    # algorithmic flops of the multiply-accumulate: 8 N^2 per half-plane bin,
+    # plane and frame, per projection (Htf + 2 per iteration)
+    mac_flops = 0.0
+    for k, ((yc, xc), idx) in enumerate(rb.buckets):
+        p = rb.projectors[k]
+        per = sum((lambda f: f[0] * (f[1] // 2 + 1))(p.fshape(z)) for z in range(nz))
+        mac_flops += 8.0 * N * N * per * len(idx) * (2 * iters + 1)
+    mac_flops *= prof_steps
+    mac_avg_ms = mac_ms / max(mac_n, 1)
+    achieved = (mac_bytes / max(mac_n, 1)) / (mac_avg_ms / 1e3)
+    flop_rate = mac_flops / (mac_ms / 1e3)
+    desc = ("ragged time series: %d frames, Nnum=%d, lenslet grids %d..%d square (%d..%d px), %d planes "
+            "(z=-%d..%d), PSF %d..%d px, %d..%d drifting blobs per frame, Poisson peak %g, %d Prevedel RL "
+            "iterations from Htf per frame; %d canvas bucket(s) %s"
+            % (T, N, min(s[0] for s in shapes) // N, max(s[0] for s in shapes) // N, min(s[0] for s in shapes),
+               max(s[0] for s in shapes), nz, w["zs"][-1], w["zs"][-1], min(w["sides"]), max(w["sides"]),
+               min(w["nblobs"]), max(w["nblobs"]), w["peak"], iters, len(rb.buckets),
+               [(c, len(i)) for c, i in rb.buckets]))
+    res = {
+        "metric": "volumes/s, ragged time series (%d uneven frames, %d RL iterations), 1 GPU" % (T, iters),
+        "value": T * 1e3 / ms_step,
+        "unit": "volumes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": desc, "frames": T, "graph": bool(args.graph), "launch_options": tuned,
+                   "datafit": {"min": float(fits.min()), "median": float(np.median(fits)),
+                               "max": float(fits.max())}},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM, "traffic": None, "kernel": "k_mac",
+                     "launch_avg_ms": mac_avg_ms, "launches": mac_n, "bytes_per_launch": mac_bytes / max(mac_n, 1)},
+        "valu_fp32": {"kernel": "k_mac", "achieved_TFLOPs": flop_rate / 1e12, "peak_TFLOPs": PEAK_FP32 / 1e12,
+                      "frac": flop_rate / PEAK_FP32, "mac_share_of_step": mac_ms / prof_steps / ms_step},
+        "spectra_precompute_s": t_prep,
+        "device_bytes": dev_bytes,
+    }
+    if not args.no_cpu_baseline:
+        t = time.time()
+        res["cpu_baseline"] = ragged_cpu_baseline(w)
+        log("cpu baseline %.1fs" % (time.time() - t))
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rl1455", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="rl1455", choices=sorted(WORKLOADS) + sorted(RAGGED))
+    ap.add_argument("--buckets", type=int, default=2, help="ragged workloads: canvas sizes (at most)")
+    ap.add_argument("--graph", action="store_true", help="ragged workloads: replay iterations as HIP graphs")
     ap.add_argument("--weights", default=None, help="learned-initialiser weights (default: the packaged file)")
     ap.add_argument("--frames", type=int, default=1,
                     help="frames reconstructed together per step (value = frames x steps / time)")
@@ -110,6 +276,8 @@ def main():
               "sharded across GPUs with one all-reduce of the forward image per iteration; see DESIGN.md).",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload in RAGGED:
+        return ragged_main(args)
 
     import torch
     from flfhip.lfdeconv import deconv_rl_device, _fit_from_sums

@@ -87,6 +87,7 @@ class RaggedBatch(object):
         while len(projectors) < len(self.buckets):
             projectors.append(Projector_hip(device))
         self.projectors = projectors[:len(self.buckets)]
+        self._graphs = {}
 
     @contextlib.contextmanager
     def _bucket(self, k):
@@ -152,25 +153,53 @@ class RaggedBatch(object):
                 out[i] = vol[:, j, :y, :x].copy()
         return out
 
-    def deconv(self, images, maxIter, Xguess=None, prevedel=True, targetFit=None, fitOut=None, graph=False):
-        """Richardson-Lucy for every frame (lfdeconv.py:72-202 semantics per
-        frame: Htf = H^T im, start from Htf or Xguess[t]).  Returns
-        [(numZ, Y_t, X_t)] volumes.  fitOut (a list) receives, per bucket,
-        (frame indices, [per-iteration fit arrays]).  graph=True replays each
-        Prevedel iteration from a captured HIP graph (no fit reporting)."""
+    def stage(self, images):
+        """Per bucket, the zero-padded (T_b, Yc, Xc) image batch on its
+        projector's device."""
         self._check_shapes(images, ())
-        if Xguess is not None:
-            self._check_shapes(Xguess, (self.hMatrix.numZ,))
+        out = []
+        for k, ((yc, xc), idx) in enumerate(self.buckets):
+            out.append(self._pad_images(images, idx, yc, xc, self.projectors[k].device))
+        return out
+
+    def unstage(self, batches):
+        """Per-bucket (numZ, T_b, Yc, Xc) volumes -> [(numZ, Y_t, X_t)] numpy."""
+        out = [None] * len(self.shapes)
+        for k, ((yc, xc), idx) in enumerate(self.buckets):
+            Xn = batches[k].cpu().numpy()
+            for j, i in enumerate(idx):
+                y, x = self.shapes[i]
+                out[i] = Xn[:, j, :y, :x].copy()
+        return out
+
+    def deconv_staged(self, staged, maxIter, Xguess=None, prevedel=True, targetFit=None, fitOut=None,
+                      graph=False, out=None):
+        """RL on staged image batches (stage()); returns per-bucket volumes
+        (numZ, T_b, Yc, Xc), zero outside each frame.  Xguess: per-bucket
+        volumes or None (start from Htf).  out: per-bucket (Htf, X) buffers
+        to reuse.  See deconv for the other arguments."""
         if graph and (not prevedel or targetFit is not None or fitOut is not None):
             raise ValueError("graph replay runs plain Prevedel iterations only")
-        out = [None] * len(self.shapes)
+        res = []
         for k in range(len(self.buckets)):
             with self._bucket(k) as (p, idx, yc, xc):
-                im = self._pad_images(images, idx, yc, xc, p.device)
-                Htf = p.backward_device(self.hMatrix, im)
-                X = Htf.clone() if Xguess is None else self._pad_volumes(Xguess, idx, yc, xc, p.device)
+                im = staged[k]
+                Htf = p.backward_device(self.hMatrix, im, out=None if out is None else out[k][0])
+                if out is None:
+                    X = Htf.clone() if Xguess is None else Xguess[k].clone()
+                else:
+                    X = out[k][1]
+                    X.copy_(Htf if Xguess is None else Xguess[k])
                 if graph:
-                    RLGraph(p, self.hMatrix, Htf, X).run(maxIter)
+                    # one captured iteration per bucket and buffer pair; the
+                    # bucket's frame sizes live in its projector (set above)
+                    key = (k, Htf.data_ptr(), X.data_ptr())
+                    g = self._graphs.get(key) if out is not None else None
+                    if g is None:
+                        g = RLGraph(p, self.hMatrix, Htf, X)
+                        if out is not None:
+                            self._graphs[key] = g
+                    g.run(maxIter)
                 else:
                     tgt = None if targetFit is None else np.broadcast_to(
                         np.asarray(targetFit, np.float64), (len(self.shapes),))[idx]
@@ -179,8 +208,20 @@ class RaggedBatch(object):
                                      fitOut=fits)
                     if fitOut is not None:
                         fitOut.append((list(idx), fits))
-                Xn = X.cpu().numpy()
-            for j, i in enumerate(idx):
-                y, x = self.shapes[i]
-                out[i] = Xn[:, j, :y, :x].copy()
-        return out
+                res.append(X)
+        return res
+
+    def deconv(self, images, maxIter, Xguess=None, prevedel=True, targetFit=None, fitOut=None, graph=False):
+        """Richardson-Lucy for every frame (lfdeconv.py:72-202 semantics per
+        frame: Htf = H^T im, start from Htf or Xguess[t]).  Returns
+        [(numZ, Y_t, X_t)] volumes.  fitOut (a list) receives, per bucket,
+        (frame indices, [per-iteration fit arrays]).  graph=True replays each
+        Prevedel iteration from a captured HIP graph (no fit reporting)."""
+        staged = self.stage(images)
+        Xg = None
+        if Xguess is not None:
+            self._check_shapes(Xguess, (self.hMatrix.numZ,))
+            Xg = [self._pad_volumes(Xguess, idx, yc, xc, self.projectors[k].device)
+                  for k, ((yc, xc), idx) in enumerate(self.buckets)]
+        return self.unstage(self.deconv_staged(staged, maxIter, Xguess=Xg, prevedel=prevedel, targetFit=targetFit,
+                                               fitOut=fitOut, graph=graph))

@@ -88,3 +88,45 @@ def poisson_image(clean, peak, seed):
     clean = np.asarray(clean, dtype=np.float64)
     scale = peak / max(float(clean.max()), 1e-30)
     return rng.poisson(np.clip(clean, 0, None) * scale).astype(np.float32)
+
+
+def drifting_blob_frames(nz, shapes, nblobs, seed, drift=0.5, field=None, background=0.01, sigma=(1.0, 3.0),
+                         amp=(0.5, 2.0)):
+    """A time series of blob volumes seen through regions of uneven size.
+
+    One seeded set of max(nblobs) blobs fills a field (default: the largest
+    frame); blob i moves by `drift` px per frame in a seeded direction.
+    Frame t shows the first nblobs[t] blobs inside its region, the top-left
+    shapes[t] = (Y_t, X_t) of the field.  Returns [(nz, Y_t, X_t) float32].
+    """
+    T = len(shapes)
+    nblobs = np.broadcast_to(np.asarray(nblobs, int), (T,))
+    FY = field[0] if field else max(s[0] for s in shapes)
+    FX = field[1] if field else max(s[1] for s in shapes)
+    nmax = int(nblobs.max())
+    rng = np.random.default_rng(seed)
+    zc = rng.uniform(0, nz, nmax)
+    yc = rng.uniform(0, FY, nmax)
+    xc = rng.uniform(0, FX, nmax)
+    sg = rng.uniform(sigma[0], sigma[1], nmax)
+    am = rng.uniform(amp[0], amp[1], nmax)
+    ang = rng.uniform(0, 2 * np.pi, nmax)
+    out = []
+    for t, (Y, X) in enumerate(shapes):
+        vol = np.full((nz, Y, X), background, dtype=np.float32)
+        for i in range(int(nblobs[t])):
+            cy = yc[i] + drift * t * np.sin(ang[i])
+            cx = xc[i] + drift * t * np.cos(ang[i])
+            r = int(np.ceil(3 * sg[i]))
+            y0, y1 = max(0, int(cy) - r), min(Y, int(cy) + r + 1)
+            x0, x1 = max(0, int(cx) - r), min(X, int(cx) + r + 1)
+            z0, z1 = max(0, int(zc[i]) - r), min(nz, int(zc[i]) + r + 1)
+            if y0 >= y1 or x0 >= x1 or z0 >= z1:
+                continue
+            zz = np.arange(z0, z1)[:, None, None]
+            yy = np.arange(y0, y1)[None, :, None]
+            xx = np.arange(x0, x1)[None, None, :]
+            g = am[i] * np.exp(-((zz - zc[i]) ** 2 + (yy - cy) ** 2 + (xx - cx) ** 2) / (2 * sg[i] ** 2))
+            vol[z0:z1, y0:y1, x0:x1] += g.astype(np.float32)
+        out.append(vol)
+    return out

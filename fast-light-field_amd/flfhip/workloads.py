@@ -4,6 +4,13 @@ training: seeded ring PSF sets (synthetic.make_psf_set) and blob volumes.
     name     Nnum  lenslets  planes      PSF sides   blobs  Poisson peak  RL iterations
     rl1455   15    97x97     z=-20..20   15..345     400    2000          8
     rl300    15    20x20     z=-5..5     15..135     200    1000          4
+
+Time series of uneven regions (RAGGED): T frames whose lenslet grids are
+drawn uniformly from L_min..L_max (square), one shared PSF set, blobs
+drifting 0.5 px per frame (synthetic.drifting_blob_frames).
+
+    name       frames  Nnum  lenslets  planes      PSF sides  blobs/frame  peak  RL iterations
+    ragged64   64      15    20..60    z=-15..15   15..255    50..300      1000  8
 """
 import sys
 import time
@@ -14,6 +21,12 @@ WORKLOADS = {
     # name: (Nnum, lenslets, zmax, s_min, s_max, blobs, peak, iters, seed)
     "rl1455": (15, 97, 20, 15, 345, 400, 2000.0, 8, 2),
     "rl300": (15, 20, 5, 15, 135, 200, 1000.0, 4, 1),
+}
+
+
+RAGGED = {
+    # name: (frames, Nnum, L_min, L_max, zmax, s_min, s_max, blobs_min, blobs_max, peak, iters, seed)
+    "ragged64": (64, 15, 20, 60, 15, 15, 255, 50, 300, 1000.0, 8, 3),
 }
 
 
@@ -45,3 +58,28 @@ def setup(name, volume=True, frames=1):
         _log("volume: %s, %.1fs" % (vol.shape, time.time() - t))
     return dict(name=name, N=N, Y=Y, X=X, zs=zs, sides=sides, hm=hm, vol=vol, peak=peak, iters=iters, seed=seed,
                 nblobs=nblobs)
+
+
+def setup_ragged(name, volume=True):
+    """Ragged time-series workload: PSF set, per-frame shapes and (with
+    volume) the frames' blob volumes.  Frame t's lenslet grid and blob count
+    come from a generator seeded with seed + t."""
+    T, N, lmin, lmax, zmax, smin, smax, bmin, bmax, peak, iters, seed = RAGGED[name]
+    zs = list(range(-zmax, zmax + 1))
+    sides = synthetic.psf_sides(zs, smin, smax)
+    t0 = time.time()
+    hm = synthetic.make_psf_set(N, zs, sides, seed=seed)
+    _log("psf set: %d planes, sides %d..%d, %.1fs" % (len(zs), min(sides), max(sides), time.time() - t0))
+    shapes, nblobs = [], []
+    for t in range(T):
+        rng = synthetic.np.random.default_rng(seed + t)
+        L = int(rng.integers(lmin, lmax + 1))
+        shapes.append((N * L, N * L))
+        nblobs.append(int(rng.integers(bmin, bmax + 1)))
+    vols = None
+    if volume:
+        t0 = time.time()
+        vols = synthetic.drifting_blob_frames(len(zs), shapes, nblobs, seed=seed, field=(N * lmax, N * lmax))
+        _log("volumes: %d frames, %.1fs" % (T, time.time() - t0))
+    return dict(name=name, N=N, zs=zs, sides=sides, hm=hm, shapes=shapes, nblobs=nblobs, vols=vols, peak=peak,
+                iters=iters, seed=seed)

@@ -146,3 +146,40 @@ def test_autotune_and_options_keep_parity(projector):
         projector.set_option(k, v)
     with pytest.raises(Exception):
         projector.set_option("no_such_option", 1)
+
+
+def test_multiframe_full_size_equals_single_frames(projector):
+    """At the full 1455 x 1455, 41-plane geometry (three Fourier grids, PSFs
+    up to 345 px) a two-frame forward and backward projection equal two
+    one-frame projections for every frames-per-workgroup setting and plane
+    split: frames must not leak into each other's tables or accumulators."""
+    import torch
+    from flfhip import workloads
+    hm, zs, sides = workloads.psf_set("rl1455")
+    Y = X = 1455
+    nz = len(zs)
+    projector.prepare(hm, 1, Y, X, frames_capacity=2)
+    g = torch.Generator(device=projector.device)
+    g.manual_seed(3)
+    vol = torch.rand((nz, 2, Y, X), generator=g, device=projector.device)
+    img = torch.rand((2, Y, X), generator=g, device=projector.device)
+    try:
+        projector.set_option("mac_frames", 1)
+        projector.set_option("fwd_split", 1)
+        ref_f = [projector.forward_device(hm, vol[:, t:t + 1].contiguous())[0].clone() for t in range(2)]
+        ref_b = [projector.backward_device(hm, img[t:t + 1].contiguous())[:, 0].clone() for t in range(2)]
+        for fb in (2, 4):
+            for sp in (1, 3):
+                projector.set_option("mac_frames", fb)
+                projector.set_option("fwd_split", sp)
+                two_f = projector.forward_device(hm, vol)
+                two_b = projector.backward_device(hm, img)
+                torch.cuda.synchronize()
+                for t in range(2):
+                    ef = float((two_f[t] - ref_f[t]).norm() / ref_f[t].norm())
+                    eb = float((two_b[:, t] - ref_b[t]).norm() / ref_b[t].norm())
+                    print("fb %d split %d frame %d: forward %.2e backward %.2e" % (fb, sp, t, ef, eb), flush=True)
+                    assert ef < 1e-6 and eb < 1e-6
+    finally:
+        projector.set_option("mac_frames", 2)
+        projector.set_option("fwd_split", 0)
