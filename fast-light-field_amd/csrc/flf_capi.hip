@@ -73,7 +73,9 @@ struct flf_ctx {
     double* d_fitpart = nullptr;
     double* d_fitout = nullptr;
     size_t max_lds = 65536;
-    int ipt = 1;                      // items per MAC thread (FLF_ITEMS_PER_THREAD overrides)
+    int ipt = 2;                      // MAC items per lane pair's thread: 2 = lane pairs share two items
+                                      // and split the spectra (FLF_ITEMS_PER_THREAD=1 selects one item
+                                      // per thread); plan time
     int fb_max = 2;                   // frames per MAC workgroup at T >= 2 (FLF_MAC_FRAMES overrides)
     int fwd_split = 0;                // forward plane ranges, 0 = automatic (FLF_FWD_SPLIT overrides)
     bool mfma = false;                // matrix-core MAC for >= 2 frames per workgroup (FLF_MAC_MFMA=1 enables;
@@ -844,8 +846,20 @@ int flf_profile_read(flf_ctx* ctx, double* mac_ms, long long* mac_launches, doub
 int flf_grid_check(int nnum, int fY, int fX, long long* stats) {
     if (!stats) return FLF_EINVAL;
     GridTables gt;
-    std::string e = build_grid_tables(nnum, fY, fX, gt);
+    // the tables the projector plans by default: lane pairs of two items
+    std::string e = build_grid_tables(nnum, fY, fX, gt, 2);
     if (!e.empty()) return FLF_EINVAL;
+    // every workgroup holds whole pairs, and the two items of a pair share
+    // their residues (one P/M table row serves both)
+    for (auto& c : gt.chunks) {
+        if (c.n_items % 2) return FLF_EINVAL;
+        for (int i = 0; i < c.n_items; i += 2) {
+            const ItemDesc& x = gt.items[c.item_base + i];
+            const ItemDesc& y = gt.items[c.item_base + i + 1];
+            for (int j = 0; j < 4; ++j)
+                if (x.res[j] != y.res[j]) return FLF_EINVAL;
+        }
+    }
     int max_items = 0;
     for (auto& c : gt.chunks) max_items = std::max(max_items, c.n_items);
     stats[0] = (long long)gt.chunks.size();

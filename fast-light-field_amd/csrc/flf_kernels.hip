@@ -410,9 +410,11 @@ __device__ __forceinline__ void mac_tables(const MacArgs& a, const ChunkDesc* ch
 // beta = b (b <= N/2) or b + s2 - 1 (row-flipped offsets), likewise alpha.
 // This is the accumulation of py_light_field.cpp:303-341, 1029-1087 with the
 // mirror multipliers of projector.py:137-151 and the 'same' crop
-// (projector.py:792-796) folded into phases.  Each thread handles IPT
-// consecutive items that share their residues, for FB frames: every stored
-// spectrum value read from HBM serves FB frames (grid z = frame blocks).
+// (projector.py:792-796) folded into phases.  IPT = 1: one item per
+// thread; IPT = 2: a lane pair owns two consecutive items that share their
+// residues and splits the spectra between its lanes.  FB frames per
+// workgroup: every stored spectrum value read from HBM serves FB frames
+// (grid z = frame blocks).
 template <int SLOTS, int IPT, int FB>
 __global__ __launch_bounds__(FLF_CHUNK_MAX) __attribute__((amdgpu_waves_per_eu(IPT * FB >= 4 ? 2 : (IPT * FB == 2 ? 3 : FLF_MAC_WAVES))))
 void k_mac(MacArgs a) {
@@ -424,8 +426,16 @@ void k_mac(MacArgs a) {
     const long long item_base = ch->item_base;
     const int n_items = ch->n_items;
     const int tid = threadIdx.x;
-    const bool valid = tid * IPT < n_items;
-    const int tsafe = valid ? tid : 0;           // idle lanes read item 0 (in bounds) and never write
+    // IPT = 2: a lane pair owns two items that share their residues and
+    // splits the stored spectra between its lanes (lane parity lsub takes
+    // u = lsub, lsub + 2, ...), so every P/M table read serves two items
+    // while the workgroup keeps one thread per item slot; the pair adds its
+    // partial sums before each plane's epilogue.
+    constexpr int USTEP = IPT == 2 ? 2 : 1;
+    const int lsub = IPT == 2 ? (tid & 1) : 0;
+    const int slot = IPT == 2 ? (tid >> 1) : tid;
+    const bool valid = slot * IPT < n_items;
+    const int tsafe = valid ? slot : 0;          // idle lanes read item 0 (in bounds) and never write
     const int N = a.N, NN = N * N, fourN = 4 * N;
     const int t0 = blockIdx.z * FB;
     const int nf = min(FB, a.T - t0);            // frames of this block (block-uniform)
@@ -454,11 +464,12 @@ void k_mac(MacArgs a) {
         return ip;
     };
 
-    float2 tot[FB][IPT][4];
+    constexpr int QT = IPT == 2 ? 1 : IPT;        // items whose totals a lane keeps
+    float2 tot[FB][QT][4];
 #pragma unroll
     for (int f = 0; f < FB; ++f)
 #pragma unroll
-        for (int q = 0; q < IPT; ++q)
+        for (int q = 0; q < QT; ++q)
 #pragma unroll
             for (int j = 0; j < 4; ++j) tot[f][q][j] = make_float2(0.f, 0.f);
 
@@ -467,7 +478,8 @@ void k_mac(MacArgs a) {
     const int pb = (int)(((long long)blockIdx.y * a.nP) / gridDim.y);
     const int pe = (int)(((long long)(blockIdx.y + 1) * a.nP) / gridDim.y);
     const long long ustride = (long long)n_items * SLOTS;
-    const int nb = (nu + UNR - 1) / UNR;          // spectrum batches per plane
+    const int nul = (nu + USTEP - 1) / USTEP;     // spectra per lane and plane
+    const int nb = (nul + UNR - 1) / UNR;         // spectrum batches per plane
 
     // Per-plane set-up: flip factors, then per frame stage the chunk's
     // small-FFT block and build its P/M table (all threads; contains
@@ -483,7 +495,7 @@ void k_mac(MacArgs a) {
         const float2* Gp = gbase(p);
 #pragma unroll
         for (int k = 0; k < UNR; ++k) {
-            const int u = min(b * UNR + k, nu - 1);
+            const int u = min((b * UNR + k) * USTEP + lsub, nu - 1);
 #pragma unroll
             for (int q = 0; q < IPT; ++q) load_g<SLOTS>(g[k][q], Gp + u * ustride, tsafe * IPT + q, n_items);
         }
@@ -495,10 +507,23 @@ void k_mac(MacArgs a) {
         const int s2 = a.s2[p];
         const int sig = s2 - 2 + N;
         const int c2 = (s2 - 1) / 2;
+        // IPT = 2: each lane keeps its own item (q = lsub): it sends the
+        // partner's item and adds what the partner sends for its own.
+        float2 own[FB][4];
+        if constexpr (IPT == 2) {
+#pragma unroll
+            for (int f = 0; f < FB; ++f)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float2 mine = lsub ? acc[f][1][j] : acc[f][0][j];
+                    const float2 give = lsub ? acc[f][0][j] : acc[f][1][j];
+                    own[f][j] = make_float2(mine.x + __shfl_xor(give.x, 1), mine.y + __shfl_xor(give.y, 1));
+                }
+        }
         const ItemDesc* ip = items_now();
 #pragma unroll
-        for (int q = 0; q < IPT; ++q) {
-            const ItemDesc itq = ip[q];
+        for (int q = 0; q < QT; ++q) {
+            const ItemDesc itq = ip[IPT == 2 ? lsub : q];
             const int m2 = itq.m2[0], n2 = itq.n2[0];
             int outer[4];
             outer[0] = sig * (m2 + n2);
@@ -518,7 +543,10 @@ void k_mac(MacArgs a) {
                 sincospif(2.0f * fr, &sn, &cs);
                 const float2 pc = make_float2(cs, sn);
 #pragma unroll
-                for (int f = 0; f < FB; ++f) tot[f][q][j] = cadd(tot[f][q][j], cmul(cmul(acc[f][q][j], po), pc));
+                for (int f = 0; f < FB; ++f) {
+                    const float2 av = IPT == 2 ? own[f][j] : acc[f][q][j];
+                    tot[f][q][j] = cadd(tot[f][q][j], cmul(cmul(av, po), pc));
+                }
             }
         }
     };
@@ -550,8 +578,10 @@ void k_mac(MacArgs a) {
             }
             if (pn < pe) load_batch(nxt, pn, bn);
 #pragma unroll
-            for (int k = 0; k < UNR; ++k)
-                if (b * UNR + k < nu) mac_spectrum<SLOTS, IPT, FB>(acc, cur[k], pm_tab, tab, rb, b * UNR + k);
+            for (int k = 0; k < UNR; ++k) {
+                const int u = (b * UNR + k) * USTEP + lsub;
+                if (u < nu) mac_spectrum<SLOTS, IPT, FB>(acc, cur[k], pm_tab, tab, rb, u);
+            }
             if (b == nb - 1) epilogue(p);
             p = pn;
             b = bn;
@@ -568,10 +598,12 @@ void k_mac(MacArgs a) {
             if (f >= nf) break;
             float2* A = a.A + (long long)(t0 + f) * a.A_tstride + (long long)blockIdx.y * a.A_pstride;
 #pragma unroll
-            for (int q = 0; q < IPT; ++q)
+            for (int q = 0; q < QT; ++q) {
+                const ItemDesc& it = ip[IPT == 2 ? lsub : q];   // each lane of a pair stores its own item
 #pragma unroll
                 for (int j = 0; j < SLOTS; ++j)
-                    if ((ip[q].writemask >> j) & 1) A[ip[q].pos[j]] = tot[f][q][j];
+                    if ((it.writemask >> j) & 1) A[it.pos[j]] = tot[f][q][j];
+            }
         }
     }
 }
@@ -1141,7 +1173,7 @@ hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks,
         }
         return hipGetLastError();
     }
-    dim3 block(FLF_CHUNK_MAX / ipt);
+    dim3 block(FLF_CHUNK_MAX);                   // ipt 2: lane pairs share two items
     if (slots == 4) {
         if (ipt == 1) launch_mac_fb<4, 1>(a, fb, grid, block, lds, s);
         else launch_mac_fb<4, 2>(a, fb, grid, block, lds, s);

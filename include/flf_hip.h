@@ -153,8 +153,10 @@ int flf_profile_read(flf_ctx* ctx, double* mac_ms, long long* mac_launches, doub
  * flf_fourier_len: Fourier grid length of one axis for image side s1 and PSF
  * side s2 (convolutionShape, special_fftconvolve.py:142-174, with 5-smooth
  * padding of the reduced length).
- * flf_grid_check: builds the kernel tables of an fY x fX grid and verifies
- * that every half-plane bin is written exactly once; stats[6] = {workgroups,
+ * flf_grid_check: builds the kernel tables of an fY x fX grid (as planned:
+ * items in pairs of one residue class) and verifies that every half-plane
+ * bin is written exactly once and that every pair shares its residues;
+ * stats[6] = {workgroups,
  * items, max residues per orbit, stored spectra per plane, slots per item,
  * max items per workgroup}.  Returns FLF_EINVAL if the tables are invalid. */
 int flf_fourier_len(int s1, int s2, int nnum);

@@ -183,3 +183,37 @@ def test_multiframe_full_size_equals_single_frames(projector):
     finally:
         projector.set_option("mac_frames", 2)
         projector.set_option("fwd_split", 0)
+
+
+def test_one_item_per_thread_matches_lane_pairs(projector):
+    """The MAC with one item per thread (FLF_ITEMS_PER_THREAD=1, read when a
+    context is created) and the default lane pairs give the same projections
+    for one and three frames."""
+    import os
+    import torch
+    from flfhip import synthetic
+    from flfhip.projector import Projector_hip
+    N, zs = 15, [-3, 0, 2, 5]
+    hm = synthetic.make_psf_set(N, zs, synthetic.psf_sides(zs, 15, 61), seed=8)
+    old = os.environ.get("FLF_ITEMS_PER_THREAD")
+    os.environ["FLF_ITEMS_PER_THREAD"] = "1"
+    try:
+        p1 = Projector_hip()
+    finally:
+        if old is None:
+            del os.environ["FLF_ITEMS_PER_THREAD"]
+        else:
+            os.environ["FLF_ITEMS_PER_THREAD"] = old
+    g = torch.Generator(device=projector.device)
+    g.manual_seed(4)
+    for T in (1, 3):
+        vol = torch.rand((len(zs), T, 150, 165), generator=g, device=projector.device)
+        img = torch.rand((T, 150, 165), generator=g, device=projector.device)
+        a_f, a_b = projector.forward_device(hm, vol), projector.backward_device(hm, img)
+        b_f, b_b = p1.forward_device(hm, vol), p1.backward_device(hm, img)
+        torch.cuda.synchronize()
+        ef = float((a_f - b_f).norm() / b_f.norm())
+        eb = float((a_b - b_b).norm() / b_b.norm())
+        print("T=%d lane pairs vs one item per thread: forward %.2e backward %.2e" % (T, ef, eb), flush=True)
+        assert ef < 1e-6 and eb < 1e-6
+    del p1
