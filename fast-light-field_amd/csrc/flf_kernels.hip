@@ -47,9 +47,28 @@ __device__ __forceinline__ int centred_res(int k, int r) {
 #define FLF_SFFT_WAVES 8
 #endif
 constexpr int SFFT_WAVES = FLF_SFFT_WAVES;
+
+// Workgroup id -> work index such that the workgroups of one XCD take a
+// contiguous range: the dispatcher deals consecutive workgroup ids of a grid
+// row to the 8 XCDs in turn, so ids with one value of (id mod 8) share an
+// XCD and its L2.  Neighbouring work items (adjacent residues, adjacent
+// lenslet offsets) write neighbouring bytes, which then combine in one L2.
+// A bijection on [0, n): the last n mod 8 ids map to themselves.
+#ifndef FLF_XCD_REMAP
+#define FLF_XCD_REMAP 1
+#endif
+__device__ __forceinline__ int xcd_block(int bid, int n) {
+#if FLF_XCD_REMAP
+    const int q = n >> 3;
+    return bid < 8 * q ? (bid & 7) * q + (bid >> 3) : bid;
+#else
+    (void)n;
+    return bid;
+#endif
+}
 __global__ __launch_bounds__(64 * SFFT_WAVES) void k_small_fft(SmallFFTArgs a) {
     extern __shared__ float2 lds[];
-    const int off = blockIdx.x;
+    const int off = xcd_block(blockIdx.x, a.N * a.N);
     const int t = blockIdx.y;
     const int z = blockIdx.z;
     const int N = a.N;
@@ -215,7 +234,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_psf_cols(PsfColArgs a) {
 // grid (nchunks, nu), block FLF_CHUNK_MAX.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(FLF_CHUNK_MAX) void k_spec_pack(SpecPackArgs a) {
-    const ChunkDesc* ch = a.chunks + blockIdx.x;
+    const int cidx = xcd_block(blockIdx.x, gridDim.x);
+    const ChunkDesc* ch = a.chunks + cidx;
     const int u = blockIdx.y;
     const int i = threadIdx.x;
     const int n_items = ch->n_items;
@@ -340,7 +360,7 @@ __device__ __forceinline__ void mac_spectrum(float2 (&acc)[FB][IPT][4], const fl
 // barriers; frames past nf keep the tables of earlier planes and their
 // accumulators are never stored).
 template <int SLOTS>
-__device__ __forceinline__ void mac_tables(const MacArgs& a, const ChunkDesc* ch, int p, int t0, int nf,
+__device__ __forceinline__ void mac_tables(const MacArgs& a, const ChunkDesc* ch, int cidx, int p, int t0, int nf,
                                            float4* pm_tab, float2* s_stage, float2* psi, int rs) {
     constexpr int NUSE = SLOTS == 4 ? 8 : 4;
     constexpr int NPT = SLOTS == 4 ? 4 : 2;
@@ -368,7 +388,7 @@ __device__ __forceinline__ void mac_tables(const MacArgs& a, const ChunkDesc* ch
     }
     for (int f = 0; f < nf; ++f) {
         if (f > 0) __syncthreads();
-        const float2* Sp = a.S + a.S_off[p] + (long long)(t0 + f) * a.S_tstride + (long long)blockIdx.x * NN * NH;
+        const float2* Sp = a.S + a.S_off[p] + (long long)(t0 + f) * a.S_tstride + (long long)cidx * NN * NH;
         for (int i = tid; i < NN * NH; i += blockDim.x) s_stage[i] = Sp[i];
         __syncthreads();
         float4* pmf = pm_tab + f * tab;
@@ -422,7 +442,8 @@ void k_mac(MacArgs a) {
     constexpr int NRES = SLOTS == 4 ? 8 : 4;      // residues per orbit (canonical list)
     constexpr int NH = NRES / 2;                  // residues stored by the small-FFT kernel
     constexpr int UNR = IPT >= 2 ? 1 : FLF_MAC_UNR;   // spectra whose loads are issued together
-    const ChunkDesc* ch = a.chunks + blockIdx.x;
+    const int cidx = xcd_block(blockIdx.x, gridDim.x);
+    const ChunkDesc* ch = a.chunks + cidx;
     const long long item_base = ch->item_base;
     const int n_items = ch->n_items;
     const int tid = threadIdx.x;
@@ -486,7 +507,7 @@ void k_mac(MacArgs a) {
     // barriers; frames past nf keep the tables of earlier planes and their
     // accumulators are never stored).
     auto prologue = [&](int p) {
-        mac_tables<SLOTS>(a, ch, p, t0, nf, pm_tab, s_stage, psi, rs);
+        mac_tables<SLOTS>(a, ch, cidx, p, t0, nf, pm_tab, s_stage, psi, rs);
     };
 
     auto gbase = [&](int p) { return a.G + item_base * SLOTS * nu * a.nP + ((long long)p * nu) * ustride; };
@@ -635,7 +656,8 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void k_mac_mfma(MacArgs a) {
     constexpr int NT = (FB * CPF + 15) / 16;      // 16-column tiles
     constexpr int KS = SLOTS / 2;                 // 4-deep k steps per spectrum (slot pairs)
     constexpr int MAXT = (FLF_MAX_TILES + MFMA_WAVES - 1) / MFMA_WAVES;   // tiles per wave
-    const ChunkDesc* ch = a.chunks + blockIdx.x;
+    const int cidx = xcd_block(blockIdx.x, gridDim.x);
+    const ChunkDesc* ch = a.chunks + cidx;
     const long long item_base = ch->item_base;
     const int n_items = ch->n_items;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -705,7 +727,7 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void k_mac_mfma(MacArgs a) {
     const int pe = (int)(((long long)(blockIdx.y + 1) * a.nP) / gridDim.y);
     for (int p = pb; p < pe; ++p) {
         if (!a.enable[p]) continue;
-        mac_tables<SLOTS>(a, ch, p, t0, nf, pm_tab, s_stage, psi, rs);
+        mac_tables<SLOTS>(a, ch, cidx, p, t0, nf, pm_tab, s_stage, psi, rs);
         v4f acc[MAXT][NT];
 #pragma unroll
         for (int q = 0; q < MAXT; ++q)
