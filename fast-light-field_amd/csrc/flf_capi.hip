@@ -702,11 +702,33 @@ int flf_set_psf(flf_ctx* ctx, int z, int dir, const float* quadrant_host) {
     return FLF_OK;
 }
 
+namespace {
+// Forward projection of the active frames; with vol2, frames t >= T/2 come
+// from vol2 (frame t - T/2) and both sources hold T/2 frames.
+int run_forward(flf_ctx* ctx, const float* vol, const float* vol2, float* img, const int* planes, int nplanes,
+                hipStream_t st);
+}  // namespace
+
 int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, int nplanes, void* stream) {
     if (!ctx) return FLF_EINVAL;
     if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");
     if (!vol || !img) return fail(ctx, FLF_EINVAL, "null volume or image pointer");
-    hipStream_t st = (hipStream_t)stream;
+    return run_forward(ctx, vol, nullptr, img, planes, nplanes, (hipStream_t)stream);
+}
+
+int flf_forward_pair(flf_ctx* ctx, const float* volA, const float* volB, float* img, const int* planes, int nplanes,
+                     void* stream) {
+    if (!ctx) return FLF_EINVAL;
+    if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");
+    if (!volA || !volB || !img) return fail(ctx, FLF_EINVAL, "null volume or image pointer");
+    if (ctx->T % 2) return fail(ctx, FLF_EINVAL, "flf_forward_pair needs an even active frame count");
+    if (ctx->ragged) return fail(ctx, FLF_EINVAL, "flf_forward_pair does not take per-frame sizes");
+    return run_forward(ctx, volA, volB, img, planes, nplanes, (hipStream_t)stream);
+}
+
+namespace {
+int run_forward(flf_ctx* ctx, const float* vol, const float* vol2, float* img, const int* planes, int nplanes,
+                hipStream_t st) {
     std::vector<char> sel;
     int rc = resolve_planes(ctx, planes, nplanes, sel);
     if (rc) return rc;
@@ -739,7 +761,10 @@ int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, i
         const int nP = (int)g.planes.size();
         SmallFFTArgs a{};
         a.src = vol;
-        a.src_off = g.d_srcoff_fwd;
+        // two sources: both hold T/2 frames, so the plane offsets are those of T/2 frames
+        a.src_off = vol2 ? g.srcoff_T[T / 2 - 1] : g.d_srcoff_fwd;
+        a.src2 = vol2;
+        a.tsplit = vol2 ? T / 2 : T;
         a.src_tstride = (long long)ctx->Y * ctx->X;
         a.src_pitch = ctx->X;
         a.Y = ctx->Y; a.X = ctx->X; a.N = N;
@@ -786,6 +811,7 @@ int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, i
     HIPCHK(launch_ifft_rows_sum(rs, ctx->Y, T, st));
     return FLF_OK;
 }
+}  // namespace
 
 int flf_backward(flf_ctx* ctx, const float* img, float* vol, const int* planes, int nplanes, void* stream) {
     if (!ctx) return FLF_EINVAL;
@@ -803,6 +829,12 @@ int flf_rl_scale(const float* Htf, const float* v, float* X, long long n, void* 
     if (!Htf || !v || !X || n < 0) return FLF_EINVAL;
     if (n == 0) return FLF_OK;
     return flf::launch_rl_scale(X, Htf, v, n, (hipStream_t)stream) == hipSuccess ? FLF_OK : FLF_EHIP;
+}
+
+int flf_rl_ratio(const float* Htf, const float* v, float* X, long long n, void* stream) {
+    if (!Htf || !v || !X || n < 0) return FLF_EINVAL;
+    if (n == 0) return FLF_OK;
+    return flf::launch_rl_scale(X, Htf, v, n, (hipStream_t)stream, true) == hipSuccess ? FLF_OK : FLF_EHIP;
 }
 
 int flf_datafit(flf_ctx* ctx, const float* img, const float* im, double* out_host, void* stream) {

@@ -20,6 +20,8 @@ struct SmallFFTArgs {
     const long long* S_off;      // per source plane z: element offset into S (device array)
     long long S_tstride;         // per frame
     const int2* fdims;           // per frame valid (Y, X) <= (Y, X), or null (all full size)
+    const float* src2;           // frames t >= tsplit read from src2 (frame t - tsplit), or null
+    int tsplit;
 };
 
 struct PsfRowArgs {
@@ -148,7 +150,8 @@ hipError_t launch_ifft_rows_plane(const IfftRowPlaneArgs& a, int nplanes, hipStr
 hipError_t launch_fit(const float* img, const float* im, long long n_per_frame, int T, double* part, int nblk,
                       double* out, hipStream_t s);
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t s);
-hipError_t launch_rl_scale(float* X, const float* Htf, const float* v, long long n, hipStream_t s);
+hipError_t launch_rl_scale(float* X, const float* Htf, const float* v, long long n, hipStream_t s,
+                           bool ratio = false);
 // Raise the dynamic-LDS limit of the FFT kernels to max_lds bytes.
 hipError_t configure_kernels(size_t max_lds);
 }  // namespace flf

@@ -82,7 +82,8 @@ __global__ __launch_bounds__(64 * SFFT_WAVES) void k_small_fft(SmallFFTArgs a) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < rY; i += blockDim.x) twY[i] = a.twY[i];
     for (int i = threadIdx.x; i < rX; i += blockDim.x) twX[i] = a.twX[i];
-    const float* src = a.src + a.src_off[z] + (long long)t * a.src_tstride;
+    const bool second = a.src2 && t >= a.tsplit;  // two-source calls: frames from two buffers
+    const float* src = (second ? a.src2 : a.src) + a.src_off[z] + (long long)(second ? t - a.tsplit : t) * a.src_tstride;
     // frame t's valid region (frames of one batch may be smaller than the plan)
     const int Yv = a.fdims ? a.fdims[t].x : a.Y, Xv = a.fdims ? a.fdims[t].y : a.X;
     const int Ys = max(0, (Yv - b + N - 1) / N);
@@ -1122,10 +1123,11 @@ __global__ __launch_bounds__(256) void k_fit_final(const double* part, int nblk,
 // X *= Htf / v, NaN -> 0, the same arithmetic as k_ifft_rows_plane mode 1.
 // Used for the first plain iterate from a uniform start, X1 = Htf / Ht(H 1),
 // whose denominator does not depend on the image.
-__global__ void k_rl_scale(float* X, const float* Htf, const float* v, long long n) {
+// ratio: X = Htf / v (the same update of X = 1, without reading X).
+__global__ void k_rl_scale(float* X, const float* Htf, const float* v, long long n, int ratio) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         const float r = Htf[i] / v[i];
-        const float xn = X[i] * r;
+        const float xn = ratio ? r : X[i] * r;
         X[i] = __builtin_isnan(xn) ? 0.f : xn;
     }
 }
@@ -1247,11 +1249,11 @@ hipError_t configure_kernels(size_t max_lds) {
     return hipSuccess;
 }
 
-hipError_t launch_rl_scale(float* X, const float* Htf, const float* v, long long n, hipStream_t s) {
+hipError_t launch_rl_scale(float* X, const float* Htf, const float* v, long long n, hipStream_t s, bool ratio) {
     long long blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_rl_scale, dim3((unsigned)blocks), dim3(256), 0, s, X, Htf, v, n);
+    hipLaunchKernelGGL(k_rl_scale, dim3((unsigned)blocks), dim3(256), 0, s, X, Htf, v, n, ratio ? 1 : 0);
     return hipGetLastError();
 }
 

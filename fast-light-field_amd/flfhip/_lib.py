@@ -37,6 +37,9 @@ SIGNATURES = {
     "flf_fourier_len": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int]),
     "flf_grid_check": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.POINTER(_c.c_longlong)]),
     "flf_set_frames": (_c.c_int, [_c.c_void_p, _c.c_int]),
+    "flf_forward_pair": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_void_p, _c.POINTER(_c.c_int),
+                                    _c.c_int, _c.c_void_p]),
+    "flf_rl_ratio": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_longlong, _c.c_void_p]),
     "flf_set_frame_dims": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_int, _c.c_void_p]),
     "flf_frame_capacity": (_c.c_int, [_c.c_void_p]),
     "flf_set_option": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.c_int]),

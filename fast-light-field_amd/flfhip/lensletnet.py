@@ -292,15 +292,14 @@ class LearnedInit(object):
         T, Y, Xw = im.shape
         den = self.plain_denominator(T, Y, Xw)
         g = self.net.volume_guess(im)
-        X1 = torch.ones_like(Htf)
-        p.rl_scale_device(X1, Htf, den)
+        X1 = torch.empty_like(Htf)
+        p.rl_ratio_device(X1, Htf, den)                        # X1 = Htf / Ht(H 1): one iteration from X = 1
         if getattr(p, "_tcap", 0) >= 2 * T:
-            both = torch.cat([g, X1], dim=1)                   # (nz, 2T, Y, X)
-            H2 = p.forward_device(hm, both)
+            H2 = p.forward_pair_device(hm, g, X1)              # (2T, Y, X): both volumes, one call
             fits = _fit(p, H2, torch.cat([im, im], dim=0))
             fit_g, fit_1 = fits[:T], fits[T:]
-            Hg, HX1 = H2[:T].contiguous(), H2[T:].contiguous()
-            del both, H2
+            Hg, HX1 = H2[:T], H2[T:]
+            del H2
         else:
             Hg = p.forward_device(hm, g)
             fit_g = _fit(p, Hg, im)

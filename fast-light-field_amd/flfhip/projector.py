@@ -173,6 +173,23 @@ class Projector_hip(object):
                     "flf_forward")
         return img
 
+    def forward_pair_device(self, hMatrix, volA, volB, planes=None):
+        """Forward projection of two volumes (numZ, S, Y, X) as one call of 2S
+        frames (flf_forward_pair): image (2S, Y, X), frames of volA first."""
+        if volA.shape != volB.shape:
+            raise ValueError("volA and volB must have one shape")
+        nz, S, Y, X = volA.shape
+        if nz != hMatrix.numZ:
+            raise ValueError("volume has %d planes, HMatrix has %d" % (nz, hMatrix.numZ))
+        self._ensure_plan(hMatrix, 2 * S, Y, X)
+        planes, arr = self._plane_array(planes, hMatrix.numZ)
+        self._ensure_psf(hMatrix, planes, 0)
+        volA, volB = volA.contiguous(), volB.contiguous()
+        img = torch.empty((2 * S, Y, X), dtype=torch.float32, device=self.device)
+        self._check(self.lib.flf_forward_pair(self._ctx, _ptr(volA), _ptr(volB), _ptr(img), arr, len(planes),
+                                              _stream_handle()), "flf_forward_pair")
+        return img
+
     def backward_device(self, hMatrix, img, planes=None, out=None):
         """img: (T, Y, X) cuda tensor -> volume (numZ, T, Y, X); planes not
         listed are zero (Projector_allC.BackwardProjectACC, projector.py:807)."""
@@ -206,6 +223,14 @@ class Projector_hip(object):
         if not (X.is_contiguous() and Htf.is_contiguous() and v.is_contiguous()):
             raise ValueError("X, Htf and v must be contiguous")
         self._check(self.lib.flf_rl_scale(_ptr(Htf), _ptr(v), _ptr(X), X.numel(), _stream_handle()), "flf_rl_scale")
+
+    def rl_ratio_device(self, X, Htf, v):
+        """X = Htf / v, NaN -> 0 (the first Prevedel iterate from X = 1)."""
+        if X.shape != Htf.shape or X.shape != v.shape:
+            raise ValueError("X, Htf and v must have one shape")
+        if not (X.is_contiguous() and Htf.is_contiguous() and v.is_contiguous()):
+            raise ValueError("X, Htf and v must be contiguous")
+        self._check(self.lib.flf_rl_ratio(_ptr(Htf), _ptr(v), _ptr(X), X.numel(), _stream_handle()), "flf_rl_ratio")
 
     def datafit_device(self, img, im):
         """Per-frame FP64 (sum((img-im)^2), sum(im^2))."""

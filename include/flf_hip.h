@@ -65,6 +65,15 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
 int flf_set_frames(flf_ctx* ctx, int T);
 int flf_frame_capacity(const flf_ctx* ctx);
 
+/* Forward projection of 2S active frames held in two buffers: frames
+ * 0..S-1 from volA, frames S..2S-1 from volB, each (nz, S, Y, X); img is
+ * (2S, Y, X).  Same result as flf_forward on the two volumes concatenated
+ * along frames (ForwardProjectACC, projector.py:849-896), without the copy:
+ * the learned initialiser projects its guess and the first plain iterate
+ * together this way.  Not with per-frame sizes. */
+int flf_forward_pair(flf_ctx* ctx, const float* volA, const float* volB, float* img, const int* planes,
+                     int nplanes, void* stream);
+
 /* Ragged batches: dims[2t], dims[2t+1] = valid rows and columns of frame t
  * (1..Y, 1..X) for the T active frames (T must equal the active count);
  * NULL returns to full-size frames.  Frame t occupies the top-left corner
@@ -139,6 +148,11 @@ int flf_datafit(flf_ctx* ctx, const float* img, const float* im, double* out_hos
  * gives the first plain RL iterate from a uniform start without projecting;
  * the learned initialiser's guard compares its guess against that iterate. */
 int flf_rl_scale(const float* Htf, const float* v, float* X, long long n, void* stream);
+
+/* X[i] = Htf[i] / v[i], NaN -> 0: flf_rl_scale applied to X = 1 without
+ * reading X (the first plain RL iterate from a uniform start,
+ * lfdeconv.py:112-126 with Xguess = 1). */
+int flf_rl_ratio(const float* Htf, const float* v, float* X, long long n, void* stream);
 
 /* Kernel timing for roofline reporting (no reference counterpart; the
  * reference times whole calls with time.time(), lf_performance_analysis.py:

@@ -217,3 +217,35 @@ def test_one_item_per_thread_matches_lane_pairs(projector):
         print("T=%d lane pairs vs one item per thread: forward %.2e backward %.2e" % (T, ef, eb), flush=True)
         assert ef < 1e-6 and eb < 1e-6
     del p1
+
+
+def test_forward_pair_equals_concatenated_frames(projector):
+    """flf_forward_pair (frames from two buffers) gives the same image as
+    flf_forward on the two volumes concatenated along frames, bit for bit;
+    flf_rl_ratio equals flf_rl_scale on X = 1."""
+    import torch
+    from flfhip import synthetic
+    N, zs = 15, [-2, 0, 3]
+    hm = synthetic.make_psf_set(N, zs, synthetic.psf_sides(zs, 15, 45), seed=12)
+    g = torch.Generator(device=projector.device)
+    g.manual_seed(6)
+    for S in (1, 2):
+        A = torch.rand((3, S, 120, 135), generator=g, device=projector.device)
+        B = torch.rand((3, S, 120, 135), generator=g, device=projector.device)
+        projector.prepare(hm, 2 * S, 120, 135)
+        ref = projector.forward_device(hm, torch.cat([A, B], dim=1))
+        got = projector.forward_pair_device(hm, A, B)
+        torch.cuda.synchronize()
+        print("S=%d pair vs concatenated: max abs diff %.3g" % (S, float((got - ref).abs().max())), flush=True)
+        assert torch.equal(got, ref)
+    Htf = torch.rand((3, 1, 64, 64), generator=g, device=projector.device)
+    v = torch.rand((3, 1, 64, 64), generator=g, device=projector.device)
+    v[0, 0, 0, :4] = 0.0                                  # 0/0 and x/0 entries
+    Htf[0, 0, 0, :2] = 0.0
+    X1 = torch.ones_like(Htf)
+    projector.rl_scale_device(X1, Htf, v)
+    X2 = torch.empty_like(Htf)
+    projector.rl_ratio_device(X2, Htf, v)
+    torch.cuda.synchronize()
+    assert torch.equal(X1, X2)
+    assert float(X2[0, 0, 0, 0]) == 0.0                   # NaN -> 0
