@@ -349,6 +349,9 @@ __device__ __forceinline__ void mac_spectrum(float2 (&acc)[FB][IPT][4], const fl
 #ifndef FLF_MAC_UNR
 #define FLF_MAC_UNR 2
 #endif
+#ifndef FLF_MAC_UNR2
+#define FLF_MAC_UNR2 1
+#endif
 #ifndef FLF_MAC_WAVES
 #define FLF_MAC_WAVES 4
 #endif
@@ -442,7 +445,7 @@ void k_mac(MacArgs a) {
     extern __shared__ float4 pm_tab[];            // [FB][NRES][rs: nu*NPT + pad], then staging
     constexpr int NRES = SLOTS == 4 ? 8 : 4;      // residues per orbit (canonical list)
     constexpr int NH = NRES / 2;                  // residues stored by the small-FFT kernel
-    constexpr int UNR = IPT >= 2 ? 1 : FLF_MAC_UNR;   // spectra whose loads are issued together
+    constexpr int UNR = IPT >= 2 ? FLF_MAC_UNR2 : FLF_MAC_UNR;   // spectra whose loads are issued together
     const int cidx = xcd_block(blockIdx.x, gridDim.x);
     const ChunkDesc* ch = a.chunks + cidx;
     const long long item_base = ch->item_base;
