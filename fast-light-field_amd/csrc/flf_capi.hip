@@ -223,9 +223,8 @@ hipEvent_t take_event(flf_ctx* ctx) {
 
 // Frames per MAC workgroup: each spectrum value read serves fb frames.
 int mac_frames(const flf_ctx* ctx, const GridTables& gt) {
-    int fb = 1;
-    while (fb * 2 <= ctx->fb_max && fb * 2 <= ctx->T && mac_lds(gt.slots, gt.nu, ctx->N, fb * 2) <= ctx->max_lds)
-        fb *= 2;
+    int fb = std::max(1, std::min(ctx->fb_max, ctx->T));
+    while (fb > 1 && mac_lds(gt.slots, gt.nu, ctx->N, fb) > ctx->max_lds) --fb;
     return fb;
 }
 
@@ -388,7 +387,7 @@ int flf_create(int device, flf_ctx** out) {
     }
     if (const char* e = getenv("FLF_MAC_FRAMES")) {
         int v = atoi(e);
-        if (v == 1 || v == 2 || v == 4) c->fb_max = v;
+        if (v >= 1 && v <= 4) c->fb_max = v;
     }
     *out = c;
     return FLF_OK;
@@ -621,7 +620,7 @@ int flf_set_option(flf_ctx* ctx, const char* name, int value) {
         if (value < 0 || value > FWD_SPLIT_MAX) return fail(ctx, FLF_EINVAL, "fwd_split must be 0..4");
         ctx->fwd_split = value;
     } else if (n == "mac_frames") {
-        if (value != 1 && value != 2 && value != 4) return fail(ctx, FLF_EINVAL, "mac_frames must be 1, 2 or 4");
+        if (value < 1 || value > 4) return fail(ctx, FLF_EINVAL, "mac_frames must be 1..4");
         ctx->fb_max = value;
     } else if (n == "mac_mfma") {
         ctx->mfma = value != 0;

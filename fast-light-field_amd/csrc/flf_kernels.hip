@@ -440,7 +440,7 @@ __device__ __forceinline__ void mac_tables(const MacArgs& a, const ChunkDesc* ch
 // workgroup: every stored spectrum value read from HBM serves FB frames
 // (grid z = frame blocks).
 template <int SLOTS, int IPT, int FB>
-__global__ __launch_bounds__(FLF_CHUNK_MAX) __attribute__((amdgpu_waves_per_eu(IPT * FB >= 4 ? 2 : (IPT * FB == 2 ? 3 : FLF_MAC_WAVES))))
+__global__ __launch_bounds__(FLF_CHUNK_MAX) __attribute__((amdgpu_waves_per_eu(IPT * FB >= 3 ? 2 : (IPT * FB == 2 ? 3 : FLF_MAC_WAVES))))
 void k_mac(MacArgs a) {
     extern __shared__ float4 pm_tab[];            // [FB][NRES][rs: nu*NPT + pad], then staging
     constexpr int NRES = SLOTS == 4 ? 8 : 4;      // residues per orbit (canonical list)
@@ -1181,15 +1181,16 @@ template <int SLOTS, int IPT>
 static void launch_mac_fb(const MacArgs& a, int fb, dim3 grid, dim3 block, size_t lds, hipStream_t s) {
     if (fb == 1) hipLaunchKernelGGL((k_mac<SLOTS, IPT, 1>), grid, block, lds, s, a);
     else if (fb == 2) hipLaunchKernelGGL((k_mac<SLOTS, IPT, 2>), grid, block, lds, s, a);
+    else if (fb == 3) hipLaunchKernelGGL((k_mac<SLOTS, IPT, 3>), grid, block, lds, s, a);
     else hipLaunchKernelGGL((k_mac<SLOTS, IPT, 4>), grid, block, lds, s, a);
 }
 hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks, int nplanes_grid, bool mfma,
                       hipStream_t s) {
-    if ((fb != 1 && fb != 2 && fb != 4) || (ipt != 1 && ipt != 2) || a.T < 1 || a.rs != mac_row_stride(slots, a.nu))
+    if (fb < 1 || fb > 4 || (ipt != 1 && ipt != 2) || a.T < 1 || a.rs != mac_row_stride(slots, a.nu))
         return hipErrorInvalidValue;
     const size_t lds = mac_lds(slots, a.nu, a.N, fb);
     dim3 grid(nchunks, nplanes_grid, (a.T + fb - 1) / fb);
-    if (mfma && fb >= 2 && ipt == 1 && a.tiles) {
+    if (mfma && (fb == 2 || fb == 4) && ipt == 1 && a.tiles) {
         dim3 block(64 * MFMA_WAVES);
         if (slots == 4) {
             if (fb == 2) hipLaunchKernelGGL((k_mac_mfma<4, 2>), grid, block, lds, s, a);
@@ -1239,10 +1240,12 @@ hipError_t launch_fit(const float* img, const float* im, long long n_per_frame, 
 hipError_t configure_kernels(size_t max_lds) {
     const void* ks[] = {(const void*)k_small_fft, (const void*)k_psf_rows, (const void*)k_psf_cols,
                         (const void*)k_ifft_cols, (const void*)k_ifft_rows_sum, (const void*)k_ifft_rows_plane,
-                        (const void*)k_mac<4, 1, 1>, (const void*)k_mac<4, 1, 2>, (const void*)k_mac<4, 1, 4>,
-                        (const void*)k_mac<4, 2, 1>, (const void*)k_mac<4, 2, 2>, (const void*)k_mac<4, 2, 4>,
-                        (const void*)k_mac<2, 1, 1>, (const void*)k_mac<2, 1, 2>, (const void*)k_mac<2, 1, 4>,
-                        (const void*)k_mac<2, 2, 1>, (const void*)k_mac<2, 2, 2>, (const void*)k_mac<2, 2, 4>,
+                        (const void*)k_mac<4, 1, 1>, (const void*)k_mac<4, 1, 2>, (const void*)k_mac<4, 1, 3>,
+                        (const void*)k_mac<4, 1, 4>, (const void*)k_mac<4, 2, 1>, (const void*)k_mac<4, 2, 2>,
+                        (const void*)k_mac<4, 2, 3>, (const void*)k_mac<4, 2, 4>, (const void*)k_mac<2, 1, 1>,
+                        (const void*)k_mac<2, 1, 2>, (const void*)k_mac<2, 1, 3>, (const void*)k_mac<2, 1, 4>,
+                        (const void*)k_mac<2, 2, 1>, (const void*)k_mac<2, 2, 2>, (const void*)k_mac<2, 2, 3>,
+                        (const void*)k_mac<2, 2, 4>,
                         (const void*)k_mac_mfma<4, 2>, (const void*)k_mac_mfma<4, 4>,
                         (const void*)k_mac_mfma<2, 2>, (const void*)k_mac_mfma<2, 4>};
     for (const void* k : ks) {

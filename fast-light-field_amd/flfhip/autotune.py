@@ -11,7 +11,7 @@ import torch
 
 CANDIDATES = {
     "fwd_split": [1, 2, 3, 4],
-    "mac_frames": [1, 2, 4],
+    "mac_frames": [1, 2, 3, 4],
     "ifft_col_batch": [2, 4, 8],
 }
 

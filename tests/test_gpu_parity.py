@@ -168,7 +168,7 @@ def test_multiframe_full_size_equals_single_frames(projector):
         projector.set_option("fwd_split", 1)
         ref_f = [projector.forward_device(hm, vol[:, t:t + 1].contiguous())[0].clone() for t in range(2)]
         ref_b = [projector.backward_device(hm, img[t:t + 1].contiguous())[:, 0].clone() for t in range(2)]
-        for fb in (2, 4):
+        for fb in (2, 3, 4):
             for sp in (1, 3):
                 projector.set_option("mac_frames", fb)
                 projector.set_option("fwd_split", sp)
