@@ -390,13 +390,20 @@ __device__ __forceinline__ void mac_tables(const MacArgs& a, const ChunkDesc* ch
         sincospif(-2.0f * (float)m / (float)f, &sn, &cs);
         psi[tid] = make_float2(cs, sn);
     }
-    for (int f = 0; f < nf; ++f) {
-        if (f > 0) __syncthreads();
-        const float2* Sp = a.S + a.S_off[p] + (long long)(t0 + f) * a.S_tstride + (long long)cidx * NN * NH;
-        for (int i = tid; i < NN * NH; i += blockDim.x) s_stage[i] = Sp[i];
-        __syncthreads();
-        float4* pmf = pm_tab + f * tab;
-        for (int idx = tid; idx < NRES * nu * NPT; idx += blockDim.x) {
+    // Stage the small-FFT blocks of all nf frames ([f][NN][NH]), then build
+    // every frame's table: one load round trip and two barriers per plane.
+    const int blk = NN * NH;
+    for (int i = tid; i < nf * blk; i += blockDim.x) {
+        const int f = i / blk, e = i - f * blk;
+        s_stage[i] = a.S[a.S_off[p] + (long long)(t0 + f) * a.S_tstride + (long long)cidx * blk + e];
+    }
+    __syncthreads();
+    {
+        const int per = NRES * nu * NPT;
+        for (int fi = tid; fi < nf * per; fi += blockDim.x) {
+            const int f = fi / per, idx = fi - f * per;
+            float4* pmf = pm_tab + f * tab;
+            const float2* sf = s_stage + f * blk;
             const int pt = idx % NPT;
             const int ru = idx / NPT;
             const int u = ru % nu, rho = ru / nu;
@@ -408,7 +415,7 @@ __device__ __forceinline__ void mac_tables(const MacArgs& a, const ChunkDesc* ch
                 float2 r = make_float2(0.f, 0.f);
                 if (off < NN) {
                     const int bo = off / N, ao = off - bo * N;
-                    r = s_stage[off * NH + rb0];
+                    r = sf[off * NH + rb0];
                     if (rho >= NH) r = cconj(r);
                     if (bo > c) r = cmul(r, psi[2 * rho]);
                     if (ao > c) r = cmul(r, psi[2 * rho + 1]);
@@ -470,8 +477,8 @@ void k_mac(MacArgs a) {
     // different residues, which then fall on distinct LDS banks.
     const int rs = a.rs;
     const int tab = NRES * rs;                    // table entries per frame
-    float2* s_stage = reinterpret_cast<float2*>(pm_tab + FB * tab);   // [NN][NH]
-    float2* psi = s_stage + NN * NH;                                        // [NRES][2]
+    float2* s_stage = reinterpret_cast<float2*>(pm_tab + FB * tab);   // [FB][NN][NH]
+    float2* psi = s_stage + FB * NN * NH;                                   // [NRES][2]
     // The item descriptors are re-read (from the caches) where they are
     // needed instead of living in registers across the spectrum loop.
     const ItemDesc* itp = a.items + item_base + (long long)tsafe * IPT;
@@ -672,8 +679,8 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void k_mac_mfma(MacArgs a) {
     const int nu = a.nu;
     const int rs = a.rs;
     const int tab = NRES * rs;
-    float2* s_stage = reinterpret_cast<float2*>(pm_tab + FB * tab);
-    float2* psi = s_stage + NN * NH;
+    float2* s_stage = reinterpret_cast<float2*>(pm_tab + FB * tab);   // [FB][NN][NH]
+    float2* psi = s_stage + FB * NN * NH;
     const float* pmf = reinterpret_cast<const float*>(pm_tab);
     const TileDesc* tiles = a.tiles + ch->tile_base;
     const int ntiles = ch->n_tiles;
@@ -1175,7 +1182,7 @@ int mac_row_stride(int slots, int nu) {
 size_t mac_lds(int slots, int nu, int N, int fb) {
     const int nres = slots == 4 ? 8 : 4;
     return (size_t)fb * nres * mac_row_stride(slots, nu) * sizeof(float4) +
-           ((size_t)N * N * (nres / 2) + 2 * nres) * sizeof(float2);
+           ((size_t)fb * N * N * (nres / 2) + 2 * nres) * sizeof(float2);
 }
 template <int SLOTS, int IPT>
 static void launch_mac_fb(const MacArgs& a, int fb, dim3 grid, dim3 block, size_t lds, hipStream_t s) {
