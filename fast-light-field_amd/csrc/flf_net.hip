@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <string>
 #include <vector>
 
@@ -348,16 +349,17 @@ __global__ void k_weights_bf16(const float* w, int Co, int Ci, int Cip, int Cop,
 
 // Nearest upsampling of the lenslet-resolution guess to the volume:
 // vol[z][t][y][x] = g[t][z][y / N][x / N].
+// One workgroup per output row (z, t, y): 32-bit index math, coalesced
+// row writes, the row's lenslet values read through the cache.
 __global__ void k_upsample(const float* g, int T, int nz, int Ly, int Lx, int N, float* vol) {
     const int Y = Ly * N, X = Lx * N;
-    const long long total = (long long)nz * T * Y * X;
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (long long)gridDim.x * blockDim.x) {
-        const int x = (int)(idx % X);
-        const int y = (int)((idx / X) % Y);
-        const int t = (int)((idx / ((long long)X * Y)) % T);
-        const int z = (int)(idx / ((long long)X * Y * T));
-        vol[idx] = g[(((long long)t * nz + z) * Ly + y / N) * Lx + x / N];
+    const int rows = nz * T * Y;
+    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int y = r % Y, zt = r / Y;
+        const int t = zt % T, z = zt / T;
+        const float* gr = g + (((long long)t * nz + z) * Ly + y / N) * Lx;
+        float* out = vol + (long long)r * X;
+        for (int x = threadIdx.x; x < X; x += blockDim.x) out[x] = gr[x / N];
     }
 }
 
@@ -739,9 +741,10 @@ int flf_net_train_step(flf_net* net, const float* img, const float* target, int 
 
 int flf_net_upsample(const float* g, int T, int nz, int Ly, int Lx, int nnum, float* vol, void* stream) {
     if (!g || !vol || T < 1 || nz < 1 || Ly < 1 || Lx < 1 || nnum < 1) return FLF_EINVAL;
-    const long long total = (long long)nz * T * Ly * nnum * Lx * nnum;
-    hipLaunchKernelGGL(k_upsample, dim3(blocks_for(total)), dim3(256), 0, (hipStream_t)stream, g, T, nz, Ly, Lx, nnum,
-                       vol);
+    const long long rows = (long long)nz * T * Ly * nnum;
+    if (rows > INT32_MAX || (long long)Lx * nnum > INT32_MAX) return FLF_EINVAL;
+    hipLaunchKernelGGL(k_upsample, dim3((unsigned)std::min<long long>(rows, 65536)), dim3(256), 0, (hipStream_t)stream,
+                       g, T, nz, Ly, Lx, nnum, vol);
     return hipGetLastError() == hipSuccess ? FLF_OK : FLF_EHIP;
 }
 

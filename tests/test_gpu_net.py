@@ -156,3 +156,17 @@ def test_upsample_exact(lib):
     assert rc == 0
     ref = np.repeat(np.repeat(g.cpu().numpy(), N, axis=2), N, axis=3).transpose(1, 0, 2, 3)
     assert np.array_equal(vol.cpu().numpy(), ref)
+
+
+def test_upsample_full_size_rows(lib):
+    """Upsampling at the 1455^2, 41-plane size with two frames: more output
+    rows (119310) than workgroups, so the row loop wraps."""
+    from flfhip.lensletnet import LensletNet  # noqa: F401  (loads the library)
+    T, nz, Ly, Lx, N = 2, 41, 97, 97, 15
+    g = torch.rand((T, nz, Ly, Lx), dtype=torch.float32, device="cuda")
+    vol = torch.empty((nz, T, Ly * N, Lx * N), dtype=torch.float32, device="cuda")
+    rc = lib.flf_net_upsample(ctypes.c_void_p(g.data_ptr()), T, nz, Ly, Lx, N, ctypes.c_void_p(vol.data_ptr()),
+                              ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    ref = g.repeat_interleave(N, dim=2).repeat_interleave(N, dim=3).permute(1, 0, 2, 3)
+    assert torch.equal(vol, ref)
