@@ -78,6 +78,7 @@ struct flf_ctx {
                                       // per thread); plan time
     int fb_max = 2;                   // frames per MAC workgroup at T >= 2 (FLF_MAC_FRAMES overrides)
     int fwd_split = 0;                // forward plane ranges, 0 = automatic (FLF_FWD_SPLIT overrides)
+    int bwd_planes = 1;               // planes per backward MAC workgroup
     bool mfma = false;                // matrix-core MAC for >= 2 frames per workgroup (FLF_MAC_MFMA=1 enables;
                                       // measured slower than the vector kernel, see the kernel's comment)
     long long bytes = 0;
@@ -319,7 +320,12 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
         m.N = N; m.fY = gt.fY; m.fX = gt.fX; m.rY = gt.rY; m.rX = gt.rX; m.rfX = gt.rfX; m.nu = gt.nu;
         m.T = T;
         m.rs = mac_row_stride(gt.slots, gt.nu);
-        rc = run_mac(ctx, m, g, m.nP, st);
+        // Each workgroup takes bwd_planes consecutive planes (loads run on
+        // across the plane boundary) and stores every plane's accumulator;
+        // the matrix-core kernel keeps one plane per workgroup.
+        m.per_plane = 1;
+        const int bp = ctx->mfma ? 1 : std::max(1, ctx->bwd_planes);
+        rc = run_mac(ctx, m, g, (m.nP + bp - 1) / bp, st);
         if (rc) return rc;
         IfftColArgs ic{};
         ic.A = g.d_A_bwd;
@@ -624,6 +630,9 @@ int flf_set_option(flf_ctx* ctx, const char* name, int value) {
         ctx->fb_max = value;
     } else if (n == "mac_mfma") {
         ctx->mfma = value != 0;
+    } else if (n == "bwd_planes") {
+        if (value < 1 || value > 8) return fail(ctx, FLF_EINVAL, "bwd_planes must be 1..8");
+        ctx->bwd_planes = value;
     } else if (n == "ifft_col_batch") {
         if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");
         if (value < 1 || value > 8) return fail(ctx, FLF_EINVAL, "ifft_col_batch must be 1..8");
@@ -643,6 +652,7 @@ int flf_get_option(const flf_ctx* ctx, const char* name, int* value) {
     if (n == "fwd_split") *value = ctx->fwd_split;
     else if (n == "mac_frames") *value = ctx->fb_max;
     else if (n == "mac_mfma") *value = ctx->mfma ? 1 : 0;
+    else if (n == "bwd_planes") *value = ctx->bwd_planes;
     else if (n == "ifft_col_batch") *value = ctx->groups.empty() ? 0 : ctx->groups[0].CBi;
     else return FLF_EINVAL;
     return FLF_OK;

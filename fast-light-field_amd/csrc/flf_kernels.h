@@ -70,6 +70,9 @@ struct MacArgs {
     int T;                       // frames (grid z covers them in blocks of FB)
     const TileDesc* tiles;       // residue-class tiles (matrix-core kernel)
     int rs;                      // P/M table residue-row stride (mac_row_stride)
+    int per_plane;               // 1: store each plane's accumulator at A + p * A_pstride (back
+                                 // projection); 0: sum the planes of a range, store at
+                                 // A + range * A_pstride (forward projection)
 };
 
 struct IfftColArgs {

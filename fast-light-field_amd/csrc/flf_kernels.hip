@@ -533,6 +533,24 @@ void k_mac(MacArgs a) {
         }
     };
 
+    // Store the totals at A + slice * A_pstride (each lane of a pair stores
+    // its own item), frames of this block only.
+    auto store = [&](int slice) {
+        const ItemDesc* ip = items_now();
+#pragma unroll
+        for (int f = 0; f < FB; ++f) {
+            if (f >= nf) break;
+            float2* A = a.A + (long long)(t0 + f) * a.A_tstride + (long long)slice * a.A_pstride;
+#pragma unroll
+            for (int q = 0; q < QT; ++q) {
+                const ItemDesc& it = ip[IPT == 2 ? lsub : q];
+#pragma unroll
+                for (int j = 0; j < SLOTS; ++j)
+                    if ((it.writemask >> j) & 1) A[it.pos[j]] = tot[f][q][j];
+            }
+        }
+    };
+
     float2 acc[FB][IPT][4];
     // Epilogue of plane p: outer phase and the 'same'-crop shift.
     auto epilogue = [&](int p) {
@@ -581,6 +599,15 @@ void k_mac(MacArgs a) {
                 }
             }
         }
+        if (a.per_plane) {                       // back projection: one output per plane
+            if (valid) store(p);
+#pragma unroll
+            for (int f = 0; f < FB; ++f)
+#pragma unroll
+                for (int q = 0; q < QT; ++q)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) tot[f][q][j] = make_float2(0.f, 0.f);
+        }
     };
 
     // Software pipeline over (plane, batch): the loads of the next batch (also
@@ -623,21 +650,7 @@ void k_mac(MacArgs a) {
         }
     }
 
-    if (valid) {
-        const ItemDesc* ip = items_now();
-#pragma unroll
-        for (int f = 0; f < FB; ++f) {
-            if (f >= nf) break;
-            float2* A = a.A + (long long)(t0 + f) * a.A_tstride + (long long)blockIdx.y * a.A_pstride;
-#pragma unroll
-            for (int q = 0; q < QT; ++q) {
-                const ItemDesc& it = ip[IPT == 2 ? lsub : q];   // each lane of a pair stores its own item
-#pragma unroll
-                for (int j = 0; j < SLOTS; ++j)
-                    if ((it.writemask >> j) & 1) A[it.pos[j]] = tot[f][q][j];
-            }
-        }
-    }
+    if (valid && !a.per_plane) store(blockIdx.y);
 }
 
 // Matrix-core variant for FB >= 2 frames per workgroup.  The items of a chunk

@@ -3,8 +3,9 @@ candidate setting at the caller's shape and keeps the fastest.
 
 The tuned options (flf_set_option) are the forward MAC's plane ranges
 (``fwd_split``), the frames per MAC workgroup (``mac_frames``, only when a call
-holds several frames) and the columns per column-transform workgroup
-(``ifft_col_batch``).  Each is searched with the others fixed (coordinate
+holds several frames), the columns per column-transform workgroup
+(``ifft_col_batch``) and the planes per back-projection MAC workgroup
+(``bwd_planes``).  Each is searched with the others fixed (coordinate
 descent, one pass); results depend on them only through FP32 rounding.
 """
 import torch
@@ -13,6 +14,7 @@ CANDIDATES = {
     "fwd_split": [1, 2, 3, 4],
     "mac_frames": [1, 2, 3, 4],
     "ifft_col_batch": [2, 4, 8],
+    "bwd_planes": [1, 2, 3, 4],
 }
 
 

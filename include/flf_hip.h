@@ -95,6 +95,7 @@ int flf_set_frame_dims(flf_ctx* ctx, const int* dims, int T, void* stream);
  *   "fwd_split"      0 (automatic) or 1..4 plane ranges of the forward MAC
  *   "mac_frames"     1..4 frames per MAC workgroup (upper bound)
  *   "mac_mfma"       0/1: matrix-core MAC for >= 2 frames per workgroup
+ *   "bwd_planes"     1..8 planes per back-projection MAC workgroup
  *   "ifft_col_batch" 1..8 columns per column-transform workgroup (after plan)
  * Results do not depend on them beyond FP32 rounding.  flfhip.autotune picks
  * them by timing. */

@@ -132,9 +132,10 @@ def test_autotune_and_options_keep_parity(projector):
     T, Y, X = 2, 120, 150
     vol = rng.random((3, T, Y, X)).astype(np.float32)
     ref = [sum(oproj.project_special(vol[z, t][None], hm.Hcc(z, False))[0] for z in range(3)) for t in range(T)]
+    saved = {k: projector.get_option(k) for k in ("fwd_split", "mac_frames", "ifft_col_batch", "bwd_planes")}
     chosen, timings = autotune.tune(projector, hm, T, Y, X, reps=1)
     print("autotune", chosen, flush=True)
-    assert set(chosen) >= {"fwd_split", "ifft_col_batch", "mac_frames"}
+    assert set(chosen) >= {"fwd_split", "ifft_col_batch", "mac_frames", "bwd_planes"}
     settings = [dict(chosen)] + [{"fwd_split": v} for v in (1, 4)] + [{"ifft_col_batch": 2}, {"mac_frames": 1}]
     for opts in settings:
         for k, v in opts.items():
@@ -142,7 +143,7 @@ def test_autotune_and_options_keep_parity(projector):
         fwd = projector.ForwardProjectACC(hm, vol, [0, 1, 2])
         for t in range(T):
             assert rel_l2(fwd[t], ref[t]) < 1e-5, (opts, t)
-    for k, v in {"fwd_split": 0, "mac_frames": 2, "ifft_col_batch": 8}.items():
+    for k, v in saved.items():
         projector.set_option(k, v)
     with pytest.raises(Exception):
         projector.set_option("no_such_option", 1)
@@ -163,15 +164,18 @@ def test_multiframe_full_size_equals_single_frames(projector):
     g.manual_seed(3)
     vol = torch.rand((nz, 2, Y, X), generator=g, device=projector.device)
     img = torch.rand((2, Y, X), generator=g, device=projector.device)
+    bwd0 = projector.get_option("bwd_planes")
     try:
         projector.set_option("mac_frames", 1)
         projector.set_option("fwd_split", 1)
+        projector.set_option("bwd_planes", 1)
         ref_f = [projector.forward_device(hm, vol[:, t:t + 1].contiguous())[0].clone() for t in range(2)]
         ref_b = [projector.backward_device(hm, img[t:t + 1].contiguous())[:, 0].clone() for t in range(2)]
         for fb in (2, 3, 4):
             for sp in (1, 3):
                 projector.set_option("mac_frames", fb)
                 projector.set_option("fwd_split", sp)
+                projector.set_option("bwd_planes", sp)
                 two_f = projector.forward_device(hm, vol)
                 two_b = projector.backward_device(hm, img)
                 torch.cuda.synchronize()
@@ -183,6 +187,7 @@ def test_multiframe_full_size_equals_single_frames(projector):
     finally:
         projector.set_option("mac_frames", 2)
         projector.set_option("fwd_split", 0)
+        projector.set_option("bwd_planes", bwd0)
 
 
 def test_one_item_per_thread_matches_lane_pairs(projector):
@@ -249,3 +254,36 @@ def test_forward_pair_equals_concatenated_frames(projector):
     torch.cuda.synchronize()
     assert torch.equal(X1, X2)
     assert float(X2[0, 0, 0, 0]) == 0.0                   # NaN -> 0
+
+
+@pytest.mark.parametrize("planes", [None, [0, 2, 3]])
+def test_backward_planes_per_workgroup(projector, planes):
+    """Back projection with 1..8 planes per MAC workgroup (bwd_planes): every
+    setting gives bit-identical planes (the per-plane arithmetic does not
+    depend on the grouping) within 1e-5 of the oracle; planes left out of
+    the call stay zero."""
+    import torch
+    hm = _hm_random(15, [15, 31, 45, 31, 61], seed=41)
+    rng = np.random.default_rng(9)
+    T, Y, X = 2, 105, 120
+    img = rng.random((T, Y, X)).astype(np.float32)
+    sel = list(range(5)) if planes is None else planes
+    ref = {z: np.stack([oproj.project_special(img[t][None], hm.Hcc(z, True))[0] for t in range(T)]) for z in sel}
+    saved = projector.get_option("bwd_planes")
+    base = None
+    try:
+        for bp in (1, 2, 3, 4, 8):
+            projector.set_option("bwd_planes", bp)
+            vol = projector.backward_device(hm, projector.asnative(img), planes=planes)
+            torch.cuda.synchronize()
+            v = vol.cpu().numpy()
+            if base is None:
+                base = v
+            assert np.array_equal(v, base), bp
+            for z in range(5):
+                if z in sel:
+                    assert rel_l2(v[z], ref[z]) < 1e-5, (bp, z)
+                else:
+                    assert not v[z].any(), (bp, z)
+    finally:
+        projector.set_option("bwd_planes", saved)
