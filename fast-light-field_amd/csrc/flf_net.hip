@@ -84,18 +84,34 @@ __global__ __launch_bounds__(256) void k_gemm_nt(GemmArgs g) {
     const bf16* pb0 = g.B + (long long)(b0ok ? bn0 : 0) * g.ldb + lk;
     const bf16* pb1 = g.B + (long long)(b1ok ? bn1 : 0) * g.ldb + lk;
     f32x16 c0 = {}, c1 = {};
-    if (kb < ke) {
-        bf16x8 a = load8(pa + kb, aok), b0 = load8(pb0 + kb, b0ok), b1 = load8(pb1 + kb, b1ok);
-        for (int k = kb; k < ke; k += 16) {
-            bf16x8 an = a, b0n = b0, b1n = b1;
-            if (k + 16 < ke) {
-                an = load8(pa + k + 16, aok);
-                b0n = load8(pb0 + k + 16, b0ok);
-                b1n = load8(pb1 + k + 16, b1ok);
+    // Ring of PF k-steps whose operand loads are in flight while earlier
+    // steps run on the matrix cores (the small-M inference GEMMs are bound
+    // by load latency); the MFMA order along K is unchanged.
+    constexpr int PF = 4;
+    const int nsteps = ke > kb ? (ke - kb) / 16 : 0;
+    bf16x8 ra[PF], rb0[PF], rb1[PF];
+#pragma unroll
+    for (int d = 0; d < PF; ++d) {
+        const bool in = d < nsteps;
+        const int k = kb + 16 * d;
+        ra[d] = load8(pa + (in ? k : kb), aok && in);
+        rb0[d] = load8(pb0 + (in ? k : kb), b0ok && in);
+        rb1[d] = load8(pb1 + (in ? k : kb), b1ok && in);
+    }
+    for (int s0 = 0; s0 < nsteps; s0 += PF) {
+#pragma unroll
+        for (int d = 0; d < PF; ++d) {
+            const int st = s0 + d;
+            if (st < nsteps) {
+                c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ra[d], rb0[d], c0, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ra[d], rb1[d], c1, 0, 0, 0);
+                if (st + PF < nsteps) {
+                    const int k = kb + 16 * (st + PF);
+                    ra[d] = load8(pa + k, aok);
+                    rb0[d] = load8(pb0 + k, b0ok);
+                    rb1[d] = load8(pb1 + k, b1ok);
+                }
             }
-            c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, c0, 0, 0, 0);
-            c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1, c1, 0, 0, 0);
-            a = an; b0 = b0n; b1 = b1n;
         }
     }
     // epilogue
