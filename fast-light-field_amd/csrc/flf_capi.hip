@@ -608,11 +608,13 @@ int flf_set_frame_dims(flf_ctx* ctx, const int* dims, int T, void* stream) {
         ctx->ragged = false;
         return FLF_OK;
     }
-    // The host copy outlives the asynchronous upload (it is only rewritten
-    // by the next call, which is ordered after this one on the caller's stream).
+    // Ordered on the caller's stream after the work already queued (which may
+    // still read the previous sizes); the stream is synchronised so that the
+    // host copy can be rewritten by the next call.
     ctx->fdims_host.assign(d.begin(), d.end());
     HIPCHK(hipMemcpyAsync(ctx->d_fdims, ctx->fdims_host.data(), T * sizeof(int2), hipMemcpyHostToDevice,
                           (hipStream_t)stream));
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
     ctx->ragged = true;
     return FLF_OK;
 }

@@ -317,7 +317,8 @@ class LearnedInit(object):
         a rejected frame's plain iterate counts as one iteration."""
         from .lfdeconv import deconv_rl_device
         X, HX, info = self.initial_volume(im, Htf)
+        fit0 = np.where(info["accepted"], info["fit_guess"], info["fit_plain1"])   # fit of the starting X
         n = deconv_rl_device(self.projector, self.hMatrix, Htf, maxIter, X, im=im, targetFit=targetFit,
-                             fitOut=fitOut, HX0=HX)
+                             fitOut=fitOut, HX0=HX, fit0=fit0)
         info["iterations"] = n + (0 if info["accepted"].all() else 1)
         return X, info["iterations"], info

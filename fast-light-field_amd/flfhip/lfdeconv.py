@@ -65,26 +65,29 @@ def _fit_from_sums(sums):
 
 
 def deconv_rl_device(projector, hMatrix, Htf, maxIter, X, im=None, prevedel=True, trust=None, targetFit=None,
-                     fitOut=None, planes=None, HX0=None):
+                     fitOut=None, planes=None, HX0=None, fit0=None):
     """Device-resident RL iterations.  Htf, X (numZ, T, Y, X) and im (T, Y, X)
     are float32 tensors on the projector's device; X is updated in place.
     Returns the number of iterations run.  With targetFit (array-like per
     frame, or scalar) it stops once every frame's fit of the current X is
     <= target (the fit of X is measured by the forward projection that
     starts the next iteration, so this costs no extra projection).  HX0, if
-    given, is the forward projection of the starting X (saves one)."""
+    given, is the forward projection of the starting X (saves one); fit0,
+    with HX0, is its per-frame data fit (saves one reduction)."""
     it = 0
     target = None if targetFit is None else np.broadcast_to(np.asarray(targetFit, np.float64), (X.shape[1],))
     need_fit = im is not None and (fitOut is not None or target is not None)
     while True:
         if it >= maxIter and not need_fit:
             break
+        known = None
         if HX0 is not None:
             HX, HX0 = HX0, None
+            known, fit0 = fit0, None
         else:
             HX = projector.forward_device(hMatrix, X, planes)
         if need_fit:
-            fit = _fit_from_sums(projector.datafit_device(HX, im))
+            fit = np.asarray(known, np.float64) if known is not None else _fit_from_sums(projector.datafit_device(HX, im))
             if fitOut is not None:
                 fitOut.append(fit)
             if target is not None and np.all(fit <= target):

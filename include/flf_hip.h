@@ -86,7 +86,8 @@ int flf_forward_pair(flf_ctx* ctx, const float* volA, const float* volB, float* 
  * (deconvolve_time_series.py:21-23) and plans one convolution shape per
  * image size (special_fftconvolve.py:142-174); here frames of uneven
  * lenslet grids share one plan and one spectrum cache.  flf_set_frames
- * resets to full-size frames.  The upload is ordered on stream. */
+ * resets to full-size frames.  The upload is ordered on stream, which is
+ * synchronised before returning. */
 int flf_set_frame_dims(flf_ctx* ctx, const int* dims, int T, void* stream);
 
 /* Launch-shape options (no reference counterpart; the reference's
