@@ -427,6 +427,8 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
     // Group planes by Fourier grid.
     for (int z = 0; z < nz; ++z) {
         int fY = fourier_len(Y, s2[z], nnum), fX = fourier_len(X, s2[z], nnum);
+        if (fY > FLF_MAX_FOURIER || fX > FLF_MAX_FOURIER)
+            return fail(ctx, FLF_EINVAL, "Fourier grid longer than " + std::to_string(FLF_MAX_FOURIER));
         int gi = -1;
         for (size_t i = 0; i < ctx->groups.size(); ++i)
             if (ctx->groups[i].gt.fY == fY && ctx->groups[i].gt.fX == fX) gi = (int)i;

@@ -35,6 +35,15 @@ __device__ __forceinline__ void cmacc(float2& acc, float2 s, float2 g) {
 }
 
 // e^{-2 pi i * num / den} for integers, with the argument reduced exactly.
+// 32-bit form for |num| < 2^31 (the MAC epilogues: |outer| < 4N (s2 + N)).
+__device__ __forceinline__ float2 phase_frac_i(int num, int den) {
+    int m = num % den;
+    if (m < 0) m += den;
+    if (2 * m > den) m -= den;
+    float s, c;
+    sincospif(-2.0f * (float)m / (float)den, &s, &c);
+    return make_float2(c, s);
+}
 __device__ __forceinline__ float2 phase_frac(long long num, long long den) {
     long long m = num % den;
     if (m < 0) m += den;

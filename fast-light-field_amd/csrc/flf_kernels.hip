@@ -583,10 +583,11 @@ void k_mac(MacArgs a) {
 #pragma unroll
             for (int j = 0; j < SLOTS; ++j) {
                 const int k0 = itq.pos[j] / a.rfX, k1 = itq.pos[j] - k0 * a.rfX;
-                const float2 po = phase_frac(outer[j], fourN);
+                const float2 po = phase_frac_i(outer[j], fourN);
                 // exp(+2 pi i c2 (k0/fY + k1/fX))
-                const int e0 = (int)(((long long)c2 * k0) % a.fY);
-                const int e1 = (int)(((long long)c2 * k1) % a.fX);
+                // c2 < f / 2 and k < f <= FLF_MAX_FOURIER: 32-bit products
+                const int e0 = (c2 * k0) % a.fY;
+                const int e1 = (c2 * k1) % a.fX;
                 float fr = (float)e0 / (float)a.fY + (float)e1 / (float)a.fX;
                 fr -= rintf(fr);
                 float sn, cs;
@@ -821,9 +822,9 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void k_mac_mfma(MacArgs a) {
                                         : jj == 2 ? sig * (n2 + m2) : sig * (m2 - n2);
                         const int pos = it.pos[jj];
                         const int k0 = pos / a.rfX, k1 = pos - k0 * a.rfX;
-                        const float2 po = phase_frac(outer, fourN);
-                        const int e0 = (int)(((long long)c2 * k0) % a.fY);
-                        const int e1 = (int)(((long long)c2 * k1) % a.fX);
+                        const float2 po = phase_frac_i(outer, fourN);
+                        const int e0 = (c2 * k0) % a.fY;
+                        const int e1 = (c2 * k1) % a.fX;
                         float fr = (float)e0 / (float)a.fY + (float)e1 / (float)a.fX;
                         fr -= rintf(fr);
                         float sn, cs;
