@@ -20,30 +20,50 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// q = n / d and r = n % d for 0 <= n < 2^22 and d >= 1, from the float
+// reciprocal rd = 1/d of the pass (one correction step makes it exact:
+// n * rd is within 0.5 of n / d).
+__device__ __forceinline__ int div_small(int n, int d, float rd, int& r) {
+    int q = (int)((float)n * rd);
+    r = n - q * d;
+    if (r < 0) { --q; r += d; } else if (r >= d) { ++q; r -= d; }
+    return q;
+}
+
+// a[t] *= w^t for t = 1..R-1, w = tw[k * tstep] (conjugated for the inverse):
+// one table read, the higher powers by products (a few ulp).
+template <int R>
+__device__ __forceinline__ void twiddle_powers(float2 (&a)[5], const float2* __restrict__ tw, int idx, bool inv) {
+    float2 w1 = tw[idx];
+    if (inv) w1.y = -w1.y;
+    float2 w[5];
+    w[1] = w1;
+    if (R > 2) w[2] = cmul(w1, w1);
+    if (R > 3) w[3] = cmul(w[2], w1);
+    if (R > 4) w[4] = cmul(w[2], w[2]);
+#pragma unroll
+    for (int t = 1; t < R; ++t) a[t] = cmul(a[t], w[t]);
+}
+
 template <int R, int LMAX>
 __device__ __forceinline__ void wave_pass(float2* buf, int es, int L, int Ns, const float2* __restrict__ tw,
                                           bool inv, int lane) {
     constexpr int MAXQ = (LMAX / R + 63) / 64;
     const int M = L / R;
     const int tstep = L / (Ns * R);
+    const float rNs = 1.0f / (float)Ns;
     const float sign = inv ? 1.f : -1.f;
     float2 v[MAXQ][R];
 #pragma unroll
     for (int q = 0; q < MAXQ; ++q) {
         const int j = lane + 64 * q;
         if (j < M) {
-            const int k = j % Ns;
+            int k;
+            div_small(j, Ns, rNs, k);
             float2 a[5];
 #pragma unroll
             for (int t = 0; t < R; ++t) a[t] = buf[(j + t * M) * es];
-            if (k != 0) {
-#pragma unroll
-                for (int t = 1; t < R; ++t) {
-                    float2 w = tw[t * k * tstep];
-                    if (inv) w.y = -w.y;
-                    a[t] = cmul(a[t], w);
-                }
-            }
+            if (k != 0) twiddle_powers<R>(a, tw, k * tstep, inv);
             dft_small<R>(a, sign);
 #pragma unroll
             for (int t = 0; t < R; ++t) v[q][t] = a[t];
@@ -54,8 +74,8 @@ __device__ __forceinline__ void wave_pass(float2* buf, int es, int L, int Ns, co
     for (int q = 0; q < MAXQ; ++q) {
         const int j = lane + 64 * q;
         if (j < M) {
-            const int k = j % Ns;
-            const int o = (j / Ns) * Ns * R + k;
+            int k;
+            const int o = div_small(j, Ns, rNs, k) * Ns * R + k;   // recomputed: fewer live registers
 #pragma unroll
             for (int t = 0; t < R; ++t) buf[(o + t * Ns) * es] = v[q][t];
         }
@@ -90,26 +110,23 @@ __device__ __forceinline__ void wave_pass_ns(float2* buf, int es, int ss, int ns
     const int M = L / R;
     const int tot = nseq * M;
     const int tstep = L / (Ns * R);
+    const float rNs = 1.0f / (float)Ns, rM = 1.0f / (float)M;
     const float sign = inv ? 1.f : -1.f;
     float2 v[MAXQ][R];
+    int ov[MAXQ];
 #pragma unroll
     for (int q = 0; q < MAXQ; ++q) {
         const int j = lane + 64 * q;
         if (j < tot) {
-            const int sq = j / M, jj = j - sq * M;
-            const int k = jj % Ns;
+            int jj, k;
+            const int sq = div_small(j, M, rM, jj);
+            const int g = div_small(jj, Ns, rNs, k);
+            ov[q] = sq * ss + (g * Ns * R + k) * es;
             const float2* b = buf + sq * ss;
             float2 a[5];
 #pragma unroll
             for (int t = 0; t < R; ++t) a[t] = b[(jj + t * M) * es];
-            if (k != 0) {
-#pragma unroll
-                for (int t = 1; t < R; ++t) {
-                    float2 w = tw[t * k * tstep];
-                    if (inv) w.y = -w.y;
-                    a[t] = cmul(a[t], w);
-                }
-            }
+            if (k != 0) twiddle_powers<R>(a, tw, k * tstep, inv);
             dft_small<R>(a, sign);
 #pragma unroll
             for (int t = 0; t < R; ++t) v[q][t] = a[t];
@@ -120,12 +137,9 @@ __device__ __forceinline__ void wave_pass_ns(float2* buf, int es, int ss, int ns
     for (int q = 0; q < MAXQ; ++q) {
         const int j = lane + 64 * q;
         if (j < tot) {
-            const int sq = j / M, jj = j - sq * M;
-            const int k = jj % Ns;
-            const int o = (jj / Ns) * Ns * R + k;
-            float2* b = buf + sq * ss;
+            const int o = ov[q];
 #pragma unroll
-            for (int t = 0; t < R; ++t) b[(o + t * Ns) * es] = v[q][t];
+            for (int t = 0; t < R; ++t) buf[o + t * Ns * es] = v[q][t];
         }
     }
     wave_sync();
