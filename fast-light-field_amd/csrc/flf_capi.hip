@@ -21,7 +21,8 @@ namespace {
 struct Group {
     GridTables gt;
     std::vector<int> planes;          // global plane ids, ascending
-    RadixPlan rpY, rpX, rpYr, rpXr;   // fY, fX, rY, rX
+    RadixPlan rpY, rpX, rpYr, rpXr;   // fY, fX (block-level PSF transforms), rY, rX (small FFT)
+    RadixPlan rpYw, rpXw, rpXhw;      // fY, fX, fX/2 for the wave-level inverse transforms (wide radices)
     int RB = 1, CBp = 1, CBi = 1;     // FFT batching (psf rows, psf cols, ifft cols)
     ChunkDesc* d_chunks = nullptr;
     ItemDesc* d_items = nullptr;
@@ -332,7 +333,7 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
         ic.A_bstride = (long long)gt.fY * gt.rfX;
         ic.nsum = 1;
         ic.rfX = gt.rfX; ic.CB = g.CBi; ic.Yout = ctx->Y;
-        ic.rpY = g.rpY;
+        ic.rpY = g.rpYw;
         ic.twY = g.twY;
         ic.out = ctx->d_T_bwd + g.Tbwd_off;
         ic.out_bstride = (long long)ctx->Y * gt.rfX;
@@ -440,7 +441,8 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
             if (!e.empty()) return fail(ctx, FLF_EINVAL, e);
             if (!make_radix_plan(fY, g.rpY) || !make_radix_plan(fX, g.rpX) ||
                 !make_radix_plan(g.gt.rY, g.rpYr) || !make_radix_plan(g.gt.rX, g.rpXr) ||
-                (fX % 2 == 0 && !make_radix_plan(fX / 2, g.rpXh)))
+                !make_radix_plan(fY, g.rpYw, true) || !make_radix_plan(fX, g.rpXw, true) ||
+                (fX % 2 == 0 && (!make_radix_plan(fX / 2, g.rpXh) || !make_radix_plan(fX / 2, g.rpXhw, true))))
                 return fail(ctx, FLF_EINVAL, "Fourier grid is not 5-smooth");
             gi = (int)ctx->groups.size() - 1;
         }
@@ -534,8 +536,8 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
             rd.T_tstride = (long long)Y * gt.rfX;
             rd.rfX = gt.rfX;
             rd.scale = 1.0f / ((float)gt.fY * (float)gt.fX);
-            rd.rp = g.rpX;
-            rd.rph = g.rpXh;
+            rd.rp = g.rpXw;
+            rd.rph = g.rpXhw;
             rd.twf = g.twX;
             rd.twh = g.twXh;
         }
@@ -765,8 +767,8 @@ int run_forward(flf_ctx* ctx, const float* vol, const float* vol2, float* img, c
         rs.grp[gi].tstride = (long long)ctx->Y * gt.rfX;
         rs.grp[gi].rfX = gt.rfX;
         rs.grp[gi].scale = 1.0f / ((float)gt.fY * (float)gt.fX);
-        rs.grp[gi].rp = g.rpX;
-        rs.grp[gi].rph = g.rpXh;
+        rs.grp[gi].rp = g.rpXw;
+        rs.grp[gi].rph = g.rpXhw;
         rs.grp[gi].twf = g.twX;
         rs.grp[gi].twh = g.twXh;
         if (!any) continue;
@@ -815,7 +817,7 @@ int run_forward(flf_ctx* ctx, const float* vol, const float* vol2, float* img, c
         ic.nsum = sp;
         ic.A_sstride = m.A_pstride;
         ic.rfX = gt.rfX; ic.CB = g.CBi; ic.Yout = ctx->Y;
-        ic.rpY = g.rpY;
+        ic.rpY = g.rpYw;
         ic.twY = g.twY;
         ic.out = g.d_T_fwd;
         ic.out_bstride = (long long)ctx->Y * gt.rfX;

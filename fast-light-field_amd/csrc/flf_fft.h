@@ -13,8 +13,63 @@
 
 namespace flf {
 
+// Largest radix of a pass (wave-level transforms; the block-level passes
+// below use 2..5).
+constexpr int FLF_RADIX_MAX = 8;
+
 template <int R>
-__device__ __forceinline__ void dft_small(float2 (&a)[5], float sign) {
+__device__ __forceinline__ void dft_small(float2* a, float sign);
+
+// cos and sin of 2 pi m / N for m = 0..3: the internal twiddles of the
+// two-level butterflies below (N = 6, 8).
+template <int N>
+__device__ __forceinline__ float2 unit_root(int m, float sign) {
+    constexpr float c6[4] = {1.000000000e+00f, 5.000000000e-01f, -5.000000000e-01f, -1.000000000e+00f};
+    constexpr float s6[4] = {0.000000000e+00f, 8.660254038e-01f, 8.660254038e-01f, 0.000000000e+00f};
+    constexpr float c8[4] = {1.000000000e+00f, 7.071067812e-01f, 0.000000000e+00f, -7.071067812e-01f};
+    constexpr float s8[4] = {0.000000000e+00f, 7.071067812e-01f, 1.000000000e+00f, 7.071067812e-01f};
+    const float c = N == 6 ? c6[m] : c8[m];
+    const float sn = N == 6 ? s6[m] : s8[m];
+    return make_float2(c, sign * sn);
+}
+
+// DFT of size N = R1 * R2 as R2 DFTs of size R1 (inputs R2 apart), internal
+// twiddles W_N^(n2 k1), then R1 DFTs of size R2 (Cooley-Tukey, outputs in
+// natural order): X[k1 + R1 k2] = sum_n2 W_R2^(n2 k2) W_N^(n2 k1)
+// sum_n1 W_R1^(n1 k1) x[R2 n1 + n2], W_K = exp(sign 2 pi i / K).
+template <int R1, int R2>
+__device__ __forceinline__ void dft_two_level(float2* a, float sign) {
+    constexpr int N = R1 * R2;
+    // Stage 1 in place: the size-R1 transform of inputs a[R2 n1 + n2] (n1 <
+    // R1) returns output k1 to slot R2 k1 + n2.
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+        float2 t[5];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) t[n1] = a[R2 * n1 + n2];
+        dft_small<R1>(t, sign);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1)
+            a[R2 * k1 + n2] = (n2 * k1 == 0) ? t[k1] : cmul(t[k1], unit_root<N>(n2 * k1, sign));
+    }
+    // Stage 2: size-R2 transforms of a[R2 k1 + n2] (n2 < R2), output k2 to
+    // X[k1 + R1 k2]; the outputs are permuted into place afterwards.
+    float2 y[N];
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+        float2 t[5];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) t[n2] = a[R2 * k1 + n2];
+        dft_small<R2>(t, sign);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) y[k1 + R1 * k2] = t[k2];
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) a[i] = y[i];
+}
+
+template <int R>
+__device__ __forceinline__ void dft_small(float2* a, float sign) {
     if constexpr (R == 2) {
         float2 t = a[0];
         a[0] = cadd(t, a[1]);
@@ -57,6 +112,10 @@ __device__ __forceinline__ void dft_small(float2 (&a)[5], float sign) {
         a[4] = csub(m1, iu1);
         a[2] = cadd(m2, iu2);
         a[3] = csub(m2, iu2);
+    } else if constexpr (R == 6) {
+        dft_two_level<2, 3>(a, sign);
+    } else if constexpr (R == 8) {
+        dft_two_level<2, 4>(a, sign);
     }
 }
 

@@ -51,14 +51,22 @@ int fourier_len(int s1, int s2, int N) {
     return N * smooth5_at_least(red);
 }
 
-bool make_radix_plan(int L, RadixPlan& rp) {
+bool make_radix_plan(int L, RadixPlan& rp, bool wide) {
     rp.L = L;
     rp.n = 0;
     int m = L;
-    const int order[4] = {4, 2, 3, 5};
-    for (int k = 0; k < 4; ++k) {
+    // Block-level transforms use radices 2..5; wave-level ones also take the
+    // composite radices 8 and 6 first, one pass (an LDS round trip) fewer at
+    // most lengths used here.  Radices 9 and 10 were tried: their butterflies
+    // raised the register count of every wave-FFT kernel past the occupancy
+    // its LDS allows, with spills.
+    const int narrow[4] = {4, 2, 3, 5};
+    const int wider[6] = {8, 6, 4, 5, 3, 2};
+    const int* order = wide ? wider : narrow;
+    const int nord = wide ? 6 : 4;
+    for (int k = 0; k < nord; ++k) {
         int R = order[k];
-        while (m % R == 0) {   // after the 4s at most one factor 2 remains
+        while (m % R == 0) {
             if (rp.n >= FLF_MAX_RADIX_PASSES) return false;
             rp.r[rp.n++] = R;
             m /= R;

@@ -16,7 +16,7 @@ int smooth5_at_least(int n);
 int fourier_len(int s1, int s2, int N);
 
 // Radix plan (4s first, then 2, 3, 5).  Returns false if L is not 5-smooth.
-bool make_radix_plan(int L, RadixPlan& rp);
+bool make_radix_plan(int L, RadixPlan& rp, bool wide = false);
 
 // Index tables for one Fourier grid (fY, fX).
 struct GridTables {

@@ -31,18 +31,18 @@ __device__ __forceinline__ int div_small(int n, int d, float rd, int& r) {
 }
 
 // a[t] *= w^t for t = 1..R-1, w = tw[k * tstep] (conjugated for the inverse):
-// one table read, the higher powers by products (a few ulp).
+// one table read, the higher powers as running products (at most R - 2
+// roundings, a few ulp), two values live.
 template <int R>
-__device__ __forceinline__ void twiddle_powers(float2 (&a)[5], const float2* __restrict__ tw, int idx, bool inv) {
+__device__ __forceinline__ void twiddle_powers(float2* a, const float2* __restrict__ tw, int idx, bool inv) {
     float2 w1 = tw[idx];
     if (inv) w1.y = -w1.y;
-    float2 w[5];
-    w[1] = w1;
-    if (R > 2) w[2] = cmul(w1, w1);
-    if (R > 3) w[3] = cmul(w[2], w1);
-    if (R > 4) w[4] = cmul(w[2], w[2]);
+    float2 w = w1;
 #pragma unroll
-    for (int t = 1; t < R; ++t) a[t] = cmul(a[t], w[t]);
+    for (int t = 1; t < R; ++t) {
+        a[t] = cmul(a[t], w);
+        if (t + 1 < R) w = cmul(w, w1);
+    }
 }
 
 template <int R, int LMAX>
@@ -60,7 +60,7 @@ __device__ __forceinline__ void wave_pass(float2* buf, int es, int L, int Ns, co
         if (j < M) {
             int k;
             div_small(j, Ns, rNs, k);
-            float2 a[5];
+            float2 a[FLF_RADIX_MAX];
 #pragma unroll
             for (int t = 0; t < R; ++t) a[t] = buf[(j + t * M) * es];
             if (k != 0) twiddle_powers<R>(a, tw, k * tstep, inv);
@@ -91,6 +91,8 @@ __device__ void wave_fft(float2* buf, int es, const RadixPlan& rp, const float2*
     for (int p = 0; p < rp.n; ++p) {
         const int R = rp.r[p];
         switch (R) {
+            case 8: wave_pass<8, LMAX>(buf, es, rp.L, Ns, tw, inv, lane); break;
+            case 6: wave_pass<6, LMAX>(buf, es, rp.L, Ns, tw, inv, lane); break;
             case 4: wave_pass<4, LMAX>(buf, es, rp.L, Ns, tw, inv, lane); break;
             case 2: wave_pass<2, LMAX>(buf, es, rp.L, Ns, tw, inv, lane); break;
             case 3: wave_pass<3, LMAX>(buf, es, rp.L, Ns, tw, inv, lane); break;
@@ -123,7 +125,7 @@ __device__ __forceinline__ void wave_pass_ns(float2* buf, int es, int ss, int ns
             const int g = div_small(jj, Ns, rNs, k);
             ov[q] = sq * ss + (g * Ns * R + k) * es;
             const float2* b = buf + sq * ss;
-            float2 a[5];
+            float2 a[FLF_RADIX_MAX];
 #pragma unroll
             for (int t = 0; t < R; ++t) a[t] = b[(jj + t * M) * es];
             if (k != 0) twiddle_powers<R>(a, tw, k * tstep, inv);
@@ -145,6 +147,8 @@ __device__ __forceinline__ void wave_pass_ns(float2* buf, int es, int ss, int ns
     wave_sync();
 }
 
+// Radices 2..5 only (plans from make_radix_plan without the wide set): the
+// composite radices' code raised the small FFT's registers and slowed it.
 template <int LMAX, int NS>
 __device__ void wave_fft_ns(float2* buf, int es, int ss, int nseq, const RadixPlan& rp, const float2* __restrict__ tw,
                             bool inv, int lane) {
