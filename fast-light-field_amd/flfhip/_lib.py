@@ -75,7 +75,7 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(
-            "%s not found: build the HIP library first (python -c 'import __graft_entry__ as g; g.build()' "
+            "%s not found: build the HIP library first (python -m flfhip.build, "
             "or make -C fast-light-field_amd/csrc)" % LIB_PATH)
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():

@@ -1,6 +1,6 @@
 """Richardson-Lucy on the GPU (lfdeconv.DeconvRL through Projector_hip) against
 the FP64 oracle: Prevedel and standard updates, with and without a trust
-mask, several iterations and frames; north-star bar 1e-4 on the volume."""
+mask, several iterations and frames; 1e-4 relative L2 on the volume."""
 import numpy as np
 import pytest
 
