@@ -272,7 +272,7 @@ def main():
                     help="also run one device copy of this many MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (frames or planes would be "
+        print("bench.py measures one GPU only: this implementation has no multi-GPU path (frames or planes would be "
               "sharded across GPUs with one all-reduce of the forward image per iteration; see DESIGN.md).",
               file=sys.stderr)
         sys.exit(2)

@@ -470,7 +470,7 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
         // (half-length rows) and 128 (reduced grid) points.
         if (gt.fY > 2048 || gt.fX > 2048 || gt.rY > 128 || gt.rX > 128)
             return fail(ctx, FLF_EINVAL, "Fourier grid " + std::to_string(gt.fY) + "x" + std::to_string(gt.fX) +
-                                             " exceeds the transform sizes this build supports");
+                                             " exceeds the transform sizes this library supports");
         if (small_fft_lds(gt.rY, gt.rX) > ctx->max_lds)
             return fail(ctx, FLF_EINVAL, "reduced grid too large for the small-FFT kernel");
         {

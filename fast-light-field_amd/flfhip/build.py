@@ -9,7 +9,7 @@ CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 
 def build(verbose=False, force=True):
     """Compile every HIP source into flfhip/libflf_hip.so.  Always recompiles
-    when force=True (the build check must exercise the compiler)."""
+    when force=True (a build request always runs the compiler)."""
     cmd = ["make", "-C", CSRC, "-j4"]
     if force:
         cmd.append("-B")

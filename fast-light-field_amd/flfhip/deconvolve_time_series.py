@@ -124,7 +124,7 @@ def main(argv, maxIter=8, numParallel=32):
         print("\033[0;31mERROR: Invalid number of RL iterations: %d\033[0m" % args.numIter)
         sys.exit(1)
     if args.platform == "cpu":
-        print("\033[0;33mNOTE: this build deconvolves on the GPU (there is no CPU path)\033[0m")
+        print("\033[0;33mNOTE: deconvolution runs on the GPU (there is no CPU path)\033[0m")
     if args.learned and args.ragged:
         print("\033[0;31mERROR: --learned applies to batches of one image shape\033[0m")
         sys.exit(1)

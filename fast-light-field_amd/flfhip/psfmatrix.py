@@ -86,7 +86,7 @@ def GetPathFormats(matPath):
 
 def LoadMatrix(matPath, numZ=None, zStart=0, **_ignored):
     """psfmatrix.py:169-188 for an existing memmap cache.  Converting a
-    Matlab .mat file needs h5py, which this build does not use."""
+    Matlab .mat file needs h5py, which this package does not use."""
     mmapPath, hfmt, htfmt = GetPathFormats(matPath)
     shape_file = mmapPath + "/HReducedShape.npy"
     if not os.path.exists(shape_file):

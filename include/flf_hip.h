@@ -141,7 +141,7 @@ int flf_backward_rl_update(flf_ctx* ctx, const float* img, float* X, const float
 /* Forward-model data fit per frame, FP64 sums: out[2t] = sum((img-im)^2),
  * out[2t+1] = sum(im^2) (host array of 2*T doubles).  The reference has no
  * counterpart (DeconvRL always runs maxIter, lfdeconv.py:185-196); this is
- * the convergence measure of the build's RL driver. Synchronises `stream`. */
+ * the convergence measure of this library's RL driver. Synchronises `stream`. */
 int flf_datafit(flf_ctx* ctx, const float* img, const float* im, double* out_host, void* stream);
 
 /* X[i] *= Htf[i] / v[i], NaN -> 0, for n elements on `stream`: the Prevedel

@@ -1,7 +1,7 @@
 """Richardson-Lucy deconvolution in FP64 (oracle; test use only).
 
 Restates lfdeconv.DeconvRL (lfdeconv.py:72-202) with the projector passed in
-as two callables, and adds the forward-model data-fit the build uses as its
+as two callables, and adds the forward-model data-fit this library uses as its
 convergence measure (the reference has none; it always runs maxIter).
 """
 import numpy as np
