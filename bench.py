@@ -273,7 +273,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this implementation has no multi-GPU path (frames or planes would be "
-              "sharded across GPUs with one all-reduce of the forward image per iteration; see DESIGN.md).",
+              "sharded across GPUs with one all-reduce of the forward image per iteration).",
               file=sys.stderr)
         sys.exit(2)
     if args.workload in RAGGED:
