@@ -363,8 +363,8 @@ def main():
         out = {}
 
         def learned_step():
-            proj.backward_device(hm, im, out=Htf)
-            Xl, n_it, info = li.deconvolve(im, Htf, 4 * iters, targetFit=fit_plain_t)
+            # Htf = Ht(im) into Htf, beside the network's guess
+            Xl, n_it, info = li.deconvolve(im, None, 4 * iters, targetFit=fit_plain_t, Htf_out=Htf)
             out.update(X=Xl, n=n_it, info=info)
 
         for _ in range(args.warmup):

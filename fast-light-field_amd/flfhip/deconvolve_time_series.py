@@ -77,9 +77,8 @@ def _deconv_uniform(projector, hMatrix, images, maxIter, learned, targetFit):
     T, Y, X = images.shape
     projector.prepare(hMatrix, T, Y, X, frames_capacity=2 * T)
     im = projector.asnative(images)
-    Htf = projector.backward_device(hMatrix, im)
     init = LearnedInit(learned, projector, hMatrix)
-    Xv, n, info = init.deconvolve(im, Htf, maxIter, targetFit=targetFit)
+    Xv, n, info = init.deconvolve(im, None, maxIter, targetFit=targetFit)     # Htf computed beside the guess
     _log("learned start: guess kept for %d of %d frame(s); %d RL iteration(s)"
          % (int(np.sum(info["accepted"])), T, n))
     Xn = projector.asnumpy(Xv)

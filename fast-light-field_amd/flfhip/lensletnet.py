@@ -269,6 +269,7 @@ class LearnedInit(object):
                              % (net.nnum, net.nz, hMatrix.Nnum, hMatrix.numZ))
         self.net, self.projector, self.hMatrix = net, projector, hMatrix
         self._den, self._den_key = None, None
+        self._side = None
 
     def plain_denominator(self, T, Y, X):
         key = (T, Y, X)
@@ -281,17 +282,36 @@ class LearnedInit(object):
             self._den_key = key
         return self._den
 
-    def initial_volume(self, im, Htf):
+    def guess_and_htf(self, im, Htf_out=None):
+        """The network's guess and Htf = Ht(im), computed side by side: the
+        guess (which needs only the image) on a second stream while the back
+        projection runs on the current one.  The network's small GEMMs then
+        fill the back projection's FFT phases instead of following it.  Both
+        results are the same as when computed one after the other."""
+        p, hm = self.projector, self.hMatrix
+        main = torch.cuda.current_stream(p.device)
+        if self._side is None:
+            self._side = torch.cuda.Stream(device=p.device)    # default priority
+        self._side.wait_stream(main)                            # im and earlier work are ready
+        with torch.cuda.stream(self._side):
+            g = self.net.volume_guess(im)
+        Htf = p.backward_device(hm, im, out=Htf_out)
+        main.wait_stream(self._side)
+        g.record_stream(main)                                   # g is used and freed on the current stream
+        return g, Htf
+
+    def initial_volume(self, im, Htf, guess=None):
         """im (T, Y, X), Htf (nz, T, Y, X) on the device.  Returns (X, HX,
         info): the starting volume, its forward projection and per-frame
         guard results (accepted, fit_guess, fit_plain1).  When the projector
         was planned for 2T frames (prepare(..., frames_capacity=2T)), the
         guess and the plain iterate are projected in one call, so every
-        cached spectrum value read serves both."""
+        cached spectrum value read serves both.  guess: the network's guess
+        for im if already computed (guess_and_htf)."""
         p, hm = self.projector, self.hMatrix
         T, Y, Xw = im.shape
         den = self.plain_denominator(T, Y, Xw)
-        g = self.net.volume_guess(im)
+        g = self.net.volume_guess(im) if guess is None else guess
         X1 = torch.empty_like(Htf)
         p.rl_ratio_device(X1, Htf, den)                        # X1 = Htf / Ht(H 1): one iteration from X = 1
         if getattr(p, "_tcap", 0) >= 2 * T:
@@ -311,12 +331,18 @@ class LearnedInit(object):
             Hg[t] = HX1[t]
         return g, Hg, {"accepted": accept, "fit_guess": fit_g, "fit_plain1": fit_1}
 
-    def deconvolve(self, im, Htf, maxIter, targetFit=None, fitOut=None):
+    def deconvolve(self, im, Htf, maxIter, targetFit=None, fitOut=None, Htf_out=None):
         """Guarded learned start, then Prevedel RL until every frame's fit is
         <= targetFit (or maxIter iterations).  Returns (X, iterations, info);
-        a rejected frame's plain iterate counts as one iteration."""
+        a rejected frame's plain iterate counts as one iteration.  Htf None:
+        Htf = Ht(im) is computed here (into Htf_out if given) beside the
+        network's guess (guess_and_htf) and returned as info["Htf"]."""
         from .lfdeconv import deconv_rl_device
-        X, HX, info = self.initial_volume(im, Htf)
+        guess = None
+        if Htf is None:
+            guess, Htf = self.guess_and_htf(im, Htf_out)
+        X, HX, info = self.initial_volume(im, Htf, guess=guess)
+        info["Htf"] = Htf
         fit0 = np.where(info["accepted"], info["fit_guess"], info["fit_plain1"])   # fit of the starting X
         n = deconv_rl_device(self.projector, self.hMatrix, Htf, maxIter, X, im=im, targetFit=targetFit,
                              fitOut=fitOut, HX0=HX, fit0=fit0)

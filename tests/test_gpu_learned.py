@@ -77,3 +77,25 @@ def test_trained_network_saves_iterations(projector):
     assert info["accepted"][0]
     assert n <= 3
     assert float(fits_l[-1][0]) <= fits_p[8]
+
+
+def test_guess_beside_htf_matches_sequential(projector):
+    """Htf computed on the current stream while the network's guess runs on a
+    second one: the same Htf, guess and RL result as one after the other,
+    over repeated calls (the buffers of one call are reused by the next)."""
+    from flfhip.lensletnet import LearnedInit, LensletNet
+    hm, im, Htf = _problem(projector, "rl300", 20, seed=13, peak=1000.0)
+    net = LensletNet(hm.Nnum, hm.numZ, width=32, nblocks=1, seed=2)
+    li = LearnedInit(net, projector, hm)
+    g_seq = net.volume_guess(im)
+    for rep in range(3):
+        out = torch.full_like(Htf, float("nan"))
+        g, H2 = li.guess_and_htf(im, Htf_out=out)
+        torch.cuda.synchronize()
+        print("guess beside Htf, call", rep, flush=True)
+        assert H2.data_ptr() == out.data_ptr()
+        assert torch.equal(H2, Htf) and torch.equal(g, g_seq)
+    Xa, na, ia = li.deconvolve(im, Htf, 3)
+    Xb, nb, ib = li.deconvolve(im, None, 3)
+    torch.cuda.synchronize()
+    assert na == nb and torch.equal(Xa, Xb) and torch.equal(ib["Htf"], Htf)
