@@ -98,8 +98,12 @@ int flf_set_frame_dims(flf_ctx* ctx, const int* dims, int T, void* stream);
  *   "mac_mfma"       0/1: matrix-core MAC for >= 2 frames per workgroup
  *   "bwd_planes"     1..8 planes per back-projection MAC workgroup
  *   "ifft_col_batch" 1..8 columns per column-transform workgroup (after plan)
- * Results do not depend on them beyond FP32 rounding.  flfhip.autotune picks
- * them by timing. */
+ *   "group_streams"  0/1 (default 0): the Fourier-grid groups of a projection
+ *                    run side by side on library-owned streams, forked from
+ *                    and joined back to the caller's stream by events (one
+ *                    stream while the caller's stream is captured)
+ * Results do not depend on them beyond FP32 rounding (group_streams: not at
+ * all).  flfhip.autotune picks them by timing. */
 int flf_set_option(flf_ctx* ctx, const char* name, int value);
 int flf_get_option(const flf_ctx* ctx, const char* name, int* value);
 
