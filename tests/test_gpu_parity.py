@@ -340,3 +340,12 @@ def test_group_streams_bit_identical(projector, side_stream):
     ref = oproj.forward_project(v0, [hm.Hcc(z, False) for z in range(len(sides))])
     for t in range(T):
         assert rel_l2(many[0][t].cpu().numpy(), ref[t]) < 1e-5
+
+
+@pytest.mark.parametrize("N", [7, 19])
+def test_pitch_with_large_prime_factor_rejected(projector, N):
+    """A lenslet pitch whose Fourier grid cannot be 5-smooth is refused at
+    plan time with a clear error, before any kernel runs."""
+    hm = _hm_random(N, [2 * N + 1], seed=N)
+    with pytest.raises(ValueError, match="5-smooth"):       # FLF_EINVAL
+        projector.prepare(hm, 1, 4 * N, 4 * N)
