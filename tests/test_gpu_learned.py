@@ -99,3 +99,42 @@ def test_guess_beside_htf_matches_sequential(projector):
     Xb, nb, ib = li.deconvolve(im, None, 3)
     torch.cuda.synchronize()
     assert na == nb and torch.equal(Xa, Xb) and torch.equal(ib["Htf"], Htf)
+
+
+@pytest.mark.parametrize("packaged", [True, False])
+def test_two_frame_step_matches_single_frames(projector, packaged):
+    """A step of two frames (the guard's pair projection then holds two
+    frames per source) gives each frame the guard decision, data fits and
+    volume of a one-frame step: with the packaged network (guess kept) and
+    with one whose guess is ~0 (plain iterate kept)."""
+    from flfhip import synthetic, workloads
+    from flfhip.lensletnet import LearnedInit, LensletNet, default_weights_path
+    hm, zs, sides = workloads.psf_set("rl1455")
+    N = hm.Nnum
+    Y = X = N * 40
+    if packaged:
+        net = LensletNet.load(default_weights_path(N, hm.numZ, "rl1455"))
+    else:
+        net = LensletNet(N, hm.numZ, width=32, nblocks=1, seed=3)
+        p = net.init_params(3)
+        p[-hm.numZ:] = -30.0
+        net.set_params(p)
+    li = LearnedInit(net, projector, hm)
+    projector.prepare(hm, 2, Y, X, frames_capacity=4)
+    vol = synthetic.blob_volume(hm.numZ, Y, X, 150, seed=31, T=2)
+    clean = projector.forward_device(hm, projector.asnative(vol))
+    im2 = projector.asnative(synthetic.poisson_image(projector.asnumpy(clean), 2000.0, 31))
+    X2, n2, info2 = li.deconvolve(im2, None, 2)
+    X2 = X2.clone()
+    torch.cuda.synchronize()
+    print("two-frame step", packaged, info2["accepted"], info2["fit_guess"], info2["fit_plain1"], flush=True)
+    assert bool(np.all(info2["accepted"])) == packaged
+    for t in range(2):
+        projector.prepare(hm, 1, Y, X, frames_capacity=2)
+        X1, n1, info1 = li.deconvolve(im2[t:t + 1].contiguous(), None, 2)
+        torch.cuda.synchronize()
+        assert bool(info1["accepted"][0]) == bool(info2["accepted"][t])
+        for k in ("fit_guess", "fit_plain1"):
+            assert abs(info1[k][0] - info2[k][t]) <= 1e-6 * info1[k][0], (t, k)
+        assert n1 == n2
+        assert rel_l2(X2[:, t].cpu().numpy(), X1[:, 0].cpu().numpy()) < 1e-5, t
