@@ -245,15 +245,24 @@ int fork_groups(flf_ctx* ctx, hipStream_t st, int nactive, GroupFork& f) {
     HIPCHK(hipStreamIsCapturing(st, &cs));
     if (cs != hipStreamCaptureStatusNone) return FLF_OK;
     const size_t want = (size_t)std::min(nactive, GROUP_STREAMS_MAX);
-    while (ctx->gstreams.size() < want) {
-        hipStream_t s = nullptr;
-        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        ctx->gstreams.push_back(s);
-    }
-    while (ctx->gevents.size() < ctx->gstreams.size() + 1) {
-        hipEvent_t e = nullptr;
-        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        ctx->gevents.push_back(e);
+    if (ctx->gstreams.size() < want || ctx->gevents.size() < want + 1) {
+        // created on the context's device; the caller's current device is kept
+        int cur = -1;
+        HIPCHK(hipGetDevice(&cur));
+        if (cur != ctx->device) HIPCHK(hipSetDevice(ctx->device));
+        hipError_t e = hipSuccess;
+        while (e == hipSuccess && ctx->gstreams.size() < want) {
+            hipStream_t s = nullptr;
+            e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+            if (e == hipSuccess) ctx->gstreams.push_back(s);
+        }
+        while (e == hipSuccess && ctx->gevents.size() < ctx->gstreams.size() + 1) {
+            hipEvent_t ev = nullptr;
+            e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+            if (e == hipSuccess) ctx->gevents.push_back(ev);
+        }
+        if (cur != ctx->device) HIPCHK(hipSetDevice(cur));
+        HIPCHK(e);
     }
     HIPCHK(hipEventRecord(ctx->gevents[0], st));
     for (hipStream_t s : ctx->gstreams) HIPCHK(hipStreamWaitEvent(s, ctx->gevents[0], 0));

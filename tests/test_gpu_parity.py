@@ -144,6 +144,8 @@ def test_autotune_and_options_keep_parity(projector):
         for t in range(T):
             assert rel_l2(fwd[t], ref[t]) < 1e-5, (opts, t)
     for k, v in saved.items():
+        if k == "ifft_col_batch" and v == 0:
+            continue                     # read before any plan: every new plan sets its own
         projector.set_option(k, v)
     with pytest.raises(Exception):
         projector.set_option("no_such_option", 1)
