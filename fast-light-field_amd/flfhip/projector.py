@@ -273,6 +273,13 @@ class Projector_hip(object):
     def ForwardProjectACC(self, hMatrix, realspace, planes, progress=None, logPrint=False, numjobs=1):
         if len(realspace.shape) != 4:
             raise ValueError("realspace must be (numZ, T, Y, X)")
+        if planes is not None:
+            planes = list(planes)            # any iterable of plane indices, read once
+            if not planes:
+                # the reference has no projection to return then (its plane
+                # sum stays None, projector.py:854, 895); forward_device
+                # returns zeros
+                raise ValueError("ForwardProjectACC needs at least one plane")
         if isinstance(realspace, np.ndarray) and realspace.dtype != np.float32:
             raise TypeError("realspace must be float32")
         vol = self.asnative(realspace)

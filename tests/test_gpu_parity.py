@@ -351,3 +351,22 @@ def test_pitch_with_large_prime_factor_rejected(projector, N):
     hm = _hm_random(N, [2 * N + 1], seed=N)
     with pytest.raises(ValueError, match="5-smooth"):       # FLF_EINVAL
         projector.prepare(hm, 1, 4 * N, 4 * N)
+
+
+def test_empty_plane_list(projector):
+    """planes = []: BackwardProjectACC returns an all-zero volume, like the
+    reference (projector.py:807); ForwardProjectACC refuses, where the
+    reference fails on its empty plane sum (projector.py:854, 895); the
+    device-level forward returns the empty sum, a zero image."""
+    import torch
+    hm = _hm_random(15, [31, 45], seed=51)
+    rng = np.random.default_rng(3)
+    img = rng.random((1, 90, 105)).astype(np.float32) + 0.1
+    vol = rng.random((2, 1, 90, 105)).astype(np.float32)
+    back = projector.BackwardProjectACC(hm, img, [])
+    assert back.shape == (2, 1, 90, 105) and not back.any()
+    with pytest.raises(ValueError):
+        projector.ForwardProjectACC(hm, vol, [])
+    out = projector.forward_device(hm, projector.asnative(vol) + 1.0, planes=[])
+    torch.cuda.synchronize()
+    assert out.shape == (1, 90, 105) and float(out.abs().max()) == 0.0
