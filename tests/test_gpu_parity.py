@@ -370,3 +370,39 @@ def test_empty_plane_list(projector):
     out = projector.forward_device(hm, projector.asnative(vol) + 1.0, planes=[])
     torch.cuda.synchronize()
     assert out.shape == (1, 90, 105) and float(out.abs().max()) == 0.0
+
+
+def test_largest_grid_adjoint_and_linearity(projector):
+    """The largest Fourier grid the kernels take (1920 = 15 x 128: a 1575 px
+    image with a 345 px PSF): back projection through Ht = adjoint_quadrant(H)
+    is the adjoint of forward projection, <H x, y> = <x, Ht y>, and forward
+    projection is linear; one lenslet row more needs a 135-point reduced grid
+    and is refused at plan time.  The PSFs are rings (radially symmetric, as
+    physical light-field PSFs are): only then does the adjoint set keep the
+    8-fold symmetry that quadrant storage of Ht assumes (for an arbitrary
+    quadrant it does not, in the reference's storage as here)."""
+    import torch
+    from flfhip import synthetic
+    from flfhip.psfmatrix import MemoryHMatrix, adjoint_quadrant
+    rng = np.random.default_rng(61)
+    Q = synthetic.ring_quadrant(15, 345, 100.0, 1.0, rng).astype(np.float32)
+    hm = MemoryHMatrix([Q], [adjoint_quadrant(Q)])
+    Y = X = 1575
+    projector.prepare(hm, 1, Y, X)
+    assert projector.fshape(0) == (1920, 1920)
+    g = torch.Generator(device=projector.device)
+    g.manual_seed(9)
+    x = torch.rand((1, 1, Y, X), generator=g, device=projector.device)
+    x2 = torch.rand((1, 1, Y, X), generator=g, device=projector.device)
+    y = torch.rand((1, Y, X), generator=g, device=projector.device)
+    Hx = projector.forward_device(hm, x)
+    Hty = projector.backward_device(hm, y)
+    lhs = float((Hx.double() * y.double()).sum())
+    rhs = float((x.double() * Hty.double()).sum())
+    print("largest grid: <Hx,y> %.10e <x,Hty> %.10e" % (lhs, rhs), flush=True)
+    assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
+    lin = projector.forward_device(hm, 2.0 * x + 3.0 * x2)
+    ref = 2.0 * Hx.double() + 3.0 * projector.forward_device(hm, x2).double()
+    assert float(torch.linalg.vector_norm(lin.double() - ref) / torch.linalg.vector_norm(ref)) < 1e-6
+    with pytest.raises(ValueError, match="exceeds"):
+        projector.prepare(hm, 1, Y + 15, X + 15)
