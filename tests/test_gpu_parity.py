@@ -406,3 +406,24 @@ def test_largest_grid_adjoint_and_linearity(projector):
     assert float(torch.linalg.vector_norm(lin.double() - ref) / torch.linalg.vector_norm(ref)) < 1e-6
     with pytest.raises(ValueError, match="exceeds"):
         projector.prepare(hm, 1, Y + 15, X + 15)
+
+
+def test_full_size_plane_vs_oracle(projector):
+    """The 1455 x 1455 / 41-plane workload's largest-PSF plane (z = -20,
+    345 px, Fourier grid 1800): forward and back projection of one plane
+    within 1e-5 relative L2 of the FP64 oracle (about 10 s of CPU)."""
+    import torch
+    from flfhip import workloads
+    hm, zs, sides = workloads.psf_set("rl1455")
+    assert sides[0] == 345
+    Y = X = 1455
+    rng = np.random.default_rng(71)
+    x = rng.random((1, Y, X)).astype(np.float32)
+    vol = np.zeros((hm.numZ, 1, Y, X), np.float32)
+    vol[0] = x
+    fwd = projector.ForwardProjectACC(hm, vol, [0])
+    back = projector.BackwardProjectACC(hm, x, [0])
+    ef = rel_l2(fwd, oproj.project_special(x.astype(np.float64), hm.Hcc(0, False)))
+    eb = rel_l2(back[0], oproj.project_special(x.astype(np.float64), hm.Hcc(0, True)))
+    print("1455^2 plane z=-20: forward rel-L2 %.2e, backward %.2e" % (ef, eb), flush=True)
+    assert ef < 1e-5 and eb < 1e-5
