@@ -427,3 +427,37 @@ def test_full_size_plane_vs_oracle(projector):
     eb = rel_l2(back[0], oproj.project_special(x.astype(np.float64), hm.Hcc(0, True)))
     print("1455^2 plane z=-20: forward rel-L2 %.2e, backward %.2e" % (ef, eb), flush=True)
     assert ef < 1e-5 and eb < 1e-5
+
+
+def test_full_size_rl_two_planes_vs_oracle(projector):
+    """Prevedel RL at 1455 x 1455 restricted to two planes of the 41-plane
+    workload (z = -20, 345 px PSF, grid 1800; z = 0, 15 px PSF, grid 1500):
+    Htf, two iterations from X = 1 and the per-iteration data fit within the
+    bar of the FP64 oracle (volume 1e-4 relative L2, fit 1e-5)."""
+    import torch
+    from flfhip import workloads
+    from flfhip.lfdeconv import deconv_rl_device
+    from oracle import rl as orl
+    hm, zs, sides = workloads.psf_set("rl1455")
+    planes = [0, 20]
+    assert (sides[0], sides[20]) == (345, 15)
+    Y = X = 1455
+    rng = np.random.default_rng(72)
+    im = (rng.random((1, Y, X)) + 0.5).astype(np.float32)
+    imt = projector.asnative(im)
+    Htf = projector.backward_device(hm, imt, planes=planes)
+    Xv = torch.ones((hm.numZ, 1, Y, X), dtype=torch.float32, device=projector.device)
+    fits = []
+    deconv_rl_device(projector, hm, Htf, 2, Xv, im=imt, planes=planes, fitOut=fits)
+    Qf = [hm.Hcc(z, False) for z in planes]
+    Qb = [hm.Hcc(z, True) for z in planes]
+    Xr, rec = orl.deconv_rl(lambda v: oproj.forward_project(v, Qf), lambda i: oproj.backward_project(i, Qb), None, 2,
+                            Xguess=np.ones((2, 1, Y, X)), im=im.astype(np.float64), record_fit=True)
+    Xg = Xv.cpu().numpy()
+    for k, z in enumerate(planes):
+        e = rel_l2(Xg[z, 0], Xr[k, 0])
+        print("1455^2 RL(2) plane %d rel-L2 %.2e" % (z, e), flush=True)
+        assert e < 1e-4
+    got = np.array([float(f[0]) for f in fits[:2]])
+    ref = np.ravel(rec)[:2]
+    assert np.max(np.abs(got - ref) / ref) < 1e-5
