@@ -9,7 +9,9 @@ uses: ``Nnum``, ``numZ``, ``zStart``, ``iterableBRange``, ``HPathFormat``,
 ``Hcc(cc, useHt)`` and ``PSFShape(cc)``.
 
 ``adjoint_quadrant`` derives the back-projection PSF set Ht from H (the role
-of calcHt_new, generate_psf.py:646-719) for any odd PSF side.
+of calcHt_new, generate_psf.py:646-719) for any odd PSF side: by default the
+exact adjoint of the forward projection, or, with ``reference_window=True``,
+the truncated set calcHt_new itself writes.
 """
 import os
 
@@ -136,11 +138,19 @@ def full_psf(Q):
     return out
 
 
-def adjoint_quadrant(Q):
+def adjoint_quadrant(Q, reference_window=False):
     """Quadrant of the back-projection PSF set Ht for the forward set Q:
         Ht_a[e] = H_{(a + e - c2) mod N}[2 c2 - e]        (2-D indices)
-    which makes back-projection the exact adjoint of forward projection
-    (the relation calcHt_new builds, generate_psf.py:646-719)."""
+    which makes back-projection the exact adjoint of forward projection.
+
+    calcHt_new (generate_psf.py:646-719) stamps the same values, but its
+    s2 x s2 cut around the image centre and the following im_shift2 by
+    (h-1-a, h-1-b) pixels (generate_psf.py:711-716) zero the first h-1-b rows
+    and h-1-a columns of Ht[b][a].  reference_window=True reproduces that
+    truncation, so that the result equals the Ht the reference saves (for the
+    PSF sides s2 = 2kN+1 it accepts, generate_psf.py:680); the default keeps
+    those border values, which are zero for a PSF whose outermost h-1 rows
+    and columns are zero."""
     Q = np.asarray(Q)
     h = Q.shape[0]
     nnum = 2 * h - 1
@@ -157,4 +167,7 @@ def adjoint_quadrant(Q):
         for a in range(h):
             sa = (a + e - c2) % nnum
             out[b, a] = H[sb[:, None], sa[None, :], flip[:, None], flip[None, :]]
+            if reference_window:
+                out[b, a, :h - 1 - b, :] = 0
+                out[b, a, :, :h - 1 - a] = 0
     return out

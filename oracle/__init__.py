@@ -11,8 +11,10 @@ each function citing the reference file:line it follows:
 * ``shapes``    - Fourier-grid shape rules (special_fftconvolve.py:71-80,
                   122-174).
 * ``psf``       - quadrant <-> full PSF expansion by the 8-fold symmetry
-                  (py_light_field.cpp:1514-1562, projector.py:159-189) and the
-                  adjoint back-projection PSF set.
+                  (py_light_field.cpp:1514-1562, projector.py:159-189), the
+                  adjoint back-projection PSF set, and a restatement of the
+                  reference's own Ht construction, calcHt_new
+                  (generate_psf.py:646-719).
 * ``projector`` - the direct per-offset convolution projector
                   (projector.py:1084-1126) and a literal restatement of the
                   special-FFT projector (py_light_field.cpp:284-341, 711-1102,
@@ -34,8 +36,17 @@ How each function is pinned (none is "parity unpinned"):
   algorithm: masked full-grid convolution vs lenslet-subsampled Fourier
   tiling with mirrored/transposed spectra) to 1e-12, for square and
   non-square grids, odd and even Fourier sizes.
-* ``rl.deconv_rl``: closed form of one iteration on a 1-plane identity-PSF
-  system, and monotone data-fit on a small synthetic problem.
+* ``psf.calc_ht_stamp``: equals the exact adjoint (``adjoint_psf``) on every
+  pixel it keeps, zero on the h-1-b leading rows and h-1-a leading columns
+  it drops; a planted missing rotation fails.
+* ``rl.deconv_rl`` (Prevedel): closed form of one iteration on a 1-plane
+  identity-PSF system, and monotone data-fit on a small synthetic problem.
+* ``rl.deconv_rl`` (standard, ``prevedel=False``) and
+  ``rl.rl_update_standard`` with a trust mask: identity-PSF closed form
+  (X1 = im, a fixed point), a hand-built N = 3 trust-mask case checked
+  against X0 * B(im / F X0) / B(trust) written with explicit loops, the
+  0/0 -> 0 and x/0 -> inf semantics of lfdeconv.py:156-164, and a planted
+  double trust division that must fail.
 
 The compiled reference (py_light_field + vendored FFTW 3.3.8) is not built
 here: FFTW needs the headers its own configure step generates, which the

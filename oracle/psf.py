@@ -79,3 +79,67 @@ def adjoint_psf(H):
             # Ht[b,a][e0,e1] = H[src_b[e0], src_a[e1]][2c2-e0, 2c2-e1]
             Ht[b, a] = H[src_b[:, None], src_a[None, :], (2 * c2 - e)[:, None], (2 * c2 - e)[None, :]]
     return Ht
+
+
+def mat_layout(Q):
+    """Quadrant in the generator's own layout from the HMatrix layout.
+
+    generate_psf.py:533 builds H[p, q, aa, bb, z] and saves ``H.T`` to HDF5
+    (generate_psf.py:777); h5py reads it back as _H[z, bb, aa, q, p] and
+    LoadRawMatrixData keeps _H[z, :h, :h] (psfmatrix.py:141-146), so
+    Hcc[bb][aa][q][p] == H[p, q, aa, bb].  Returns H[p, q, aa, bb] for one
+    plane."""
+    return np.asarray(Q).transpose(3, 2, 1, 0)
+
+
+def hcc_layout(Hpq):
+    """Inverse of ``mat_layout``."""
+    return np.asarray(Hpq).transpose(3, 2, 1, 0)
+
+
+def calc_ht_stamp(Hpq):
+    """Restatement of calcHt_new with backwardProject_new
+    (generate_psf.py:646-719), FP64, one plane.
+
+    Hpq is one plane in the generator's layout, H[p, q, aa, bb] with
+    aa, bb < h (the full quadrant, generate_psf.py:533).  For each target
+    offset (a0, b0) the reference back-projects a single lit pixel by
+    stamping the 180-degree-rotated PSF of every lenslet offset (aa, bb) at
+    the positions the lenslet sub-sampling would read (generate_psf.py:
+    661-691), cuts an s2 x s2 window around the image centre (711-712) and
+    shifts it by (h-1-a0, h-1-b0) pixels with zero fill (im_shift2,
+    generate_psf.py:457-478; both shifts are >= 0 here).  Valid only for
+    PSF sides s2 = 2kN+1, the reference's own assertion (generate_psf.py:
+    680)."""
+    Hpq = np.asarray(Hpq, dtype=np.float64)
+    s2, _, h, _ = Hpq.shape
+    nnum = 2 * h - 1
+    c2 = s2 // 2
+    if c2 % nnum != 0:
+        raise ValueError("calcHt_new needs s2 = 2kN+1 (generate_psf.py:680)")
+    blocks = -(-s2 // nnum)
+    side = (blocks + 2) * nnum if blocks % 2 == 1 else (blocks + 3) * nnum
+    centre = -(-side // 2)
+    origin = centre - (-(-nnum // 2))
+    rot = Hpq[::-1, ::-1]
+    out = np.zeros_like(Hpq)
+    for a0 in range(h):
+        for b0 in range(h):
+            canvas = np.zeros((side, side))
+            for aa in range(nnum):
+                K = rot[:, :, aa, :] if aa <= nnum // 2 else rot[::-1, :, nnum - 1 - aa, :]
+                for bb in range(nnum):
+                    Kab = K[:, :, bb] if bb <= nnum // 2 else K[:, ::-1, nnum - 1 - bb]
+                    sa = (aa - a0) % nnum
+                    sb = (bb - b0) % nnum
+                    sub = Kab[sa::nnum, sb::nnum]
+                    da = origin + a0 - c2 + sa
+                    db = origin + b0 - c2 + sb
+                    canvas[da:da + sub.shape[0] * nnum:nnum, db:db + sub.shape[1] * nnum:nnum] = sub
+            lo, hi = centre - c2 - 1, centre + c2
+            cut = canvas[lo:hi, lo:hi]
+            sx, sy = h - 1 - a0, h - 1 - b0
+            shifted = np.zeros_like(cut)
+            shifted[sx:, sy:] = cut[:s2 - sx, :s2 - sy]
+            out[:, :, a0, b0] = shifted
+    return out

@@ -182,3 +182,164 @@ def test_rl_reduces_misfit():
     _, fits = orl.deconv_rl(fwd, bwd, None, 6, Xguess=np.ones_like(vol), im=im, record_fit=True)
     fits = [f[0] for f in fits]
     assert fits[-1] < fits[0] * 0.7
+
+
+def test_rl_standard_identity_psf_closed_form():
+    # Standard update (lfdeconv.py:151-164, 191-193) with F = B = identity:
+    # X1 = X0 * B(im / F X0) = X0 * im / X0 = im for any positive X0; two
+    # frames share the one plane.
+    N, s2 = 5, 3
+    h = (N + 1) // 2
+    Q = np.zeros((h, h, s2, s2))
+    Q[:, :, 1, 1] = 1.0
+    rng = np.random.default_rng(12)
+    im = rng.random((2, 10, 10)) + 0.5
+    fwd = lambda v: oproj.forward_project(v, [Q])
+    bwd = lambda i: oproj.backward_project(i, [Q])
+    X0 = rng.random((1, 2, 10, 10)) + 0.1
+    X1 = orl.deconv_rl(fwd, bwd, None, 1, Xguess=X0, im=im, prevedel=False)
+    np.testing.assert_allclose(X1[0], im, rtol=1e-12)
+    # a second iteration is a fixed point
+    X2 = orl.deconv_rl(fwd, bwd, None, 2, Xguess=X0, im=im, prevedel=False)
+    np.testing.assert_allclose(X2[0], im, rtol=1e-12)
+
+
+def _loops_project(src, Kfull):
+    """Per-plane operator out[p] = sum_q src[q] K_{q mod N}[p - q + c2]
+    written with explicit loops (no library convolution)."""
+    Y, X = src.shape
+    nnum, s2 = Kfull.shape[0], Kfull.shape[2]
+    c2 = s2 // 2
+    out = np.zeros((Y, X))
+    for q0 in range(Y):
+        for q1 in range(X):
+            for p0 in range(max(0, q0 - c2), min(Y, q0 + c2 + 1)):
+                for p1 in range(max(0, q1 - c2), min(X, q1 + c2 + 1)):
+                    out[p0, p1] += src[q0, q1] * Kfull[q0 % nnum, q1 % nnum, p0 - q0 + c2, p1 - q1 + c2]
+    return out
+
+
+def _trust_case():
+    # N = 3, one plane, 9 x 9 image, 5 px PSF; a trust mask with a zero
+    # block, and an image that is zero there (the reference asserts this,
+    # lfdeconv.py:106-109).  The block is small enough that B(trust) > 0
+    # everywhere (every pixel has trusted neighbours inside the PSF), so no
+    # 0/0 arises whose FFT rounding would decide the result.
+    N, s2, Y = 3, 5, 9
+    h = (N + 1) // 2
+    rng = np.random.default_rng(31)
+    Q = rng.random((h, h, s2, s2)) + 0.05
+    H = opsf.expand_quadrant(Q)
+    # back projection applies the stored Ht quadrant expanded by the same
+    # symmetry rules (projector.py:806-849)
+    Ht = opsf.expand_quadrant(opsf.quadrant_of(opsf.adjoint_psf(H)))
+    trust = np.ones((Y, Y))
+    trust[1:3, 5:7] = 0.0
+    im = (rng.random((1, Y, Y)) + 0.2) * trust
+    X0 = rng.random((1, 1, Y, Y)) + 0.3
+    return Q, H, Ht, trust, im, X0
+
+
+def _trust_expected(H, Ht, trust, im, X0):
+    Y, X = im.shape[1:]
+    HX = _loops_project(X0[0, 0], H)
+    err = np.zeros((Y, X))
+    for i in range(Y):
+        for j in range(X):
+            err[i, j] = im[0, i, j] / HX[i, j]
+    Berr = _loops_project(err, Ht)
+    Btrust = _loops_project(trust, Ht)
+    X1 = np.zeros((Y, X))
+    for i in range(Y):
+        for j in range(X):
+            v = X0[0, 0, i, j] * Berr[i, j] / Btrust[i, j]
+            X1[i, j] = 0.0 if np.isnan(v) else v
+    return X1
+
+
+def test_rl_standard_trust_mask_hand_built():
+    # deconv_rl(prevedel=False, trust_mask=...) against X0 * B(im / F X0) /
+    # B(trust) with every operator and elementwise step as explicit loops
+    # (lfdeconv.py:106-110, 151-164, 191-193).
+    Q, H, Ht, trust, im, X0 = _trust_case()
+    Qt = opsf.quadrant_of(Ht)
+    fwd = lambda v: oproj.forward_project(v, [Q])
+    bwd = lambda i: oproj.backward_project(i, [Qt])
+    got = orl.deconv_rl(fwd, bwd, None, 1, Xguess=X0, im=im, prevedel=False, trust_mask=trust)
+    want = _trust_expected(H, Ht, trust, im, X0)
+    assert np.max(np.abs(got[0, 0] - want)) < 1e-12 * np.max(np.abs(want))
+    assert np.all(want[1:3, 5:7] > 0)      # the untrusted pixels are still updated
+
+
+def test_rl_standard_planted_double_trust_fails():
+    # Planting a plausible mistake in the oracle (the trust normalisation
+    # applied twice) must break the hand-built check above.
+    Q, H, Ht, trust, im, X0 = _trust_case()
+    Qt = opsf.quadrant_of(Ht)
+    fwd = lambda v: oproj.forward_project(v, [Q])
+    bwd = lambda i: oproj.backward_project(i, [Qt])
+    orig = orl.rl_update_standard
+
+    def twice(X, back, trust_back=None):
+        X = orig(X, back, trust_back)
+        return X / trust_back if trust_back is not None else X
+
+    orl.rl_update_standard = twice
+    try:
+        bad = orl.deconv_rl(fwd, bwd, None, 1, Xguess=X0, im=im, prevedel=False, trust_mask=trust)
+    finally:
+        orl.rl_update_standard = orig
+    want = _trust_expected(H, Ht, trust, im, X0)
+    assert np.max(np.abs(bad[0, 0] - want)) > 1e-3 * np.max(np.abs(want))
+
+
+def test_rl_standard_division_semantics():
+    # lfdeconv.py:156-164: X *= back; X /= trustBack; NaN -> 0.  So
+    # 0 * b / 0 = NaN -> 0, x / 0 = inf (kept), and a NaN in back is zeroed.
+    X = np.array([0.0, 1.0, 2.0, 1.0, 3.0])
+    back = np.array([1.0, 1.0, 0.0, 2.0, np.nan])
+    tb = np.array([0.0, 0.0, 1.0, 4.0, 1.0])
+    out = orl.rl_update_standard(X.copy(), back, tb)
+    assert out[0] == 0.0 and np.isposinf(out[1]) and out[2] == 0.0 and out[3] == 0.5 and out[4] == 0.0
+    out2 = orl.rl_update_standard(X.copy(), back)
+    np.testing.assert_array_equal(out2, [0.0, 1.0, 0.0, 2.0, 0.0])
+
+
+@pytest.mark.parametrize("N,s2", [(3, 7), (5, 11), (3, 13), (7, 29)])
+def test_calc_ht_stamp_matches_adjoint_quadrant(N, s2):
+    # calcHt_new (generate_psf.py:646-719) restated in the generator's own
+    # layout, converted to the HMatrix layout, equals the product's
+    # adjoint_quadrant with the reference's window truncation exactly; and
+    # without the truncation the two agree wherever the reference keeps a
+    # value.  The quadrant keeps the transpose symmetry the projector
+    # assumes (its lower triangle is never read, py_light_field.cpp:1514-1521).
+    from flfhip.psfmatrix import adjoint_quadrant
+    rng = np.random.default_rng(N * 100 + s2)
+    h = (N + 1) // 2
+    Q = opsf.quadrant_of(opsf.expand_quadrant(rng.random((h, h, s2, s2)) + 0.1))
+    ref = opsf.hcc_layout(opsf.calc_ht_stamp(opsf.mat_layout(Q)))
+    np.testing.assert_array_equal(adjoint_quadrant(Q, reference_window=True), ref)
+    exact = adjoint_quadrant(Q)
+    for b in range(h):
+        for a in range(h):
+            kept = ref[b, a] != 0
+            np.testing.assert_array_equal(exact[b, a][kept], ref[b, a][kept])
+            assert kept[h - 1 - b:, h - 1 - a:].all()
+
+
+def test_calc_ht_stamp_planted_flip_fails():
+    # Dropping the 180-degree rotation of the stamped PSF (a plausible
+    # mistake) must break agreement with the adjoint.
+    from flfhip.psfmatrix import adjoint_quadrant
+    rng = np.random.default_rng(4)
+    N, s2 = 5, 11
+    h = (N + 1) // 2
+    Q = opsf.quadrant_of(opsf.expand_quadrant(rng.random((h, h, s2, s2)) + 0.1))
+    Hpq = opsf.mat_layout(Q)
+    bad = opsf.hcc_layout(opsf.calc_ht_stamp(Hpq[::-1, ::-1].copy()))
+    assert np.max(np.abs(bad - adjoint_quadrant(Q, reference_window=True))) > 1e-2
+
+
+def test_calc_ht_stamp_rejects_unsupported_side():
+    with pytest.raises(ValueError):
+        opsf.calc_ht_stamp(np.zeros((9, 9, 3, 3)))
