@@ -359,7 +359,7 @@ def main():
     if not args.plain_only and os.path.exists(wpath):
         net = lensletnet.LensletNet.load(wpath)
         li = lensletnet.LearnedInit(net, proj, hm)
-        li.plain_denominator(T, Y, X)          # cached per PSF set and frame shape, like the spectra
+        li.plain_denominator(Y, X)             # cached per PSF set and frame shape, like the spectra
         out = {}
 
         def learned_step():

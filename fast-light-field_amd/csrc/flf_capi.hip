@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/flf_hip.h"
+#include "flf_devptr.h"
 #include "flf_kernels.h"
 #include "flf_plan.h"
 
@@ -31,8 +32,7 @@ struct Group {
     int* d_au = nullptr;
     int* d_uoff = nullptr;
     int* d_s2 = nullptr;
-    unsigned char* d_enable = nullptr;
-    std::vector<unsigned char> enable_host;
+    std::vector<unsigned char> enable_host;   // planes of the current projection (kernel-argument mask)
     float2* d_G[2] = {nullptr, nullptr};
     long long* d_srcoff_fwd = nullptr;   // active frame count's tables (point into the *_T lists)
     long long* d_Soff_fwd = nullptr;
@@ -69,8 +69,6 @@ struct flf_ctx {
     bool ragged = false;
     float2* d_T_bwd = nullptr;
     long long* d_zeros = nullptr;     // zero offsets (length max planes per group)
-    int* d_plane_ids = nullptr;
-    std::vector<int> plane_ids_host;
     double* d_fitpart = nullptr;
     double* d_fitout = nullptr;
     size_t max_lds = 65536;
@@ -156,7 +154,6 @@ void free_all(flf_ctx* ctx) {
     ctx->ragged = false;
     ctx->d_T_bwd = nullptr;
     ctx->d_zeros = nullptr;
-    ctx->d_plane_ids = nullptr;
     ctx->d_fitpart = nullptr;
     ctx->d_fitout = nullptr;
 }
@@ -202,8 +199,10 @@ int resolve_planes(flf_ctx* ctx, const int* planes, int nplanes, std::vector<cha
     return FLF_OK;
 }
 
-// Upload the enable mask of a group if it changed (synchronous copy of a few bytes).
-int set_enable(flf_ctx* ctx, Group& g, const std::vector<char>& sel, int dir, bool& any, hipStream_t st) {
+// Planes of group g selected for this projection (sel: per global plane);
+// `any` tells whether the group has work.  The selection reaches the kernels
+// as a by-value mask (plane_mask), never through device memory.
+int set_enable(flf_ctx* ctx, Group& g, const std::vector<char>& sel, int dir, bool& any) {
     std::vector<unsigned char> en(g.planes.size());
     any = false;
     for (size_t p = 0; p < g.planes.size(); ++p) {
@@ -215,12 +214,27 @@ int set_enable(flf_ctx* ctx, Group& g, const std::vector<char>& sel, int dir, bo
             any = true;
         }
     }
-    if (en != g.enable_host) {
-        g.enable_host = en;
-        HIPCHK(hipMemcpyAsync(g.d_enable, g.enable_host.data(), en.size(), hipMemcpyHostToDevice, st));
-    }
+    g.enable_host = en;
     return FLF_OK;
 }
+
+PlaneMask plane_mask(const Group& g) {
+    PlaneMask m{};
+    for (size_t p = 0; p < g.enable_host.size(); ++p)
+        if (g.enable_host[p]) m.w[p >> 5] |= 1u << (p & 31);
+    return m;
+}
+
+// Device memory checks of the caller's arrays (flf_devptr.h), before any
+// launch.  Sizes in float elements.
+int need_dev(flf_ctx* ctx, const void* p, long long nfloat, const char* what) {
+    std::string why;
+    if (!device_span(ctx->device, p, (size_t)nfloat * sizeof(float), what, why)) return fail(ctx, FLF_EINVAL, why);
+    return FLF_OK;
+}
+
+long long volume_floats(const flf_ctx* ctx, int frames) { return (long long)ctx->nz * frames * ctx->Y * ctx->X; }
+long long image_floats(const flf_ctx* ctx, int frames) { return (long long)frames * ctx->Y * ctx->X; }
 
 // Streams for the Fourier-grid groups of one projection.  Every group reads
 // and writes buffers of its own (small-FFT output, accumulators, its column
@@ -378,7 +392,7 @@ int run_backward_groups(flf_ctx* ctx, const float* img, const std::vector<char>&
         m.tiles = g.d_tiles;
         m.S = g.d_S_bwd; m.S_off = ctx->d_zeros;
         m.S_tstride = s_frame_stride(ctx, g);
-        m.s2 = g.d_s2; m.enable = g.d_enable;
+        m.s2 = g.d_s2; m.enable = plane_mask(g);
         m.nP = (int)g.planes.size();
         m.A = g.d_A_bwd;
         m.A_tstride = (long long)gt.fY * gt.rfX;
@@ -412,15 +426,20 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
     if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");
     if (!img || !vol) return fail(ctx, FLF_EINVAL, "null image or volume pointer");
     if (mode == 1 && !Htf) return fail(ctx, FLF_EINVAL, "Prevedel update needs Htf");
+    int rc;
+    if ((rc = need_dev(ctx, img, image_floats(ctx, ctx->T), "image"))) return rc;
+    if ((rc = need_dev(ctx, vol, volume_floats(ctx, ctx->T), "volume"))) return rc;
+    if (mode == 1 && (rc = need_dev(ctx, Htf, volume_floats(ctx, ctx->T), "Htf"))) return rc;
+    if (mode == 2 && trust && (rc = need_dev(ctx, trust, volume_floats(ctx, 1), "trust"))) return rc;
     std::vector<char> sel;
-    int rc = resolve_planes(ctx, planes, nplanes, sel);
+    rc = resolve_planes(ctx, planes, nplanes, sel);
     if (rc) return rc;
     // enable flags of every group on st, then the groups side by side
     std::vector<char> active(ctx->groups.size(), 0);
     int nactive = 0;
     for (size_t gi = 0; gi < ctx->groups.size(); ++gi) {
         bool any = false;
-        rc = set_enable(ctx, ctx->groups[gi], sel, 1, any, st);
+        rc = set_enable(ctx, ctx->groups[gi], sel, 1, any);
         if (rc) return rc;
         active[gi] = any ? 1 : 0;
         nactive += any ? 1 : 0;
@@ -441,14 +460,9 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
             wbuf = std::max(wbuf, c2r_wave_buffer(ctx->groups[ctx->group_of[z]].gt.fX));
         }
     if (ids.empty()) return FLF_OK;
-    if (ids != ctx->plane_ids_host) {
-        ctx->plane_ids_host = ids;
-        HIPCHK(hipMemcpyAsync(ctx->d_plane_ids, ctx->plane_ids_host.data(), ids.size() * sizeof(int),
-                              hipMemcpyHostToDevice, st));
-    }
     IfftRowPlaneArgs r{};
     r.pd = ctx->d_rowdesc;
-    r.plane_ids = ctx->d_plane_ids;
+    for (size_t i = 0; i < ids.size(); ++i) r.plane_ids[i] = (unsigned short)ids[i];
     r.Tbuf = ctx->d_T_bwd;
     r.vol = vol; r.Htf = Htf; r.trust = trust;
     r.mode = mode; r.T = ctx->T; r.Y = ctx->Y; r.X = ctx->X;
@@ -506,7 +520,7 @@ void flf_destroy(flf_ctx* ctx) {
     delete ctx;
 }
 
-const char* flf_last_error(const flf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* flf_last_error(const flf_ctx* ctx) { return ctx ? ctx->err.c_str() : tls_error().c_str(); }
 
 int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2) {
     if (!ctx) return FLF_EINVAL;
@@ -515,6 +529,8 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
     ctx->err.clear();
     if (nnum < 3 || nnum % 2 == 0 || nnum > FLF_MAX_N) return fail(ctx, FLF_EINVAL, "Nnum must be odd, 3..31");
     if (Y < 1 || X < 1 || T < 1 || nz < 1 || !s2) return fail(ctx, FLF_EINVAL, "bad geometry");
+    if (nz > FLF_MAX_PLANES) return fail(ctx, FLF_EINVAL, "more than " + std::to_string(FLF_MAX_PLANES) + " planes");
+    if ((long long)nz * T > 65535) return fail(ctx, FLF_EINVAL, "planes x frames above 65535");
     for (int z = 0; z < nz; ++z)
         if (s2[z] < 1 || s2[z] % 2 == 0) return fail(ctx, FLF_EINVAL, "PSF side must be odd and positive");
     ctx->N = nnum; ctx->Y = Y; ctx->X = X; ctx->T = T; ctx->Tcap = T; ctx->nz = nz;
@@ -613,7 +629,6 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
         g.d_srcoff_fwd = g.srcoff_T[T - 1];
         g.d_Soff_fwd = g.soff_T[T - 1];
         g.enable_host.assign(nP, 1);
-        if ((rc = upload(ctx, &g.d_enable, g.enable_host))) return rc;
         const long long items = (long long)gt.items.size();
         for (int d = 0; d < 2; ++d) {
             if ((rc = dalloc(ctx, &g.d_G[d], (size_t)items * gt.slots * gt.nu * nP))) return rc;
@@ -657,8 +672,6 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
     }
     ctx->d_rowdesc = ctx->rowdesc_T[T - 1];
     if ((rc = upload(ctx, &ctx->d_zeros, std::vector<long long>(maxP, 0)))) return rc;
-    if ((rc = dalloc(ctx, &ctx->d_plane_ids, (size_t)nz))) return rc;
-    ctx->plane_ids_host.clear();
     if ((rc = dalloc(ctx, &ctx->d_fitpart, (size_t)2 * FIT_BLOCKS * T))) return rc;
     if ((rc = dalloc(ctx, &ctx->d_fitout, (size_t)2 * T))) return rc;
     if ((rc = dalloc(ctx, &ctx->d_fdims, (size_t)T))) return rc;
@@ -819,34 +832,52 @@ int flf_set_psf(flf_ctx* ctx, int z, int dir, const float* quadrant_host) {
 
 namespace {
 // Forward projection of the active frames; with vol2, frames t >= T/2 come
-// from vol2 (frame t - T/2) and both sources hold T/2 frames.
-int run_forward(flf_ctx* ctx, const float* vol, const float* vol2, float* img, const int* planes, int nplanes,
-                hipStream_t st);
+// from vol2 (frame t - T/2) and both sources hold T/2 frames.  img may be
+// null when ratio is set (ratio = im / image).
+int run_forward(flf_ctx* ctx, const float* vol, const float* vol2, float* img, const float* im, float* ratio,
+                const int* planes, int nplanes, hipStream_t st);
 }  // namespace
 
 int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, int nplanes, void* stream) {
     if (!ctx) return FLF_EINVAL;
     if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");
-    if (!vol || !img) return fail(ctx, FLF_EINVAL, "null volume or image pointer");
-    return run_forward(ctx, vol, nullptr, img, planes, nplanes, (hipStream_t)stream);
+    int rc;
+    if ((rc = need_dev(ctx, vol, volume_floats(ctx, ctx->T), "volume"))) return rc;
+    if ((rc = need_dev(ctx, img, image_floats(ctx, ctx->T), "image"))) return rc;
+    return run_forward(ctx, vol, nullptr, img, nullptr, nullptr, planes, nplanes, (hipStream_t)stream);
+}
+
+int flf_forward_ratio(flf_ctx* ctx, const float* vol, const float* im, float* img, float* ratio, const int* planes,
+                      int nplanes, void* stream) {
+    if (!ctx) return FLF_EINVAL;
+    if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");
+    int rc;
+    if ((rc = need_dev(ctx, vol, volume_floats(ctx, ctx->T), "volume"))) return rc;
+    if ((rc = need_dev(ctx, im, image_floats(ctx, ctx->T), "im"))) return rc;
+    if ((rc = need_dev(ctx, ratio, image_floats(ctx, ctx->T), "ratio"))) return rc;
+    if (img && (rc = need_dev(ctx, img, image_floats(ctx, ctx->T), "image"))) return rc;
+    return run_forward(ctx, vol, nullptr, img, im, ratio, planes, nplanes, (hipStream_t)stream);
 }
 
 int flf_forward_pair(flf_ctx* ctx, const float* volA, const float* volB, float* img, const int* planes, int nplanes,
                      void* stream) {
     if (!ctx) return FLF_EINVAL;
     if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");
-    if (!volA || !volB || !img) return fail(ctx, FLF_EINVAL, "null volume or image pointer");
     if (ctx->T % 2) return fail(ctx, FLF_EINVAL, "flf_forward_pair needs an even active frame count");
     if (ctx->ragged) return fail(ctx, FLF_EINVAL, "flf_forward_pair does not take per-frame sizes");
-    return run_forward(ctx, volA, volB, img, planes, nplanes, (hipStream_t)stream);
+    int rc;
+    if ((rc = need_dev(ctx, volA, volume_floats(ctx, ctx->T / 2), "volA"))) return rc;
+    if ((rc = need_dev(ctx, volB, volume_floats(ctx, ctx->T / 2), "volB"))) return rc;
+    if ((rc = need_dev(ctx, img, image_floats(ctx, ctx->T), "image"))) return rc;
+    return run_forward(ctx, volA, volB, img, nullptr, nullptr, planes, nplanes, (hipStream_t)stream);
 }
 
 namespace {
 int run_forward_groups(flf_ctx* ctx, const float* vol, const float* vol2, const IfftRowSumArgs& rs,
                        const GroupFork& fk);
 
-int run_forward(flf_ctx* ctx, const float* vol, const float* vol2, float* img, const int* planes, int nplanes,
-                hipStream_t st) {
+int run_forward(flf_ctx* ctx, const float* vol, const float* vol2, float* img, const float* im, float* ratio,
+                const int* planes, int nplanes, hipStream_t st) {
     std::vector<char> sel;
     int rc = resolve_planes(ctx, planes, nplanes, sel);
     if (rc) return rc;
@@ -856,6 +887,8 @@ int run_forward(flf_ctx* ctx, const float* vol, const float* vol2, float* img, c
     rs.X = ctx->X;
     rs.Y = ctx->Y;
     rs.img = img;
+    rs.im = im;
+    rs.ratio = ratio;
     rs.img_tstride = (long long)ctx->Y * ctx->X;
     rs.fdims = frame_dims(ctx);
     rs.maxfX = 1;
@@ -865,7 +898,7 @@ int run_forward(flf_ctx* ctx, const float* vol, const float* vol2, float* img, c
         Group& g = ctx->groups[gi];
         const GridTables& gt = g.gt;
         bool any = false;
-        rc = set_enable(ctx, g, sel, 0, any, st);
+        rc = set_enable(ctx, g, sel, 0, any);
         if (rc) return rc;
         nactive += any ? 1 : 0;
         rs.grp[gi].active = any ? 1 : 0;
@@ -927,7 +960,7 @@ int run_forward_groups(flf_ctx* ctx, const float* vol, const float* vol2, const 
         m.chunks = g.d_chunks; m.items = g.d_items; m.u_bu = g.d_bu; m.u_au = g.d_au; m.uoff = g.d_uoff;
         m.tiles = g.d_tiles;
         m.S = g.d_S_fwd; m.S_off = g.d_Soff_fwd; m.S_tstride = a.S_tstride;
-        m.s2 = g.d_s2; m.enable = g.d_enable;
+        m.s2 = g.d_s2; m.enable = plane_mask(g);
         m.nP = nP;
         m.A = g.d_A_fwd;
         m.A_tstride = (long long)gt.fY * gt.rfX;
@@ -969,26 +1002,70 @@ int flf_backward_rl_update(flf_ctx* ctx, const float* img, float* X, const float
     return run_backward(ctx, img, X, Htf, trust, mode, planes, nplanes, (hipStream_t)stream);
 }
 
-int flf_rl_scale(const float* Htf, const float* v, float* X, long long n, void* stream) {
-    if (!Htf || !v || !X || n < 0) return FLF_EINVAL;
-    if (n == 0) return FLF_OK;
-    return flf::launch_rl_scale(X, Htf, v, n, (hipStream_t)stream) == hipSuccess ? FLF_OK : FLF_EHIP;
+namespace {
+// Shared checks of flf_rl_scale / flf_rl_ratio: every array on the current
+// device and as large as the slabs it covers.
+int rl_scale_call(const float* Htf, const float* v, float* X, int nz, int T, long long npix, int v_frames,
+                  void* stream, bool ratio) {
+    tls_error().clear();
+    if (nz < 1 || T < 1 || npix < 0 || (v_frames != 1 && v_frames != T) || (long long)nz * T > 65535) {
+        tls_error() = "flf_rl_scale: bad nz, T, npix or v_frames";
+        return FLF_EINVAL;
+    }
+    if (npix == 0) return FLF_OK;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return FLF_EHIP;
+    const size_t nx = (size_t)nz * T * npix * sizeof(float), nv = (size_t)nz * v_frames * npix * sizeof(float);
+    std::string why;
+    if (!device_span(dev, Htf, nx, "Htf", why) || !device_span(dev, v, nv, "v", why) ||
+        !device_span(dev, X, nx, "X", why)) {
+        tls_error() = why;
+        return FLF_EINVAL;
+    }
+    return flf::launch_rl_scale(X, Htf, v, nz, T, npix, v_frames, (hipStream_t)stream, ratio) == hipSuccess
+               ? FLF_OK
+               : FLF_EHIP;
+}
+}  // namespace
+
+int flf_rl_scale(const float* Htf, const float* v, float* X, int nz, int T, long long npix, int v_frames,
+                 void* stream) {
+    return rl_scale_call(Htf, v, X, nz, T, npix, v_frames, stream, false);
 }
 
-int flf_rl_ratio(const float* Htf, const float* v, float* X, long long n, void* stream) {
-    if (!Htf || !v || !X || n < 0) return FLF_EINVAL;
-    if (n == 0) return FLF_OK;
-    return flf::launch_rl_scale(X, Htf, v, n, (hipStream_t)stream, true) == hipSuccess ? FLF_OK : FLF_EHIP;
+int flf_rl_ratio(const float* Htf, const float* v, float* X, int nz, int T, long long npix, int v_frames,
+                 void* stream) {
+    return rl_scale_call(Htf, v, X, nz, T, npix, v_frames, stream, true);
 }
 
-int flf_datafit(flf_ctx* ctx, const float* img, const float* im, double* out_host, void* stream) {
+int flf_datafit(flf_ctx* ctx, const float* img, const float* im, int im_frames, double* out_host, void* stream) {
     if (!ctx) return FLF_EINVAL;
     if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");
-    if (!img || !im || !out_host) return fail(ctx, FLF_EINVAL, "null pointer");
+    if (!out_host) return fail(ctx, FLF_EINVAL, "null output pointer");
+    if (im_frames == 0) im_frames = ctx->T;
+    if (im_frames < 1 || im_frames > ctx->T) return fail(ctx, FLF_EINVAL, "im_frames outside 1..active frames");
+    int rc;
+    if ((rc = need_dev(ctx, img, image_floats(ctx, ctx->T), "image"))) return rc;
+    if ((rc = need_dev(ctx, im, image_floats(ctx, im_frames), "im"))) return rc;
     hipStream_t st = (hipStream_t)stream;
-    HIPCHK(launch_fit(img, im, (long long)ctx->Y * ctx->X, ctx->T, ctx->d_fitpart, FIT_BLOCKS, ctx->d_fitout, st));
+    HIPCHK(launch_fit(img, im, (long long)ctx->Y * ctx->X, ctx->T, im_frames, ctx->d_fitpart, FIT_BLOCKS,
+                      ctx->d_fitout, st));
     HIPCHK(hipMemcpyAsync(out_host, ctx->d_fitout, sizeof(double) * 2 * ctx->T, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    return FLF_OK;
+}
+
+int flf_check_device_buffer(int device, const void* p, long long bytes) {
+    tls_error().clear();
+    if (bytes < 0) {
+        tls_error() = "negative size";
+        return FLF_EINVAL;
+    }
+    std::string why;
+    if (!device_span(device, p, (size_t)bytes, "buffer", why)) {
+        tls_error() = why;
+        return FLF_EINVAL;
+    }
     return FLF_OK;
 }
 

@@ -3,6 +3,15 @@
 #include "flf_common.h"
 
 #define FLF_MAX_GROUPS 8
+#define FLF_MAX_PLANES 256       // planes per plan: the plane selection travels in kernel arguments
+
+// Planes of a grid group selected for one projection, as a bitmask carried
+// by value in the kernel arguments (a captured graph replays its own
+// selection, whatever later calls select).
+struct PlaneMask {
+    unsigned int w[FLF_MAX_PLANES / 32];
+    __host__ __device__ bool on(int p) const { return (w[p >> 5] >> (p & 31)) & 1u; }
+};
 
 struct SmallFFTArgs {
     const float* src;            // source base
@@ -62,7 +71,7 @@ struct MacArgs {
     const long long* S_off;      // per plane in group
     long long S_tstride;
     const int* s2;               // per plane in group
-    const unsigned char* enable; // per plane in group
+    PlaneMask enable;            // planes of the group in this projection
     int nP;
     float2* A;
     long long A_tstride, A_pstride;
@@ -104,9 +113,11 @@ struct IfftRowSumArgs {
     int ngroups;
     int maxfX;
     int X, Y;
-    float* img;
+    float* img;                  // forward image, or null (not stored)
     long long img_tstride;
     const int2* fdims;           // per frame valid size; outside it the image is 0
+    const float* im;             // with ratio: the measured image (T, Y, X)
+    float* ratio;                // if set: ratio = im / image (standard RL, lfdeconv.py:191), 0 outside
 };
 
 struct RowPlaneDesc {
@@ -122,7 +133,7 @@ struct RowPlaneDesc {
 
 struct IfftRowPlaneArgs {
     const RowPlaneDesc* pd;      // per plane z
-    const int* plane_ids;        // planes to process (grid z)
+    unsigned short plane_ids[FLF_MAX_PLANES];   // planes to process (grid z), by value
     const float2* Tbuf;
     float* vol;                  // (nz, T, Y, X)
     const float* Htf;            // mode 1
@@ -150,11 +161,16 @@ hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks,
 hipError_t launch_ifft_cols(const IfftColArgs& a, int nbatch, hipStream_t s);
 hipError_t launch_ifft_rows_sum(const IfftRowSumArgs& a, int Y, int T, hipStream_t s);
 hipError_t launch_ifft_rows_plane(const IfftRowPlaneArgs& a, int nplanes, hipStream_t s);
-hipError_t launch_fit(const float* img, const float* im, long long n_per_frame, int T, double* part, int nblk,
-                      double* out, hipStream_t s);
+// Data-fit sums: frame t of img against frame t % im_frames of im; part holds
+// 2 * nblk * T doubles of partial sums, out 2 * T.
+hipError_t launch_fit(const float* img, const float* im, long long n_per_frame, int T, int im_frames, double* part,
+                      int nblk, double* out, hipStream_t s);
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t s);
-hipError_t launch_rl_scale(float* X, const float* Htf, const float* v, long long n, hipStream_t s,
-                           bool ratio = false);
+// X = [X *] Htf / v over nslab (plane, frame) slabs of npix values; v holds
+// one slab per plane (v_frames = 1, shared by the T frames) or per slab
+// (v_frames = T).
+hipError_t launch_rl_scale(float* X, const float* Htf, const float* v, int nz, int T, long long npix, int v_frames,
+                           hipStream_t s, bool ratio = false);
 // Raise the dynamic-LDS limit of the FFT kernels to max_lds bytes.
 hipError_t configure_kernels(size_t max_lds);
 }  // namespace flf

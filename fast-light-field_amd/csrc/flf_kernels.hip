@@ -617,7 +617,7 @@ void k_mac(MacArgs a) {
     // across a plane boundary, i.e. before the next plane's set-up) are in
     // flight while the current batch is accumulated.
     int p = pb;
-    while (p < pe && !a.enable[p]) ++p;
+    while (p < pe && !a.enable.on(p)) ++p;
     if (p < pe) {
         float2 gA[UNR][IPT][4], gB[UNR][IPT][4];
         load_batch(gA, p, 0);
@@ -636,7 +636,7 @@ void k_mac(MacArgs a) {
             if (bn == nb) {
                 bn = 0;
                 pn = p + 1;
-                while (pn < pe && !a.enable[pn]) ++pn;
+                while (pn < pe && !a.enable.on(pn)) ++pn;
             }
             if (pn < pe) load_batch(nxt, pn, bn);
 #pragma unroll
@@ -753,7 +753,7 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void k_mac_mfma(MacArgs a) {
     const int pb = (int)(((long long)blockIdx.y * a.nP) / gridDim.y);
     const int pe = (int)(((long long)(blockIdx.y + 1) * a.nP) / gridDim.y);
     for (int p = pb; p < pe; ++p) {
-        if (!a.enable[p]) continue;
+        if (!a.enable.on(p)) continue;
         mac_tables<SLOTS>(a, ch, cidx, p, t0, nf, pm_tab, s_stage, psi, rs);
         v4f acc[MAXT][NT];
 #pragma unroll
@@ -1030,11 +1030,22 @@ __global__ __launch_bounds__(256) void k_ifft_rows_sum(IfftRowSumArgs a) {
     }
     if (y < a.Y) {
         const int Yv = a.fdims ? a.fdims[t].x : a.Y, Xv = a.fdims ? a.fdims[t].y : a.X;
-        float* out = a.img + (long long)t * a.img_tstride + (long long)y * a.X;
+        const long long ro = (long long)t * a.img_tstride + (long long)y * a.X;
+        if (a.img) {
 #pragma unroll
-        for (int q = 0; q < XPL; ++q) {
-            const int x = lane + 64 * q;
-            if (x < a.X) out[x] = (y < Yv && x < Xv) ? acc[q] : 0.f;
+            for (int q = 0; q < XPL; ++q) {
+                const int x = lane + 64 * q;
+                if (x < a.X) a.img[ro + x] = (y < Yv && x < Xv) ? acc[q] : 0.f;
+            }
+        }
+        if (a.ratio) {
+            // standard RL's error image im / HX (lfdeconv.py:191) with IEEE
+            // division: 0/0 gives NaN and x/0 inf, as in numpy
+#pragma unroll
+            for (int q = 0; q < XPL; ++q) {
+                const int x = lane + 64 * q;
+                if (x < a.X) a.ratio[ro + x] = (y < Yv && x < Xv) ? a.im[ro + x] / acc[q] : 0.f;
+            }
         }
     }
 }
@@ -1049,7 +1060,7 @@ __global__ __launch_bounds__(256) void k_ifft_rows_plane(IfftRowPlaneArgs a) {
     extern __shared__ float2 lds[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int y = blockIdx.x * 4 + wv, t = blockIdx.y;
-    const int z = a.plane_ids[blockIdx.z];
+    const int z = a.plane_ids[blockIdx.z];      // kernel argument (scalar load)
     const RowPlaneDesc& pd = a.pd[z];           // read in place (no private copy)
     const int L = pd.rp.L;
     float2* buf = lds + wv * a.wbuf;
@@ -1099,11 +1110,11 @@ __global__ __launch_bounds__(256) void k_ifft_rows_plane(IfftRowPlaneArgs a) {
 // with a fixed reduction order (deterministic).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_fit_partial(const float* img, const float* im, long long n_per_frame,
-                                                     double* part, int nblk) {
+                                                     int im_frames, double* part, int nblk) {
     __shared__ double sd[256], si[256];
     const int t = blockIdx.y;
     const float* a = img + (long long)t * n_per_frame;
-    const float* b = im + (long long)t * n_per_frame;
+    const float* b = im + (long long)(t % im_frames) * n_per_frame;
     double d2 = 0.0, i2 = 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n_per_frame;
          i += (long long)nblk * blockDim.x) {
@@ -1148,13 +1159,19 @@ __global__ __launch_bounds__(256) void k_fit_final(const double* part, int nblk,
 // Prevedel update with a precomputed denominator (lfdeconv.py:112-126):
 // X *= Htf / v, NaN -> 0, the same arithmetic as k_ifft_rows_plane mode 1.
 // Used for the first plain iterate from a uniform start, X1 = Htf / Ht(H 1),
-// whose denominator does not depend on the image.
-// ratio: X = Htf / v (the same update of X = 1, without reading X).
-__global__ void k_rl_scale(float* X, const float* Htf, const float* v, long long n, int ratio) {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        const float r = Htf[i] / v[i];
-        const float xn = ratio ? r : X[i] * r;
-        X[i] = __builtin_isnan(xn) ? 0.f : xn;
+// whose denominator does not depend on the image (one slab per plane serves
+// every frame: v_frames = 1).  ratio: X = Htf / v (the same update of X = 1,
+// without reading X).  grid (x blocks, nz * T slabs).
+__global__ void k_rl_scale(float* X, const float* Htf, const float* v, int T, long long npix, int v_frames,
+                           int ratio) {
+    const int slab = blockIdx.y;
+    const int z = slab / T, t = slab - z * T;
+    const long long o = (long long)slab * npix;
+    const float* vs = v + ((long long)z * v_frames + (v_frames == 1 ? 0 : t)) * npix;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += (long long)gridDim.x * blockDim.x) {
+        const float r = Htf[o + i] / vs[i];
+        const float xn = ratio ? r : X[o + i] * r;
+        X[o + i] = __builtin_isnan(xn) ? 0.f : xn;
     }
 }
 
@@ -1254,9 +1271,9 @@ hipError_t launch_ifft_rows_plane(const IfftRowPlaneArgs& a, int nplanes, hipStr
     hipLaunchKernelGGL(k_ifft_rows_plane, dim3((a.Y + 3) / 4, a.T, nplanes), dim3(256), lds, s, a);
     return hipGetLastError();
 }
-hipError_t launch_fit(const float* img, const float* im, long long n_per_frame, int T, double* part, int nblk,
-                      double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_fit_partial, dim3(nblk, T), dim3(256), 0, s, img, im, n_per_frame, part, nblk);
+hipError_t launch_fit(const float* img, const float* im, long long n_per_frame, int T, int im_frames, double* part,
+                      int nblk, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_fit_partial, dim3(nblk, T), dim3(256), 0, s, img, im, n_per_frame, im_frames, part, nblk);
     hipLaunchKernelGGL(k_fit_final, dim3(T), dim3(256), 0, s, part, nblk, out);
     return hipGetLastError();
 }
@@ -1278,14 +1295,16 @@ hipError_t configure_kernels(size_t max_lds) {
     return hipSuccess;
 }
 
-hipError_t launch_rl_scale(float* X, const float* Htf, const float* v, long long n, hipStream_t s, bool ratio) {
-    long long blocks = (n + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
+hipError_t launch_rl_scale(float* X, const float* Htf, const float* v, int nz, int T, long long npix, int v_frames,
+                           hipStream_t s, bool ratio) {
+    long long blocks = (npix + 255) / 256;
+    if (blocks > 65535) blocks = 65535;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_rl_scale, dim3((unsigned)blocks), dim3(256), 0, s, X, Htf, v, n, ratio ? 1 : 0);
+    // slabs on y (nz * T <= 65535 is checked by the caller), pixels on x
+    hipLaunchKernelGGL(k_rl_scale, dim3((unsigned)blocks, (unsigned)(nz * T)), dim3(256), 0, s, X, Htf, v, T, npix,
+                       v_frames, ratio ? 1 : 0);
     return hipGetLastError();
 }
-
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t s) {
     long long blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;

@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "../../include/flf_hip.h"
+#include "flf_devptr.h"
 
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -720,6 +721,11 @@ int flf_net_get_grads(flf_net* net, float* host) {
 
 int flf_net_forward(flf_net* net, const float* img, int T, int Ly, int Lx, float* out, void* stream) {
     if (!net || !img || !out || T < 1 || Ly < 1 || Lx < 1) return nfail(net, FLF_EINVAL, "bad network input");
+    std::string why;
+    const long long npx = (long long)T * Ly * net->N * Lx * net->N;
+    if (!flf::device_span(net->device, img, (size_t)npx * sizeof(float), "image", why) ||
+        !flf::device_span(net->device, out, (size_t)T * net->nz * Ly * Lx * sizeof(float), "output", why))
+        return nfail(net, FLF_EINVAL, why);
     hipStream_t s = (hipStream_t)stream;
     int rc = forward(net, img, T, Ly, Lx, s);
     if (rc) return rc;
@@ -734,6 +740,11 @@ int flf_net_forward(flf_net* net, const float* img, int T, int Ly, int Lx, float
 int flf_net_train_step(flf_net* net, const float* img, const float* target, int T, int Ly, int Lx, float lr,
                        double* loss_host, void* stream) {
     if (!net || !img || !target || T < 1 || Ly < 1 || Lx < 1) return nfail(net, FLF_EINVAL, "bad training input");
+    std::string why;
+    const long long npx = (long long)T * Ly * net->N * Lx * net->N;
+    if (!flf::device_span(net->device, img, (size_t)npx * sizeof(float), "image", why) ||
+        !flf::device_span(net->device, target, (size_t)T * net->nz * Ly * Lx * sizeof(float), "target", why))
+        return nfail(net, FLF_EINVAL, why);
     hipStream_t s = (hipStream_t)stream;
     int rc = forward(net, img, T, Ly, Lx, s);
     if (rc) return rc;
@@ -756,9 +767,18 @@ int flf_net_train_step(flf_net* net, const float* img, const float* target, int 
 }
 
 int flf_net_upsample(const float* g, int T, int nz, int Ly, int Lx, int nnum, float* vol, void* stream) {
+    flf::tls_error().clear();
     if (!g || !vol || T < 1 || nz < 1 || Ly < 1 || Lx < 1 || nnum < 1) return FLF_EINVAL;
     const long long rows = (long long)nz * T * Ly * nnum;
     if (rows > INT32_MAX || (long long)Lx * nnum > INT32_MAX) return FLF_EINVAL;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return FLF_EHIP;
+    std::string why;
+    if (!flf::device_span(dev, g, (size_t)T * nz * Ly * Lx * sizeof(float), "guess", why) ||
+        !flf::device_span(dev, vol, (size_t)rows * Lx * nnum * sizeof(float), "volume", why)) {
+        flf::tls_error() = why;
+        return FLF_EINVAL;
+    }
     hipLaunchKernelGGL(k_upsample, dim3((unsigned)std::min<long long>(rows, 65536)), dim3(256), 0, (hipStream_t)stream,
                        g, T, nz, Ly, Lx, nnum, vol);
     return hipGetLastError() == hipSuccess ? FLF_OK : FLF_EHIP;
@@ -767,7 +787,16 @@ int flf_net_upsample(const float* g, int T, int nz, int Ly, int Lx, int nnum, fl
 // C = A * B^T for bf16 row-major A [M][K], B [N][K]; fp32 C [M][N].  Exposed
 // for the MFMA layout test.
 int flf_mfma_gemm_nt(const void* A, const void* B, float* C, int M, int N, int K, void* stream) {
+    flf::tls_error().clear();
     if (!A || !B || !C || M < 1 || N < 1 || K < 16 || K % 16) return FLF_EINVAL;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return FLF_EHIP;
+    std::string why;
+    if (!flf::device_span(dev, A, (size_t)M * K * 2, "A", why) || !flf::device_span(dev, B, (size_t)N * K * 2, "B", why) ||
+        !flf::device_span(dev, C, (size_t)M * N * sizeof(float), "C", why)) {
+        flf::tls_error() = why;
+        return FLF_EINVAL;
+    }
     GemmArgs a{};
     a.A = (const bf16*)A; a.lda = K;
     a.B = (const bf16*)B; a.ldb = K;

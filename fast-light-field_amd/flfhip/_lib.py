@@ -33,18 +33,23 @@ SIGNATURES = {
                                 _c.c_void_p]),
     "flf_backward_rl_update": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_void_p,
                                           _c.c_int, _c.POINTER(_c.c_int), _c.c_int, _c.c_void_p]),
-    "flf_datafit": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.POINTER(_c.c_double), _c.c_void_p]),
+    "flf_forward_ratio": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_void_p,
+                                     _c.POINTER(_c.c_int), _c.c_int, _c.c_void_p]),
+    "flf_datafit": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_int, _c.POINTER(_c.c_double),
+                               _c.c_void_p]),
     "flf_fourier_len": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int]),
     "flf_grid_check": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.POINTER(_c.c_longlong)]),
     "flf_set_frames": (_c.c_int, [_c.c_void_p, _c.c_int]),
     "flf_forward_pair": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_void_p, _c.POINTER(_c.c_int),
                                     _c.c_int, _c.c_void_p]),
-    "flf_rl_ratio": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_longlong, _c.c_void_p]),
+    "flf_rl_ratio": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_int, _c.c_int, _c.c_longlong, _c.c_int,
+                                _c.c_void_p]),
     "flf_set_frame_dims": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_int, _c.c_void_p]),
     "flf_frame_capacity": (_c.c_int, [_c.c_void_p]),
     "flf_set_option": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.c_int]),
     "flf_get_option": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.POINTER(_c.c_int)]),
-    "flf_rl_scale": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_longlong, _c.c_void_p]),
+    "flf_rl_scale": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_int, _c.c_int, _c.c_longlong, _c.c_int,
+                                _c.c_void_p]),
     "flf_net_create": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.POINTER(_c.c_void_p)]),
     "flf_net_destroy": (None, [_c.c_void_p]),
     "flf_net_last_error": (_c.c_char_p, [_c.c_void_p]),
@@ -60,6 +65,7 @@ SIGNATURES = {
                                     _c.c_void_p]),
     "flf_mfma_gemm_nt": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_int, _c.c_int, _c.c_int,
                                     _c.c_void_p]),
+    "flf_check_device_buffer": (_c.c_int, [_c.c_int, _c.c_void_p, _c.c_longlong]),
     "flf_profile_enable": (_c.c_int, [_c.c_void_p, _c.c_int]),
     "flf_profile_read": (_c.c_int, [_c.c_void_p, _c.POINTER(_c.c_double), _c.POINTER(_c.c_longlong),
                                     _c.POINTER(_c.c_double)]),

@@ -7,7 +7,7 @@ the method itself: seeded blob volumes are forward-projected through the same
 PSF set with Poisson noise and reconstructed by plain RL; the network learns
 the lenslet-block mean of that reconstruction.  The reference has no learned
 component; plain RL from a uniform start stays the fallback (the guard in
-``guess_or_fallback``).
+``LearnedInit.initial_volume``).
 """
 import ctypes
 import os
@@ -271,11 +271,13 @@ class LearnedInit(object):
         self._den, self._den_key = None, None
         self._side = None
 
-    def plain_denominator(self, T, Y, X):
-        key = (T, Y, X)
+    def plain_denominator(self, Y, X):
+        """Ht(H 1) for one frame, (numZ, 1, Y, X): the same for every frame of
+        this size, so one slab per plane serves a whole batch."""
+        key = (Y, X)
         if self._den_key != key:
             p, hm = self.projector, self.hMatrix
-            ones = torch.ones((hm.numZ, T, Y, X), dtype=torch.float32, device=p.device)
+            ones = torch.ones((hm.numZ, 1, Y, X), dtype=torch.float32, device=p.device)
             H1 = p.forward_device(hm, ones)
             del ones
             self._den = p.backward_device(hm, H1)
@@ -310,13 +312,13 @@ class LearnedInit(object):
         for im if already computed (guess_and_htf)."""
         p, hm = self.projector, self.hMatrix
         T, Y, Xw = im.shape
-        den = self.plain_denominator(T, Y, Xw)
+        den = self.plain_denominator(Y, Xw)
         g = self.net.volume_guess(im) if guess is None else guess
         X1 = torch.empty_like(Htf)
         p.rl_ratio_device(X1, Htf, den)                        # X1 = Htf / Ht(H 1): one iteration from X = 1
         if getattr(p, "_tcap", 0) >= 2 * T:
             H2 = p.forward_pair_device(hm, g, X1)              # (2T, Y, X): both volumes, one call
-            fits = _fit(p, H2, torch.cat([im, im], dim=0))
+            fits = _fit(p, H2, im)                              # frame t and T + t against im[t]
             fit_g, fit_1 = fits[:T], fits[T:]
             Hg, HX1 = H2[:T], H2[T:]
             del H2

@@ -73,7 +73,19 @@ def deconv_rl_device(projector, hMatrix, Htf, maxIter, X, im=None, prevedel=True
     <= target (the fit of X is measured by the forward projection that
     starts the next iteration, so this costs no extra projection).  HX0, if
     given, is the forward projection of the starting X (saves one); fit0,
-    with HX0, is its per-frame data fit (saves one reduction)."""
+    with HX0, is its per-frame data fit (saves one reduction).  The standard
+    update (prevedel=False) takes its error image im / HX from the pass that
+    writes HX (flf_forward_ratio) and ignores HX0."""
+    T, Y, Xw = X.shape[1:]
+    projector._need(X, "X", (hMatrix.numZ, T, Y, Xw))
+    if prevedel:
+        projector._need(Htf, "Htf", (hMatrix.numZ, T, Y, Xw))
+    if im is not None:
+        projector._need(im, "im", (T, Y, Xw))
+    if not prevedel:
+        if im is None:
+            raise ValueError("standard RL needs the image im")
+        HX0 = fit0 = None
     it = 0
     target = None if targetFit is None else np.broadcast_to(np.asarray(targetFit, np.float64), (X.shape[1],))
     need_fit = im is not None and (fitOut is not None or target is not None)
@@ -81,11 +93,14 @@ def deconv_rl_device(projector, hMatrix, Htf, maxIter, X, im=None, prevedel=True
         if it >= maxIter and not need_fit:
             break
         known = None
+        err = None
         if HX0 is not None:
             HX, HX0 = HX0, None
             known, fit0 = fit0, None
-        else:
+        elif prevedel:
             HX = projector.forward_device(hMatrix, X, planes)
+        else:
+            HX, err = projector.forward_ratio_device(hMatrix, X, im, planes, store_image=need_fit)
         if need_fit:
             fit = np.asarray(known, np.float64) if known is not None else _fit_from_sums(projector.datafit_device(HX, im))
             if fitOut is not None:
@@ -97,7 +112,6 @@ def deconv_rl_device(projector, hMatrix, Htf, maxIter, X, im=None, prevedel=True
         if prevedel:
             projector.rl_update_device(hMatrix, HX, X, Htf=Htf, mode=1, planes=planes)
         else:
-            err = im / HX
             projector.rl_update_device(hMatrix, err, X, trust=trust, mode=2, planes=planes)
         it += 1
     return it
@@ -111,11 +125,16 @@ class RLGraph(object):
     the graph reads and writes (copy new data into them, do not rebind).
     The capture happens on first use, after one eager warm-up iteration on a
     scratch copy, so every lazy allocation and table upload is done before
-    capture; the captured work contains no memset or host-memory copy."""
+    capture; the captured work contains no memset or host-memory copy.  The
+    plane selection is part of the captured kernel arguments; the plan's
+    device tables (and per-frame sizes) are not, so the graph is captured
+    again whenever the projector has been planned again or its frame sizes
+    or frame count changed since the capture (Projector_hip.state_key)."""
 
     def __init__(self, projector, hMatrix, Htf, X, planes=None):
         self.projector, self.hMatrix, self.Htf, self.X, self.planes = projector, hMatrix, Htf, X, planes
         self.graph = None
+        self._state = None
 
     def _capture(self):
         p, hm = self.projector, self.hMatrix
@@ -129,9 +148,15 @@ class RLGraph(object):
             HX = p.forward_device(hm, self.X, self.planes)
             p.rl_update_device(hm, HX, self.X, Htf=self.Htf, mode=1, planes=self.planes)
         self._hx = HX
+        self._state = p.state_key()
 
     def run(self, iterations):
-        if self.graph is None:
+        T, Y, Xw = self.X.shape[1:]
+        p = self.projector
+        # the plan for this shape must be the active one (a no-op when it is)
+        p._ensure_plan(self.hMatrix, T, Y, Xw)
+        if self.graph is None or self._state != p.state_key():
+            self.graph = None
             self._capture()
         for _ in range(iterations):
             self.graph.replay()
@@ -147,12 +172,22 @@ def DeconvRL(hMatrix, Htf, maxIter, Xguess, logPrint=True, numjobs=1, projector=
         return _deconv_generic(hMatrix, Htf, maxIter, Xguess, logPrint, numjobs, projector, im, prevedel, trustMask)
     if Htf is None:
         assert im is not None
+    # shapes are checked before anything moves to the device
     if im is not None:
-        imT = projector.asnative(im)
-        T, Y, X = imT.shape
+        if len(np.shape(im)) != 3:
+            raise ValueError("im must be (T, Y, X)")
+        T, Y, X = np.shape(im)
     else:
-        T, Y, X = Htf.shape[1:]
-        imT = None
+        if Htf is None or len(np.shape(Htf)) != 4:
+            raise ValueError("Htf must be (numZ, T, Y, X) when im is not given")
+        T, Y, X = np.shape(Htf)[1:]
+    want = (hMatrix.numZ, T, Y, X)
+    for name, arr in (("Htf", Htf), ("Xguess", Xguess)):
+        if arr is not None and tuple(np.shape(arr)) != want:
+            raise ValueError("%s has shape %s, expected %s (numZ,) + image shape" % (name, tuple(np.shape(arr)), want))
+    if trustMask is not None and tuple(np.shape(trustMask)) != (Y, X):
+        raise ValueError("trustMask has shape %s, expected %s" % (tuple(np.shape(trustMask)), (Y, X)))
+    imT = projector.asnative(im) if im is not None else None
     projector.prepare(hMatrix, T, Y, X)
     if Htf is None:
         if prevedel or Xguess is None:

@@ -23,6 +23,10 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _shape_str(shape):
+    return "(" + ", ".join(str(int(v)) for v in shape) + ")"
+
+
 class Projector_hip(object):
     def __init__(self, device=0):
         if not torch.cuda.is_available():
@@ -40,6 +44,14 @@ class Projector_hip(object):
         self._key = None
         self._tcap = self._frames = 0
         self._loaded = set()
+        # plan_generation counts flf_plan calls (each frees the previous
+        # plan's tables); _dims is the per-frame size setting (None: full
+        # size), whose values the library uploads on every ragged call.  A
+        # captured HIP graph holds pointers to the plan's tables and reads the
+        # uploaded sizes, so it is replayed only while both are unchanged
+        # (lfdeconv.RLGraph).
+        self.plan_generation = 0
+        self._dims = None
 
     def __del__(self):
         lib = getattr(self, "lib", None)
@@ -87,6 +99,7 @@ class Projector_hip(object):
             if T != self._frames:
                 self._check(self.lib.flf_set_frames(self._ctx, int(T)), "flf_set_frames")
                 self._frames = T
+                self._set_dims_state(None)      # flf_set_frames returns to full-size frames
             return
         s2 = []
         for cc in range(hMatrix.numZ):
@@ -97,6 +110,8 @@ class Projector_hip(object):
         arr = (ctypes.c_int * len(s2))(*s2)
         self._key = None
         self._loaded = set()
+        self.plan_generation += 1
+        self._dims = None
         self._check(self.lib.flf_plan(self._ctx, int(hMatrix.Nnum), int(Y), int(X), int(T), len(s2), arr), "flf_plan")
         self._key = key
         self._tcap = self._frames = int(T)
@@ -132,12 +147,24 @@ class Projector_hip(object):
         if dims is None:
             self._check(self.lib.flf_set_frame_dims(self._ctx, None, int(self._frames), _stream_handle()),
                         "flf_set_frame_dims")
+            self._set_dims_state(None)
             return
         flat = [int(v) for d in dims for v in d]
         if len(flat) != 2 * len(dims):
             raise ValueError("dims must be (Y, X) pairs")
         arr = (ctypes.c_int * len(flat))(*flat)
         self._check(self.lib.flf_set_frame_dims(self._ctx, arr, len(dims), _stream_handle()), "flf_set_frame_dims")
+        full = self._key is not None and all((int(y), int(x)) == tuple(self._key[-2:]) for y, x in dims)
+        self._set_dims_state(None if full else tuple(flat))
+
+    def _set_dims_state(self, dims):
+        self._dims = dims
+
+    def state_key(self):
+        """Identifies the device state a captured graph of this projector
+        refers to (plan tables, active frame count, per-frame sizes): it
+        changes whenever that state is freed or rewritten."""
+        return (self.plan_generation, self._frames, self._dims)
 
     def set_option(self, name, value):
         """Launch-shape option of the library (flf_set_option)."""
@@ -159,32 +186,76 @@ class Projector_hip(object):
         return planes, (ctypes.c_int * len(planes))(*planes)
 
     # ---- device-level entry points (tensors on this device) ----
+    def _need(self, t, name, shape):
+        """t must be a contiguous float32 tensor on this projector's device
+        with exactly `shape`; checked before anything is queued."""
+        if not isinstance(t, torch.Tensor):
+            raise TypeError("%s must be a torch tensor on %s" % (name, self.device))
+        if t.dtype != torch.float32:
+            raise TypeError("%s must be float32, not %s" % (name, t.dtype))
+        if t.device != self.device:
+            raise ValueError("%s is on %s, the projector on %s" % (name, t.device, self.device))
+        if tuple(t.shape) != tuple(int(v) for v in shape):
+            raise ValueError("%s has shape %s, expected %s" % (name, _shape_str(t.shape), _shape_str(shape)))
+        if not t.is_contiguous():
+            raise ValueError("%s must be contiguous" % name)
+        return t
+
+    def _image(self, img, name="image"):
+        if not isinstance(img, torch.Tensor) or img.dim() != 3:
+            raise ValueError("%s must be a (T, Y, X) tensor" % name)
+        img = img.contiguous() if img.dtype == torch.float32 else img
+        return self._need(img, name, img.shape)
+
     def forward_device(self, hMatrix, vol, planes=None):
         """vol: (numZ, T, Y, X) float32 cuda tensor -> image (T, Y, X) tensor."""
+        if not isinstance(vol, torch.Tensor) or vol.dim() != 4:
+            raise ValueError("volume must be a (numZ, T, Y, X) tensor")
         nz, T, Y, X = vol.shape
         if nz != hMatrix.numZ:
             raise ValueError("volume has %d planes, HMatrix has %d" % (nz, hMatrix.numZ))
+        vol = self._need(vol.contiguous() if vol.dtype == torch.float32 else vol, "volume", vol.shape)
         self._ensure_plan(hMatrix, T, Y, X)
         planes, arr = self._plane_array(planes, hMatrix.numZ)
         self._ensure_psf(hMatrix, planes, 0)
-        vol = vol.contiguous()
         img = torch.empty((T, Y, X), dtype=torch.float32, device=self.device)
         self._check(self.lib.flf_forward(self._ctx, _ptr(vol), _ptr(img), arr, len(planes), _stream_handle()),
                     "flf_forward")
         return img
 
+    def forward_ratio_device(self, hMatrix, vol, im, planes=None, store_image=True):
+        """Forward projection HX of vol and standard RL's error image im / HX
+        from one pass (flf_forward_ratio).  Returns (HX or None, ratio)."""
+        if not isinstance(vol, torch.Tensor) or vol.dim() != 4:
+            raise ValueError("volume must be a (numZ, T, Y, X) tensor")
+        nz, T, Y, X = vol.shape
+        if nz != hMatrix.numZ:
+            raise ValueError("volume has %d planes, HMatrix has %d" % (nz, hMatrix.numZ))
+        vol = self._need(vol.contiguous() if vol.dtype == torch.float32 else vol, "volume", vol.shape)
+        im = self._need(im, "im", (T, Y, X))
+        self._ensure_plan(hMatrix, T, Y, X)
+        planes, arr = self._plane_array(planes, hMatrix.numZ)
+        self._ensure_psf(hMatrix, planes, 0)
+        img = torch.empty((T, Y, X), dtype=torch.float32, device=self.device) if store_image else None
+        ratio = torch.empty((T, Y, X), dtype=torch.float32, device=self.device)
+        self._check(self.lib.flf_forward_ratio(self._ctx, _ptr(vol), _ptr(im),
+                                               _ptr(img) if img is not None else ctypes.c_void_p(0), _ptr(ratio),
+                                               arr, len(planes), _stream_handle()), "flf_forward_ratio")
+        return img, ratio
+
     def forward_pair_device(self, hMatrix, volA, volB, planes=None):
         """Forward projection of two volumes (numZ, S, Y, X) as one call of 2S
         frames (flf_forward_pair): image (2S, Y, X), frames of volA first."""
-        if volA.shape != volB.shape:
-            raise ValueError("volA and volB must have one shape")
+        if not isinstance(volA, torch.Tensor) or volA.dim() != 4:
+            raise ValueError("volA must be a (numZ, S, Y, X) tensor")
         nz, S, Y, X = volA.shape
         if nz != hMatrix.numZ:
             raise ValueError("volume has %d planes, HMatrix has %d" % (nz, hMatrix.numZ))
+        volA = self._need(volA, "volA", volA.shape)
+        volB = self._need(volB, "volB", volA.shape)
         self._ensure_plan(hMatrix, 2 * S, Y, X)
         planes, arr = self._plane_array(planes, hMatrix.numZ)
         self._ensure_psf(hMatrix, planes, 0)
-        volA, volB = volA.contiguous(), volB.contiguous()
         img = torch.empty((2 * S, Y, X), dtype=torch.float32, device=self.device)
         self._check(self.lib.flf_forward_pair(self._ctx, _ptr(volA), _ptr(volB), _ptr(img), arr, len(planes),
                                               _stream_handle()), "flf_forward_pair")
@@ -192,12 +263,16 @@ class Projector_hip(object):
 
     def backward_device(self, hMatrix, img, planes=None, out=None):
         """img: (T, Y, X) cuda tensor -> volume (numZ, T, Y, X); planes not
-        listed are zero (Projector_allC.BackwardProjectACC, projector.py:807)."""
+        listed are zero (Projector_allC.BackwardProjectACC, projector.py:807).
+        out: a (numZ, T, Y, X) float32 tensor to write instead (planes not
+        listed are left as they are)."""
+        img = self._image(img)
         T, Y, X = img.shape
+        if out is not None:
+            self._need(out, "out", (hMatrix.numZ, T, Y, X))
         self._ensure_plan(hMatrix, T, Y, X)
         planes, arr = self._plane_array(planes, hMatrix.numZ)
         self._ensure_psf(hMatrix, planes, 1)
-        img = img.contiguous()
         if out is None:
             out = torch.zeros((hMatrix.numZ, T, Y, X), dtype=torch.float32, device=self.device)
         self._check(self.lib.flf_backward(self._ctx, _ptr(img), _ptr(out), arr, len(planes), _stream_handle()),
@@ -205,44 +280,77 @@ class Projector_hip(object):
         return out
 
     def rl_update_device(self, hMatrix, img, X, Htf=None, trust=None, mode=1, planes=None):
-        """Back-project img and update X in place (mode 1 Prevedel, 2 standard)."""
+        """Back-project img and update X in place (mode 1 Prevedel: X *= Htf /
+        B(img); mode 2 standard: X *= B(img) [/ trust]).  X and Htf are
+        (numZ, T, Y, X), trust (numZ, 1, Y, X), all float32 and contiguous on
+        this device."""
+        img = self._image(img)
         T, Y, Xw = img.shape
+        nz = hMatrix.numZ
+        self._need(X, "X", (nz, T, Y, Xw))
+        if mode == 1:
+            if Htf is None:
+                raise ValueError("the Prevedel update needs Htf")
+            self._need(Htf, "Htf", (nz, T, Y, Xw))
+        elif mode == 2:
+            if trust is not None:
+                self._need(trust, "trust", (nz, 1, Y, Xw))
+        else:
+            raise ValueError("mode must be 1 (Prevedel) or 2 (standard)")
         self._ensure_plan(hMatrix, T, Y, Xw)
         planes, arr = self._plane_array(planes, hMatrix.numZ)
         self._ensure_psf(hMatrix, planes, 1)
-        hp = _ptr(Htf) if Htf is not None else ctypes.c_void_p(0)
-        tp = _ptr(trust) if trust is not None else ctypes.c_void_p(0)
-        self._check(self.lib.flf_backward_rl_update(self._ctx, _ptr(img.contiguous()), _ptr(X), hp, tp, int(mode),
+        hp = _ptr(Htf) if (Htf is not None and mode == 1) else ctypes.c_void_p(0)
+        tp = _ptr(trust) if (trust is not None and mode == 2) else ctypes.c_void_p(0)
+        self._check(self.lib.flf_backward_rl_update(self._ctx, _ptr(img), _ptr(X), hp, tp, int(mode),
                                                     arr, len(planes), _stream_handle()), "flf_backward_rl_update")
+
+    def _scale_args(self, X, Htf, v):
+        if not isinstance(X, torch.Tensor) or X.dim() != 4:
+            raise ValueError("X must be a (numZ, T, Y, X) tensor")
+        nz, T, Y, Xw = X.shape
+        self._need(X, "X", X.shape)
+        self._need(Htf, "Htf", X.shape)
+        if not isinstance(v, torch.Tensor) or v.dim() != 4 or v.shape[1] not in (1, T):
+            raise ValueError("v must be (numZ, 1 or T, Y, X)")
+        self._need(v, "v", (nz, v.shape[1], Y, Xw))
+        return nz, T, Y * Xw, int(v.shape[1])
 
     def rl_scale_device(self, X, Htf, v):
         """X *= Htf / v in place, NaN -> 0 (the Prevedel update with a known
-        denominator v; all three tensors of one shape)."""
-        if X.shape != Htf.shape or X.shape != v.shape:
-            raise ValueError("X, Htf and v must have one shape")
-        if not (X.is_contiguous() and Htf.is_contiguous() and v.is_contiguous()):
-            raise ValueError("X, Htf and v must be contiguous")
-        self._check(self.lib.flf_rl_scale(_ptr(Htf), _ptr(v), _ptr(X), X.numel(), _stream_handle()), "flf_rl_scale")
+        denominator v: (numZ, T, Y, X), or (numZ, 1, Y, X) shared by the
+        frames)."""
+        nz, T, npix, vf = self._scale_args(X, Htf, v)
+        self._check(self.lib.flf_rl_scale(_ptr(Htf), _ptr(v), _ptr(X), nz, T, npix, vf, _stream_handle()),
+                    "flf_rl_scale")
 
     def rl_ratio_device(self, X, Htf, v):
         """X = Htf / v, NaN -> 0 (the first Prevedel iterate from X = 1)."""
-        if X.shape != Htf.shape or X.shape != v.shape:
-            raise ValueError("X, Htf and v must have one shape")
-        if not (X.is_contiguous() and Htf.is_contiguous() and v.is_contiguous()):
-            raise ValueError("X, Htf and v must be contiguous")
-        self._check(self.lib.flf_rl_ratio(_ptr(Htf), _ptr(v), _ptr(X), X.numel(), _stream_handle()), "flf_rl_ratio")
+        nz, T, npix, vf = self._scale_args(X, Htf, v)
+        self._check(self.lib.flf_rl_ratio(_ptr(Htf), _ptr(v), _ptr(X), nz, T, npix, vf, _stream_handle()),
+                    "flf_rl_ratio")
 
     def datafit_device(self, img, im):
-        """Per-frame FP64 (sum((img-im)^2), sum(im^2))."""
-        T = img.shape[0]
+        """Per-frame FP64 (sum((img-im)^2), sum(im^2)).  im may hold fewer
+        frames than img when img's frame count is a multiple of it: frame t
+        of img is compared with frame t % len(im)."""
+        img = self._image(img)
+        im = self._image(im, "im")
+        T, Y, X = img.shape
+        Ti = im.shape[0]
+        if im.shape[1:] != img.shape[1:] or Ti < 1 or T % Ti:
+            raise ValueError("im %s does not match image %s" % (_shape_str(im.shape), _shape_str(img.shape)))
+        if self._key is None or (Y, X) != tuple(self._key[-2:]):
+            raise ValueError("datafit of (%d, %d) frames on a plan for %s" % (Y, X, self._key and self._key[-2:]))
         if T != self._frames:
             if not (1 <= T <= self._tcap):
                 raise ValueError("datafit of %d frames on a plan for %d" % (T, self._tcap))
             self._check(self.lib.flf_set_frames(self._ctx, int(T)), "flf_set_frames")
             self._frames = T
+            self._set_dims_state(None)
         out = (ctypes.c_double * (2 * T))()
-        self._check(self.lib.flf_datafit(self._ctx, _ptr(img.contiguous()), _ptr(im.contiguous()), out,
-                                         _stream_handle()), "flf_datafit")
+        self._check(self.lib.flf_datafit(self._ctx, _ptr(img), _ptr(im), int(Ti), out, _stream_handle()),
+                    "flf_datafit")
         return np.array(out[:], dtype=np.float64).reshape(T, 2)
 
     def profile(self, enable):

@@ -8,6 +8,12 @@
  * aborts; flf_last_error() gives the message.  A context is not re-entrant:
  * calls on one context must not overlap (distinct contexts are independent).
  *
+ * Every array argument that a kernel reads or writes must be device memory
+ * (hipMalloc, including a caching allocator's blocks) of the context's
+ * device, large enough for the layout below; anything else (host memory,
+ * pinned or not, another device's memory, a too-small allocation) returns
+ * FLF_EINVAL before any work is queued.  Host arrays are named *_host.
+ *
  * Array layouts (C order, float32):
  *   volume  [nz][T][Y][X]      image  [T][Y][X]
  *   PSF quadrant of one plane [h][h][s2][s2], h = (Nnum+1)/2, entry [b][a]
@@ -44,7 +50,9 @@ int flf_create(int device, flf_ctx** out);
  * py_light_field.cpp:1323-1335). */
 void flf_destroy(flf_ctx* ctx);
 
-/* Message of the last error on this context ("" if none). */
+/* Message of the last error on this context ("" if none).  With ctx NULL:
+ * the last error of this host thread's context-free calls (flf_rl_scale,
+ * flf_rl_ratio, flf_net_upsample, flf_mfma_gemm_nt). */
 const char* flf_last_error(const flf_ctx* ctx);
 
 /* Plan the geometry: lenslet pitch nnum (odd), image Y x X, T frames per
@@ -53,7 +61,7 @@ const char* flf_last_error(const flf_ctx* ctx);
  * 5-smooth multiple of nnum), builds the kernel tables and allocates the
  * spectrum cache and work buffers.  Replaces the per-plane set-up of
  * ProjectorForZ_base.__init__ (projector.py:106-151).  Drops any uploaded
- * PSFs. */
+ * PSFs.  At most 256 planes, and nz * T <= 65535. */
 int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2);
 
 /* Frames per call.  flf_plan's T is the capacity the work buffers hold;
@@ -128,6 +136,15 @@ int flf_set_psf(flf_ctx* ctx, int z, int dir, const float* quadrant_host);
  * py_light_field.cpp:1417-1740).  stream: hipStream_t or NULL. */
 int flf_forward(flf_ctx* ctx, const float* vol, float* img, const int* planes, int nplanes, void* stream);
 
+/* Forward projection fused with standard RL's error image: ratio[T][Y][X] =
+ * im / (sum over the listed planes of the projection of vol), IEEE division
+ * (0/0 = NaN, x/0 = inf, as numpy), written by the pass that writes the
+ * image; img may be NULL (image not stored).  Replaces ForwardProjectACC
+ * followed by `error = im / HXguess` in DeconvRL's standard branch
+ * (lfdeconv.py:189-192). */
+int flf_forward_ratio(flf_ctx* ctx, const float* vol, const float* im, float* img, float* ratio, const int* planes,
+                      int nplanes, void* stream);
+
 /* Backward projection: vol[z] for each listed plane (others untouched) from
  * img[T][Y][X].  Replaces Projector_allC.BackwardProjectACC with
  * rlUpdateFunc=None (projector.py:806-849). */
@@ -143,22 +160,38 @@ int flf_backward_rl_update(flf_ctx* ctx, const float* img, float* X, const float
                            int mode, const int* planes, int nplanes, void* stream);
 
 /* Forward-model data fit per frame, FP64 sums: out[2t] = sum((img-im)^2),
- * out[2t+1] = sum(im^2) (host array of 2*T doubles).  The reference has no
- * counterpart (DeconvRL always runs maxIter, lfdeconv.py:185-196); this is
- * the convergence measure of this library's RL driver. Synchronises `stream`. */
-int flf_datafit(flf_ctx* ctx, const float* img, const float* im, double* out_host, void* stream);
+ * out[2t+1] = sum(im^2) (host array of 2*T doubles), where frame t of img
+ * (T active frames) is compared with frame t % im_frames of im (im_frames 0:
+ * T; a guard that projects two volumes per image passes im once).  The
+ * reference has no counterpart (DeconvRL always runs maxIter,
+ * lfdeconv.py:185-196); this is the convergence measure of this library's RL
+ * driver.  Synchronises `stream`. */
+int flf_datafit(flf_ctx* ctx, const float* img, const float* im, int im_frames, double* out_host, void* stream);
 
-/* X[i] *= Htf[i] / v[i], NaN -> 0, for n elements on `stream`: the Prevedel
- * RL update of lfdeconv.py:112-126 with a denominator the caller already
- * holds.  With X = 1 and v = Ht(H 1) (which does not depend on the image) it
+/* X *= Htf / v, NaN -> 0, on `stream` (device of the current HIP device):
+ * the Prevedel RL update of lfdeconv.py:112-126 with a denominator the
+ * caller already holds.  X and Htf are (nz, T, npix); v is (nz, v_frames,
+ * npix) with v_frames = T, or 1 for one slab per plane shared by every
+ * frame.  With X = 1 and v = Ht(H 1) (which does not depend on the image) it
  * gives the first plain RL iterate from a uniform start without projecting;
- * the learned initialiser's guard compares its guess against that iterate. */
-int flf_rl_scale(const float* Htf, const float* v, float* X, long long n, void* stream);
+ * the learned initialiser's guard compares its guess against that iterate.
+ * nz * T <= 65535. */
+int flf_rl_scale(const float* Htf, const float* v, float* X, int nz, int T, long long npix, int v_frames,
+                 void* stream);
 
-/* X[i] = Htf[i] / v[i], NaN -> 0: flf_rl_scale applied to X = 1 without
- * reading X (the first plain RL iterate from a uniform start,
- * lfdeconv.py:112-126 with Xguess = 1). */
-int flf_rl_ratio(const float* Htf, const float* v, float* X, long long n, void* stream);
+/* X = Htf / v, NaN -> 0: flf_rl_scale applied to X = 1 without reading X
+ * (the first plain RL iterate from a uniform start, lfdeconv.py:112-126 with
+ * Xguess = 1). */
+int flf_rl_ratio(const float* Htf, const float* v, float* X, int nz, int T, long long npix, int v_frames,
+                 void* stream);
+
+/* FLF_OK if [p, p + bytes) is device memory of `device` in one allocation,
+ * else FLF_EINVAL with the reason in flf_last_error(NULL): the check every
+ * array entry point applies to its arguments, exposed so that a binding can
+ * test a buffer before handing it over.  Host-side only (queues nothing).
+ * No reference counterpart (the reference's py_light_field checks numpy
+ * dtypes and strides, py_light_field.cpp:1489-1493). */
+int flf_check_device_buffer(int device, const void* p, long long bytes);
 
 /* Kernel timing for roofline reporting (no reference counterpart; the
  * reference times whole calls with time.time(), lf_performance_analysis.py:

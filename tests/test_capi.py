@@ -90,7 +90,22 @@ def test_create_without_gpu_fails_cleanly(lib):
 def test_null_context_errors(lib):
     assert lib.flf_plan(None, 15, 10, 10, 1, 1, None) == _lib.FLF_EINVAL
     assert lib.flf_forward(None, None, None, None, 0, None) == _lib.FLF_EINVAL
-    assert lib.flf_last_error(None) == b"null context"
+    # context-free calls report through flf_last_error(NULL), per host thread
+    assert lib.flf_rl_scale(None, None, None, 0, 1, 10, 1, None) == _lib.FLF_EINVAL
+    assert b"bad nz" in lib.flf_last_error(None)
+
+
+def test_host_buffers_refused(lib):
+    # Host memory is never accepted as a kernel argument: the check runs on
+    # the host before anything is queued (here without a GPU at all).
+    a = np.zeros(64, np.float32)
+    assert lib.flf_check_device_buffer(0, a.ctypes.data_as(ctypes.c_void_p), a.nbytes) == _lib.FLF_EINVAL
+    assert b"device memory" in lib.flf_last_error(None) or b"host" in lib.flf_last_error(None)
+    assert lib.flf_check_device_buffer(0, None, 4) == _lib.FLF_EINVAL
+    assert b"null pointer" in lib.flf_last_error(None)
+    p = a.ctypes.data_as(ctypes.c_void_p)
+    # refused: FLF_EINVAL on a GPU box, FLF_EHIP here where no device exists
+    assert lib.flf_rl_scale(p, p, p, 1, 1, 64, 1, None) in (_lib.FLF_EINVAL, _lib.FLF_EHIP)
 
 
 def test_error_mapping():

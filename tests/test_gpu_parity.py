@@ -256,6 +256,7 @@ def test_forward_pair_equals_concatenated_frames(projector):
     torch.cuda.synchronize()
     assert torch.equal(X1, X2)
     assert float(X2[0, 0, 0, 0]) == 0.0                   # NaN -> 0
+    assert np.isposinf(float(X2[0, 0, 0, 2]))             # x/0 -> inf
 
 
 @pytest.mark.parametrize("planes", [None, [0, 2, 3]])
