@@ -42,7 +42,7 @@ sys.path.insert(0, os.path.join(ROOT, "fast-light-field_amd"))
 
 import numpy as np  # noqa: E402
 
-from flfhip.workloads import RAGGED, WORKLOADS, setup  # noqa: E402
+from flfhip.workloads import BATCHES, RAGGED, WORKLOADS, setup  # noqa: E402
 
 PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
 PEAK_FP32 = 157.3e12  # MI355X FP32, flop/s
@@ -253,13 +253,268 @@ def ragged_main(args):
         log("cpu baseline %.1fs" % (time.time() - t))
     print(json.dumps(res), flush=True)
 
+def _rel_to_truth(X, vol, scale):
+    """Per-frame relative L2 of a reconstruction against the ground-truth
+    volume: X is in image counts, vol in volume units; scale[t] converts
+    (the image is Poisson(scale[t] * H vol_t))."""
+    import torch
+    out = []
+    for t in range(X.shape[1]):
+        a = X[:, t].double() / float(scale[t])
+        b = vol[:, t].double()
+        out.append(float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)))
+    return out
+
+
+def _iters_to_target(fits, target):
+    """First iteration count whose fit is <= the frame's target, from a list
+    of per-iteration fit arrays (entry 0: the starting volume)."""
+    F = np.array([np.asarray(f, np.float64) for f in fits])
+    out = []
+    for t in range(F.shape[1]):
+        k = np.nonzero(F[:, t] <= target[t])[0]
+        out.append(int(k[0]) if len(k) else -1)
+    return out
+
+
+def oracle_check(proj, hm, im_t, X0_t, iters=1, planes=(0, 20)):
+    """Bounded FP64-oracle comparison of one frame: Prevedel RL restricted to
+    two planes (the largest-PSF plane and z = 0) from the given start, on the
+    GPU and in the FP64 oracle (numpy, one thread); relative L2 per plane."""
+    import torch
+    sys.path.insert(0, ROOT)
+    from oracle import projector as oproj
+    from oracle import rl as orl
+    from flfhip.lfdeconv import deconv_rl_device
+    planes = list(planes)
+    Y, X = im_t.shape[-2:]
+    proj.prepare(hm, 1, Y, X)
+    im1 = im_t.reshape(1, Y, X).contiguous()
+    Htf = proj.backward_device(hm, im1, planes=planes)
+    Xv = torch.zeros((hm.numZ, 1, Y, X), dtype=torch.float32, device=proj.device)
+    Xv[planes] = X0_t[planes].reshape(len(planes), 1, Y, X)
+    deconv_rl_device(proj, hm, Htf, iters, Xv, planes=planes)
+    Qf = [hm.Hcc(z, False) for z in planes]
+    Qb = [hm.Hcc(z, True) for z in planes]
+    t = time.time()
+    Xr = orl.deconv_rl(lambda v: oproj.forward_project(v, Qf), lambda i: oproj.backward_project(i, Qb), None, iters,
+                       Xguess=X0_t[planes].reshape(len(planes), 1, Y, X).cpu().numpy().astype(np.float64),
+                       im=im1.cpu().numpy().astype(np.float64))
+    secs = time.time() - t
+    Xg = Xv.cpu().numpy()
+    errs = [float(np.linalg.norm(Xg[z, 0] - Xr[k, 0]) / np.linalg.norm(Xr[k, 0])) for k, z in enumerate(planes)]
+    return {"frame": 0, "planes": [int(hm.PSFShape(z)[0]) for z in planes], "plane_ids": planes,
+            "iterations": iters, "start": "the learned leg's starting volume", "rel_l2_per_plane": errs,
+            "oracle_s": secs,
+            "what": "Prevedel RL of frame 0 restricted to planes %s (PSF sides %s px) on the GPU vs the FP64 numpy "
+                    "oracle from the same start" % (planes, [int(hm.PSFShape(z)[0]) for z in planes])}
+
+
+def batch_main(args):
+    """Batches of full-size frames (workloads.BATCHES): plain RL against the
+    guarded learned start on T frames reconstructed together."""
+    import torch
+    from flfhip import lensletnet
+    from flfhip.lfdeconv import deconv_rl_device, _fit_from_sums
+    from flfhip.projector import Projector_hip
+    from flfhip.workloads import setup_batch
+
+    w = setup_batch(args.workload)
+    hm, Y, X, iters, T = w["hm"], w["Y"], w["X"], w["iters"], w["frames"]
+    if args.frames > 1:
+        T = args.frames
+    nz = len(w["zs"])
+    proj = Projector_hip()
+    torch.cuda.synchronize()
+    t = time.time()
+    proj.prepare(hm, T, Y, X, frames_capacity=2 * T)
+    torch.cuda.synchronize()
+    t_prep = time.time() - t
+    log("spectra precompute: %.2fs, device bytes %.2f GB" % (t_prep, proj.lib.flf_device_bytes(proj._ctx) / 1e9))
+    tuned = None
+    if not args.no_autotune:
+        from flfhip import autotune
+        t = time.time()
+        tuned, _ = autotune.tune(proj, hm, T, Y, X)
+        log("autotune %.1fs: %s" % (time.time() - t, tuned))
+    dev = proj.device
+    t = time.time()
+    vol = lensletnet.blob_volumes_gpu(nz, Y, X, T, w["nblobs"], w["seed"], dev)
+    proj.prepare(hm, T, Y, X)
+    clean = proj.forward_device(hm, vol)
+    scale = w["peak"] / clean.reshape(T, -1).amax(dim=1)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(w["seed"])
+    im = torch.poisson(clean.clamp_min(0) * scale[:, None, None], generator=gen)
+    del clean
+    torch.cuda.synchronize()
+    log("volumes and images: %d frames, %.1fs" % (T, time.time() - t))
+    Htf = torch.empty((nz, T, Y, X), dtype=torch.float32, device=dev)
+    Xv = torch.empty_like(Htf)
+    s = torch.cuda.current_stream()
+
+    def timed(step, n):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(s)
+        for _ in range(n):
+            step()
+        e1.record(s)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    # plain leg: Htf = Ht im, then iters Prevedel iterations from X = 1, all frames together
+    def plain_step():
+        proj.backward_device(hm, im, out=Htf)
+        Xv.fill_(1.0)
+        deconv_rl_device(proj, hm, Htf, iters, Xv)
+
+    for _ in range(args.warmup):
+        plain_step()
+    plain_ms = timed(plain_step, args.steps)
+    fit_plain_t = _fit_from_sums(proj.datafit_device(proj.forward_device(hm, Xv), im))
+    gt_plain = _rel_to_truth(Xv, vol, scale)
+    X_plain = Xv.clone()
+    log("plain RL: %.1f ms per step of %d frames (%.1f volumes/s)" % (plain_ms, T, T * 1e3 / plain_ms))
+
+    wpath = args.weights or lensletnet.default_weights_path(w["N"], nz, w["base"])
+    net = lensletnet.LensletNet.load(wpath)
+    li = lensletnet.LearnedInit(net, proj, hm)
+    proj.prepare(hm, T, Y, X)
+    li.plain_denominator(Y, X)
+    out = {}
+
+    def learned_step():
+        fits = []
+        Xl, n_it, info = li.deconvolve(im, None, 4 * iters, targetFit=fit_plain_t, Htf_out=Htf, fitOut=fits,
+                                       minIter=args.min_iter)
+        out.update(X=Xl, n=n_it, info=info, fits=fits)
+
+    for _ in range(args.warmup):
+        learned_step()
+    proj.profile(True)
+    learned_ms = timed(learned_step, args.steps)
+    proj.profile(False)
+    mac_ms, mac_n, mac_bytes = proj.profile_read()
+    info = out["info"]
+    fit_l_t = _fit_from_sums(proj.datafit_device(proj.forward_device(hm, out["X"]), im))
+    gt_learned = _rel_to_truth(out["X"], vol, scale)
+    rel_vs_plain = [float(torch.linalg.vector_norm((out["X"][:, t] - X_plain[:, t]).double())
+                          / torch.linalg.vector_norm(X_plain[:, t].double())) for t in range(T)]
+    need = _iters_to_target(out["fits"], fit_plain_t)
+    log("learned: %.1f ms per step (%.1f volumes/s), %d iterations" % (learned_ms, T * 1e3 / learned_ms, out["n"]))
+
+    # the same legs on one frame (frame 0), for the single-frame figures
+    X1s = out["X"][:, :1].clone()
+    proj.prepare(hm, 1, Y, X)
+    im0 = im[:1].contiguous()
+    Htf0 = torch.empty((nz, 1, Y, X), dtype=torch.float32, device=dev)
+    X0v = torch.empty_like(Htf0)
+
+    def plain1():
+        proj.backward_device(hm, im0, out=Htf0)
+        X0v.fill_(1.0)
+        deconv_rl_device(proj, hm, Htf0, iters, X0v)
+
+    plain1()
+    plain1_ms = timed(plain1, args.steps)
+
+    def learned1():
+        li.deconvolve(im0, None, 4 * iters, targetFit=fit_plain_t[:1], Htf_out=Htf0, minIter=args.min_iter)
+
+    learned1()
+    learned1_ms = timed(learned1, args.steps)
+    log("one frame: plain %.2f ms, learned %.2f ms" % (plain1_ms, learned1_ms))
+
+    # algorithmic bytes and flops of the multiply-accumulate launches
+    s_dir = 0.0
+    bins = 0
+    for z in range(nz):
+        fy, fx = proj.fshape(z)
+        u = 36 if fy == fx else 64
+        s_dir += u * fy * (fx // 2 + 1) * 8
+        bins += fy * (fx // 2 + 1)
+    mac_avg_ms = mac_ms / max(mac_n, 1)
+    # flops of the learned step's MAC launches: Htf (T frames), the guard's
+    # pair (2T frames) and one forward + backward (2T frames) per iteration
+    nproj_frames = T + 2 * T + 2 * T * out["n"]
+    mac_flops = 8.0 * hm.Nnum ** 2 * bins * nproj_frames * args.steps
+    flop_rate = mac_flops / (mac_ms / 1e3)
+    hbm_rate = (mac_bytes / max(mac_n, 1)) / (mac_avg_ms / 1e3)
+    it_bytes = 2 * s_dir + T * (4 * nz + 2) * Y * X * 4
+    it_s = (plain_ms / 1e3) / (iters + 0.5)
+    one = lambda a, f=float: [f(x) for x in a]
+    desc = ("%s: %d frames of Nnum=%d, %dx%d px, %d planes (z=-%d..%d), PSF %d..%d px, %d..%d seeded blobs per "
+            "frame (seed %d), Poisson peak %g; target = each frame's data fit after %d plain RL iterations from a "
+            "uniform start" % (args.workload, T, w["N"], Y, X, nz, w["zs"][-1], w["zs"][-1], min(w["sides"]),
+                               max(w["sides"]), w["nblobs"][0], w["nblobs"][1], w["seed"], w["peak"], iters))
+    summ = lambda a: {"min": float(np.min(a)), "median": float(np.median(a)), "max": float(np.max(a))}
+    res = {
+        "metric": "volumes/s at fixed RL data-fit: learned-init vs plain RL, 1 GPU",
+        "value": T * 1e3 / learned_ms,
+        "unit": "volumes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": learned_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": desc, "frames": T, "launch_options": tuned, "min_iterations": args.min_iter,
+            "plain": {"ms": plain_ms, "volumes_per_s": T * 1e3 / plain_ms, "iterations": iters,
+                      "datafit": one(fit_plain_t), "rel_l2_vs_truth": one(gt_plain),
+                      "rel_l2_vs_truth_summary": summ(gt_plain)},
+            "learned": {"ms": learned_ms, "volumes_per_s": T * 1e3 / learned_ms,
+                        "iterations_run": int(out["n"]), "iterations_needed_per_frame": need,
+                        "iterations_saved_per_frame": [iters - k if k >= 0 else None for k in need],
+                        "guess_accepted": one(info["accepted"], bool),
+                        "guard_margin_per_frame": one(np.asarray(info["fit_plain1"]) - np.asarray(info["fit_guess"])),
+                        "fit_guess": one(info["fit_guess"]), "fit_plain1": one(info["fit_plain1"]),
+                        "datafit_reached": one(fit_l_t), "rel_l2_vs_truth": one(gt_learned),
+                        "rel_l2_vs_truth_summary": summ(gt_learned), "rel_l2_vs_plain": one(rel_vs_plain),
+                        "weights": os.path.relpath(wpath, ROOT)},
+            "speedup_vs_plain": plain_ms / learned_ms,
+            "one_frame": {"plain_ms": plain1_ms, "learned_ms": learned1_ms,
+                          "plain_volumes_per_s": 1e3 / plain1_ms, "learned_volumes_per_s": 1e3 / learned1_ms,
+                          "speedup": plain1_ms / learned1_ms},
+        },
+        "roofline": {"bound": "fp32", "achieved": flop_rate / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                     "frac": flop_rate / PEAK_FP32, "traffic": measured_traffic(args.workload), "kernel": "k_mac",
+                     "launch_avg_ms": mac_avg_ms, "launches": mac_n,
+                     "flops_per_launch": mac_flops / max(mac_n, 1),
+                     "note": "several frames per workgroup: the MAC is bound by FP32 FMA issue, not HBM",
+                     "hbm": {"achieved": hbm_rate / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                             "frac": hbm_rate / PEAK_HBM, "bytes_per_launch": mac_bytes / max(mac_n, 1)}},
+        "rl_iteration": {"ms": it_s * 1e3, "frames": T, "algorithmic_bytes": it_bytes,
+                         "achieved_GBps": it_bytes / it_s / 1e9, "frac": it_bytes / it_s / PEAK_HBM},
+        "spectra_precompute_s": t_prep,
+    }
+    del X1s
+    if not args.no_oracle_check:
+        # the learned leg's starting volume for frame 0 (guess or plain iterate)
+        g0, _, _ = li.initial_volume(im0, li.projector.backward_device(hm, im0))
+        t = time.time()
+        res["oracle_check"] = oracle_check(proj, hm, im0[0], g0[:, 0])
+        log("oracle check %.1fs: %s" % (time.time() - t, res["oracle_check"]["rel_l2_per_plane"]))
+    if not args.no_cpu_baseline:
+        t = time.time()
+        wb = dict(hm=hm, Y=Y, X=X, N=w["N"], sides=w["sides"], zs=w["zs"], iters=iters,
+                  vol=vol[:, :1].cpu().numpy())
+        res["cpu_baseline"] = cpu_baseline(wb)
+        log("cpu baseline %.1fs" % (time.time() - t))
+    print(json.dumps(res), flush=True)
+
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rl1455", choices=sorted(WORKLOADS) + sorted(RAGGED))
+    ap.add_argument("--workload", default="rl1455x32", choices=sorted(WORKLOADS) + sorted(RAGGED) + sorted(BATCHES))
     ap.add_argument("--buckets", type=int, default=2, help="ragged workloads: canvas sizes (at most)")
     ap.add_argument("--graph", action="store_true", help="ragged workloads: replay iterations as HIP graphs")
     ap.add_argument("--weights", default=None, help="learned-initialiser weights (default: the packaged file)")
@@ -267,6 +522,9 @@ def main():
                     help="frames reconstructed together per step (value = frames x steps / time)")
     ap.add_argument("--plain-only", action="store_true", help="time plain RL only")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-oracle-check", action="store_true", help="batch workloads: skip the FP64 oracle check")
+    ap.add_argument("--min-iter", type=int, default=0,
+                    help="batch workloads: RL iterations the learned leg runs before its target may stop it")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (reference kernel for HBM counters)")
@@ -278,6 +536,8 @@ def main():
         sys.exit(2)
     if args.workload in RAGGED:
         return ragged_main(args)
+    if args.workload in BATCHES:
+        return batch_main(args)
 
     import torch
     from flfhip.lfdeconv import deconv_rl_device, _fit_from_sums

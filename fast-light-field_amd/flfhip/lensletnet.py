@@ -333,12 +333,14 @@ class LearnedInit(object):
             Hg[t] = HX1[t]
         return g, Hg, {"accepted": accept, "fit_guess": fit_g, "fit_plain1": fit_1}
 
-    def deconvolve(self, im, Htf, maxIter, targetFit=None, fitOut=None, Htf_out=None):
+    def deconvolve(self, im, Htf, maxIter, targetFit=None, fitOut=None, Htf_out=None, minIter=0):
         """Guarded learned start, then Prevedel RL until every frame's fit is
-        <= targetFit (or maxIter iterations).  Returns (X, iterations, info);
-        a rejected frame's plain iterate counts as one iteration.  Htf None:
-        Htf = Ht(im) is computed here (into Htf_out if given) beside the
-        network's guess (guess_and_htf) and returned as info["Htf"]."""
+        <= targetFit (or maxIter iterations), after at least minIter
+        iterations.  Returns (X, iterations, info); a rejected frame's plain
+        iterate counts as one iteration.  Htf None: Htf = Ht(im) is computed
+        here (into Htf_out if given) beside the network's guess
+        (guess_and_htf) and returned as info["Htf"].  fitOut receives the fit
+        of the starting volume and of every later iterate (per frame)."""
         from .lfdeconv import deconv_rl_device
         guess = None
         if Htf is None:
@@ -347,6 +349,6 @@ class LearnedInit(object):
         info["Htf"] = Htf
         fit0 = np.where(info["accepted"], info["fit_guess"], info["fit_plain1"])   # fit of the starting X
         n = deconv_rl_device(self.projector, self.hMatrix, Htf, maxIter, X, im=im, targetFit=targetFit,
-                             fitOut=fitOut, HX0=HX, fit0=fit0)
+                             fitOut=fitOut, HX0=HX, fit0=fit0, minIter=minIter)
         info["iterations"] = n + (0 if info["accepted"].all() else 1)
         return X, info["iterations"], info

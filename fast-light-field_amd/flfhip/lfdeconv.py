@@ -65,7 +65,7 @@ def _fit_from_sums(sums):
 
 
 def deconv_rl_device(projector, hMatrix, Htf, maxIter, X, im=None, prevedel=True, trust=None, targetFit=None,
-                     fitOut=None, planes=None, HX0=None, fit0=None):
+                     fitOut=None, planes=None, HX0=None, fit0=None, minIter=0):
     """Device-resident RL iterations.  Htf, X (numZ, T, Y, X) and im (T, Y, X)
     are float32 tensors on the projector's device; X is updated in place.
     Returns the number of iterations run.  With targetFit (array-like per
@@ -75,7 +75,8 @@ def deconv_rl_device(projector, hMatrix, Htf, maxIter, X, im=None, prevedel=True
     given, is the forward projection of the starting X (saves one); fit0,
     with HX0, is its per-frame data fit (saves one reduction).  The standard
     update (prevedel=False) takes its error image im / HX from the pass that
-    writes HX (flf_forward_ratio) and ignores HX0."""
+    writes HX (flf_forward_ratio) and ignores HX0.  minIter: iterations run
+    before the target may stop the loop."""
     T, Y, Xw = X.shape[1:]
     projector._need(X, "X", (hMatrix.numZ, T, Y, Xw))
     if prevedel:
@@ -105,7 +106,7 @@ def deconv_rl_device(projector, hMatrix, Htf, maxIter, X, im=None, prevedel=True
             fit = np.asarray(known, np.float64) if known is not None else _fit_from_sums(projector.datafit_device(HX, im))
             if fitOut is not None:
                 fitOut.append(fit)
-            if target is not None and np.all(fit <= target):
+            if target is not None and it >= minIter and np.all(fit <= target):
                 break
         if it >= maxIter:
             break

@@ -11,6 +11,13 @@ drifting 0.5 px per frame (synthetic.drifting_blob_frames).
 
     name       frames  Nnum  lenslets  planes      PSF sides  blobs/frame  peak  RL iterations
     ragged64   64      15    20..60    z=-15..15   15..255    50..300      1000  8
+
+Batches of full-size frames (BATCHES): T frames of one size reconstructed
+together, each a seeded blob volume with its own blob count, on the PSF set
+of a base workload (so that a network trained for it applies).
+
+    name        base    frames  blobs/frame  peak  RL iterations  seed
+    rl1455x32   rl1455  32      100..800     2000  8              6
 """
 import sys
 import time
@@ -21,6 +28,12 @@ WORKLOADS = {
     # name: (Nnum, lenslets, zmax, s_min, s_max, blobs, peak, iters, seed)
     "rl1455": (15, 97, 20, 15, 345, 400, 2000.0, 8, 2),
     "rl300": (15, 20, 5, 15, 135, 200, 1000.0, 4, 1),
+}
+
+
+BATCHES = {
+    # name: (base workload, frames, blobs_min, blobs_max, peak, iters, seed)
+    "rl1455x32": ("rl1455", 32, 100, 800, 2000.0, 8, 6),
 }
 
 
@@ -83,3 +96,16 @@ def setup_ragged(name, volume=True):
         _log("volumes: %d frames, %.1fs" % (T, time.time() - t0))
     return dict(name=name, N=N, zs=zs, sides=sides, hm=hm, shapes=shapes, nblobs=nblobs, vols=vols, peak=peak,
                 iters=iters, seed=seed)
+
+
+def setup_batch(name):
+    """Batch workload: the base workload's PSF set and geometry, and the
+    per-frame parameters; the volumes are made on the GPU by
+    lensletnet.blob_volumes_gpu(nz, Y, X, frames, (blobs_min, blobs_max),
+    seed) (frame t's blob count is the t-th draw of that seeded generator)."""
+    base, T, bmin, bmax, peak, iters, seed = BATCHES[name]
+    N, L = WORKLOADS[base][0], WORKLOADS[base][1]
+    hm, zs, sides = psf_set(base)
+    Y = X = N * L
+    return dict(name=name, base=base, N=N, Y=Y, X=X, zs=zs, sides=sides, hm=hm, frames=T, nblobs=(bmin, bmax),
+                peak=peak, iters=iters, seed=seed)

@@ -25,3 +25,25 @@ def test_workloads_and_traffic_summary():
     t = bench.measured_traffic("rl1455")
     assert t is not None and 1e9 < t < 2e10                 # bytes per k_mac launch, from profiles/
     assert bench.measured_traffic("no_such_workload") is None
+
+
+def test_batch_workload_is_the_32_frame_configuration():
+    # 32 frames of 1455 x 1455 (Nnum 15, 41 planes, PSF up to 345 px), 100-800
+    # blobs per frame, Poisson peak 2000, seed 6, 8 plain iterations as target
+    sys.path.insert(0, ROOT)
+    import bench
+    from flfhip.workloads import BATCHES, WORKLOADS
+    base, T, bmin, bmax, peak, iters, seed = BATCHES["rl1455x32"]
+    assert (base, T, bmin, bmax, peak, iters, seed) == ("rl1455", 32, 100, 800, 2000.0, 8, 6)
+    assert WORKLOADS[base][1] * WORKLOADS[base][0] == 1455 and WORKLOADS[base][4] == 345
+    import argparse
+    src = open(os.path.join(ROOT, "bench.py")).read()
+    assert 'default="rl1455x32"' in src                     # the default bench line is this batch
+
+
+def test_iterations_to_target():
+    sys.path.insert(0, ROOT)
+    import bench
+    fits = [[0.5, 0.2], [0.3, 0.15], [0.1, 0.14]]
+    assert bench._iters_to_target(fits, [0.3, 0.2]) == [1, 0]
+    assert bench._iters_to_target(fits, [0.05, 0.14]) == [-1, 2]

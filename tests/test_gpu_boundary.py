@@ -160,7 +160,7 @@ def test_ragged_cached_graph_after_subset_call(projector):
     from flfhip.ragged import RaggedBatch
     hm = _hm()
     shapes = [(60, 60), (90, 75), (75, 90), (45, 60)]
-    vols = [synthetic.blob_volume(3, y, x, 10, seed=40 + i) for i, (y, x) in enumerate(shapes)]
+    vols = [synthetic.blob_volume(3, y, x, 10, seed=40 + i)[:, 0] for i, (y, x) in enumerate(shapes)]
     rb = RaggedBatch(hm, shapes, nbuckets=2, projectors=[projector])
     clean = rb.forward(vols)
     ims = [synthetic.poisson_image(c[None], 500.0, 7 + i)[0] for i, c in enumerate(clean)]

@@ -48,9 +48,9 @@ def test_guard_rejects_bad_guess_and_matches_plain(projector):
     net.set_params(p)
     li = LearnedInit(net, projector, hm)
     Xp, fits_p = _plain(projector, hm, Htf, im, 4)
-    # The guard projects guess and plain iterate as one two-frame call (the
-    # matrix-core MAC), the plain run one frame at a time: the same values up
-    # to FP32 summation order, hence the 1e-6 margins.
+    # The guard projects guess and plain iterate as one two-frame call (two
+    # frames per MAC workgroup), the plain run one frame at a time: the same
+    # values up to FP32 summation order, hence the 1e-6 margins.
     Xl, n, info = li.deconvolve(im, Htf, 4, targetFit=fits_p[4] * (1 + 1e-6))
     print("guard", info, "plain fits", fits_p, flush=True)
     assert not info["accepted"][0]
