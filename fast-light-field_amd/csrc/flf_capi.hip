@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -98,9 +99,51 @@ struct flf_ctx {
     std::vector<ProfRec> prof_recs;
 };
 
+// ---------------------------------------------------------------------------
+// Scratch of flf_datafit_noalloc: the per-block partial sums of the data fit
+// (2 doubles per block and frame), in a file-scope device array instead of a
+// buffer the library allocates.  Each GPU holds its own copy, from module
+// load until the process ends.  In host code `scratch` names a host-side
+// stand-in, never the device memory: kernels get the address that
+// hipGetSymbolAddress returns for the current device (noalloc_scratch).
+// ---------------------------------------------------------------------------
+#define FLF_NOALLOC_FIT_BLOCKS 256
+#define FLF_NOALLOC_MAX_FRAMES 512
+#define FLF_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) double scratch[2 * FLF_NOALLOC_FIT_BLOCKS * FLF_NOALLOC_MAX_FRAMES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[FLF_NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on `device`, looked up once per device with
+// that device current (the caller's current device is restored).
+hipError_t noalloc_scratch(int device, double** out) {
+    *out = nullptr;
+    if (device < 0 || device >= FLF_NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = (double*)p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+}  // namespace
+
 namespace {
 
 const int FIT_BLOCKS = 256;
+static_assert(FIT_BLOCKS == FLF_NOALLOC_FIT_BLOCKS, "the no-allocation data fit shares flf_datafit's reduction");
 const int FWD_SPLIT_MAX = 4;
 const int GROUP_STREAMS_MAX = 3;
 const int FFT_THREADS = 512;
@@ -504,6 +547,11 @@ int flf_create(int device, flf_ctx** out) {
         int v = atoi(e);
         if (v >= 1 && v <= 4) c->fb_max = v;
     }
+    // set-up of the no-allocation data fit's scratch address on this device
+    // (not required by the context: on failure flf_datafit_noalloc retries
+    // and reports the error itself)
+    double* sp = nullptr;
+    if (noalloc_scratch(device, &sp) != hipSuccess) (void)hipGetLastError();
     *out = c;
     return FLF_OK;
 }
@@ -1065,6 +1113,42 @@ int flf_check_device_buffer(int device, const void* p, long long bytes) {
     if (!device_span(device, p, (size_t)bytes, "buffer", why)) {
         tls_error() = why;
         return FLF_EINVAL;
+    }
+    return FLF_OK;
+}
+
+int flf_datafit_noalloc(const float* img, const float* im, int T, int im_frames, long long npix, double* out,
+                        void* stream) {
+    tls_error().clear();
+    if (T < 1 || T > FLF_NOALLOC_MAX_FRAMES) {
+        tls_error() = "flf_datafit_noalloc: T must be 1.." + std::to_string(FLF_NOALLOC_MAX_FRAMES);
+        return FLF_EINVAL;
+    }
+    if (im_frames < 1 || im_frames > T || npix < 1) {
+        tls_error() = "flf_datafit_noalloc: im_frames must be 1..T and npix >= 1";
+        return FLF_EINVAL;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return FLF_EHIP;
+    std::string why;
+    if (!device_span(dev, img, (size_t)T * npix * sizeof(float), "image", why) ||
+        !device_span(dev, im, (size_t)im_frames * npix * sizeof(float), "im", why) ||
+        !device_span(dev, out, (size_t)2 * T * sizeof(double), "out", why)) {
+        tls_error() = why;
+        return FLF_EINVAL;
+    }
+    double* part = nullptr;
+    hipError_t e = noalloc_scratch(dev, &part);
+    if (e != hipSuccess) {
+        tls_error() = std::string("scratch address: ") + hipGetErrorString(e);
+        return FLF_EHIP;
+    }
+    // every partial that k_fit_final reads is written by k_fit_partial in this
+    // call, so the scratch needs no clearing
+    e = launch_fit(img, im, npix, T, im_frames, part, FIT_BLOCKS, out, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        tls_error() = std::string("flf_datafit_noalloc: ") + hipGetErrorString(e);
+        return FLF_EHIP;
     }
     return FLF_OK;
 }

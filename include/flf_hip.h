@@ -168,6 +168,20 @@ int flf_backward_rl_update(flf_ctx* ctx, const float* img, float* X, const float
  * driver.  Synchronises `stream`. */
 int flf_datafit(flf_ctx* ctx, const float* img, const float* im, int im_frames, double* out_host, void* stream);
 
+/* The data fit of flf_datafit without any allocation, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No reference counterpart.
+ * img: T frames of npix floats, im: im_frames frames (frame t of img against
+ * frame t % im_frames), out: 2*T doubles in DEVICE memory, written on
+ * `stream` (no synchronisation, no host copy): out[2t] = sum((img-im)^2),
+ * out[2t+1] = sum(im^2), bit-identical to flf_datafit.  T <= 512.  The
+ * partial sums live in one fixed 2 MiB device array per GPU (held until the
+ * process ends) that every call on that GPU shares: calls that overlap on
+ * one GPU (several streams or threads) race and must be serialised by the
+ * caller.  Arrays on the current HIP device; errors via flf_last_error(NULL). */
+int flf_datafit_noalloc(const float* img, const float* im, int T, int im_frames, long long npix, double* out,
+                        void* stream);
+
 /* X *= Htf / v, NaN -> 0, on `stream` (device of the current HIP device):
  * the Prevedel RL update of lfdeconv.py:112-126 with a denominator the
  * caller already holds.  X and Htf are (nz, T, npix); v is (nz, v_frames,
