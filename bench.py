@@ -332,12 +332,27 @@ def batch_main(args):
     t_prep = time.time() - t
     log("spectra precompute: %.2fs, device bytes %.2f GB" % (t_prep, proj.lib.flf_device_bytes(proj._ctx) / 1e9))
     tuned = None
-    if not args.no_autotune:
+    if args.options:
+        tuned = {}
+        for kv in args.options.split(","):
+            k, v = kv.split("=")
+            proj.set_option(k, int(v))
+            tuned[k] = int(v)
+        log("launch options (given): %s" % tuned)
+    elif not args.no_autotune:
         from flfhip import autotune
         t = time.time()
         tuned, _ = autotune.tune(proj, hm, T, Y, X)
         log("autotune %.1fs: %s" % (time.time() - t, tuned))
     dev = proj.device
+    if args.calib_copy_mib:
+        # one device copy of known size: the reference kernel for the HBM counters
+        n = args.calib_copy_mib * (1 << 20) // 4
+        a_ = torch.ones(n, dtype=torch.float32, device=dev)
+        b_ = torch.empty_like(a_)
+        b_.copy_(a_)
+        torch.cuda.synchronize()
+        del a_, b_
     t = time.time()
     vol = lensletnet.blob_volumes_gpu(nz, Y, X, T, w["nblobs"], w["seed"], dev)
     proj.prepare(hm, T, Y, X)
@@ -526,6 +541,8 @@ def main():
     ap.add_argument("--min-iter", type=int, default=0,
                     help="batch workloads: RL iterations the learned leg runs before its target may stop it")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--options", default="",
+                    help="batch workloads: launch options name=value,... instead of the autotuner (for profiler runs)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (reference kernel for HBM counters)")
     args = ap.parse_args()

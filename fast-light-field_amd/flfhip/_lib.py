@@ -68,6 +68,12 @@ SIGNATURES = {
     "flf_check_device_buffer": (_c.c_int, [_c.c_int, _c.c_void_p, _c.c_longlong]),
     "flf_datafit_noalloc": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_int, _c.c_int, _c.c_longlong, _c.c_void_p,
                                        _c.c_void_p]),
+    "flf_psf_line": (_c.c_int, [_c.c_void_p, _c.c_double, _c.c_int, _c.c_void_p, _c.c_void_p]),
+    "flf_psf_wave": (_c.c_int, [_c.c_void_p, _c.c_double, _c.c_int, _c.c_int, _c.c_void_p, _c.c_void_p]),
+    "flf_psf_lightfield": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_int, _c.c_int, _c.c_int, _c.c_void_p,
+                                      _c.c_void_p]),
+    "flf_psf_normalise": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_int, _c.c_double, _c.c_int, _c.c_void_p,
+                                     _c.c_void_p]),
     "flf_profile_enable": (_c.c_int, [_c.c_void_p, _c.c_int]),
     "flf_profile_read": (_c.c_int, [_c.c_void_p, _c.POINTER(_c.c_double), _c.POINTER(_c.c_longlong),
                                     _c.POINTER(_c.c_double)]),

@@ -257,6 +257,42 @@ int flf_net_upsample(const float* g, int T, int nz, int Ly, int Lx, int nnum, fl
  * GEMM the network uses, exposed for its layout test. */
 int flf_mfma_gemm_nt(const void* A, const void* B, float* C, int M, int N, int K, void* stream);
 
+/* ---- Light-field PSF synthesis (GeneratePSF, generate_psf.py:63-798) ----
+ * The wave-optics model of the reference's PSF generator in FP64 kernels,
+ * called per plane by flfhip.psfgen (which holds the host-side bookkeeping of
+ * the generator's grids and windows).  Arrays are device memory of the
+ * current HIP device; errors via flf_last_error(NULL).  Each call
+ * synchronises `stream`.  Optics in SI units, the GeneratePSF arguments. */
+typedef struct flf_psf_optics {
+    double M, NA, MLPitch, n, fml, lam;
+    int Nnum, OSR;                 /* both odd */
+} flf_psf_optics;
+
+/* |Koi I0|^2 (the intensity of the Debye field) at the npts points x1 =
+ * j * MLPitch / Nnum / OSR, x2 = 0 of the plane at depth p3: the profile
+ * calcPSFFT computes to size the PSF (generate_psf.py:103-142). */
+int flf_psf_line(const flf_psf_optics* o, double p3, int npts, double* out, void* stream);
+
+/* The wave PSF of the plane at depth p3 on the oversampled L x L grid, L =
+ * 2 * half * OSR + 1 (half = IMG_HALFWIDTH): Debye integrals on rows lo..C-1,
+ * columns row..C-1 (C = (L+1)/2), completed by the mirror and rotations of
+ * calcPSF_p (generate_psf.py:310-350).  pattern: L*L complex (interleaved
+ * re, im doubles). */
+int flf_psf_wave(const flf_psf_optics* o, double p3, int half, int lo, double* pattern, void* stream);
+
+/* The light-field PSFs of every lenslet offset (a, b) of the quadrant for
+ * one plane (ComputeLFpsf, generate_psf.py:533-567) from its wave PSF:
+ * H[b][a][q][p] (h*h*s2*s2 doubles, the HMatrix quadrant layout, s2 = 2 *
+ * Nnum * imgsize_ref + 1), unnormalised.  lo: the wave PSF's first nonzero
+ * row (as passed to flf_psf_wave); the grid needs L <= 4095. */
+int flf_psf_lightfield(const flf_psf_optics* o, const double* pattern, int half, int imgsize_ref, int lo,
+                       double* H, void* stream);
+
+/* One plane's normalisation (generate_psf.py:594-613): values below
+ * clip * (the plane's maximum) become 0, then (per_psf) each of the h*h PSFs
+ * is divided by its sum; out: h*h*s2*s2 float32. */
+int flf_psf_normalise(const double* H, int h, int s2, double clip, int per_psf, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
