@@ -496,11 +496,13 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
     if (rj) return rj;
     // every plane's row transform (and RL update) in one launch on st
     std::vector<int> ids;
-    int wbuf = 0;
+    int wbuf = 0, prime = 0;
     for (int z = 0; z < ctx->nz; ++z)
         if (sel[z]) {
             ids.push_back(z);
-            wbuf = std::max(wbuf, c2r_wave_buffer(ctx->groups[ctx->group_of[z]].gt.fX));
+            const Group& gz = ctx->groups[ctx->group_of[z]];
+            wbuf = std::max(wbuf, c2r_wave_buffer(gz.gt.fX));
+            if (plan_has_prime(gz.rpXw) || plan_has_prime(gz.rpXhw)) prime = 1;
         }
     if (ids.empty()) return FLF_OK;
     IfftRowPlaneArgs r{};
@@ -510,6 +512,7 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
     r.vol = vol; r.Htf = Htf; r.trust = trust;
     r.mode = mode; r.T = ctx->T; r.Y = ctx->Y; r.X = ctx->X;
     r.wbuf = wbuf;
+    r.prime = prime;
     r.fdims = frame_dims(ctx);
     HIPCHK(launch_ifft_rows_plane(r, (int)ids.size(), st));
     return FLF_OK;
@@ -604,7 +607,8 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
                 !make_radix_plan(g.gt.rY, g.rpYr) || !make_radix_plan(g.gt.rX, g.rpXr) ||
                 !make_radix_plan(fY, g.rpYw, true) || !make_radix_plan(fX, g.rpXw, true) ||
                 (fX % 2 == 0 && (!make_radix_plan(fX / 2, g.rpXh) || !make_radix_plan(fX / 2, g.rpXhw, true))))
-                return fail(ctx, FLF_EINVAL, "Fourier grid is not 5-smooth");
+                return fail(ctx, FLF_EINVAL, "Fourier grid length has a prime factor other than 2, 3, 5 and one odd "
+                                             "prime up to 31");
             gi = (int)ctx->groups.size() - 1;
         }
         ctx->group_of[z] = gi;

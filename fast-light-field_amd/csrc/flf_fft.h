@@ -16,6 +16,11 @@ namespace flf {
 // Largest radix of a pass (wave-level transforms; the block-level passes
 // below use 2..5).
 constexpr int FLF_RADIX_MAX = 8;
+// Largest odd prime radix (a prime factor of Nnum above 5, Nnum <= 31).  Such
+// a radix is always the last pass of a plan: then Ns * P = L, every
+// butterfly writes its outputs onto its own inputs, and a thread can take its
+// butterflies one at a time.
+constexpr int FLF_PRIME_MAX = 31;
 
 template <int R>
 __device__ __forceinline__ void dft_small(float2* a, float sign);
@@ -171,8 +176,53 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int L, int nbatch, in
     __syncthreads();
 }
 
-// Full transform: sign = -1 forward, +1 inverse (unnormalised).
-template <int MAXB>
+// Last pass of an odd prime radix P (5 < P <= FLF_PRIME_MAX; Ns = L / P):
+// for butterfly j < M = L/P, y_q = sum_t x[j + t M] W_L^(j t) W_P^(t q),
+// written back to x[j + q M].  Twiddles by sincospif of exactly reduced
+// arguments (j t < L, t q mod P).
+__device__ __forceinline__ void stockham_pass_prime(float2* buf, int L, int P, int nbatch, int es, int bs,
+                                                    float sign) {
+    const int M = L / P;
+    const int total = nbatch * M;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int b = idx / M;
+        const int j = idx - b * M;
+        float2* base = buf + (long)b * bs;
+        float2 a[FLF_PRIME_MAX];
+#pragma unroll
+        for (int t = 0; t < FLF_PRIME_MAX; ++t) {
+            if (t < P) {
+                float2 v = base[(j + t * M) * es];
+                if (t > 0 && j > 0) {
+                    float s, c;
+                    sincospif(sign * 2.0f * (float)(j * t) / (float)L, &s, &c);
+                    v = cmul(v, make_float2(c, s));
+                }
+                a[t] = v;
+            }
+        }
+        for (int q = 0; q < P; ++q) {
+            float2 y = a[0];
+            int m = 0;
+#pragma unroll
+            for (int t = 1; t < FLF_PRIME_MAX; ++t) {
+                if (t < P) {
+                    m += q;
+                    if (m >= P) m -= P;
+                    float s, c;
+                    sincospif(sign * 2.0f * (float)m / (float)P, &s, &c);
+                    y = cadd(y, cmul(a[t], make_float2(c, s)));
+                }
+            }
+            base[(j + q * M) * es] = y;
+        }
+    }
+    __syncthreads();
+}
+
+// Full transform: sign = -1 forward, +1 inverse (unnormalised).  PRIME: the
+// plan may end with an odd prime radix above 5.
+template <int MAXB, bool PRIME = false>
 __device__ void lds_fft(float2* buf, const RadixPlan& rp, int nbatch, int es, int bs, float sign) {
     int Ns = 1;
     for (int p = 0; p < rp.n; ++p) {
@@ -181,7 +231,10 @@ __device__ void lds_fft(float2* buf, const RadixPlan& rp, int nbatch, int es, in
             case 4: stockham_pass<4, MAXB>(buf, rp.L, nbatch, es, bs, Ns, sign); break;
             case 2: stockham_pass<2, MAXB>(buf, rp.L, nbatch, es, bs, Ns, sign); break;
             case 3: stockham_pass<3, MAXB>(buf, rp.L, nbatch, es, bs, Ns, sign); break;
-            default: stockham_pass<5, MAXB>(buf, rp.L, nbatch, es, bs, Ns, sign); break;
+            case 5: stockham_pass<5, MAXB>(buf, rp.L, nbatch, es, bs, Ns, sign); break;
+            default:
+                if constexpr (PRIME) stockham_pass_prime(buf, rp.L, R, nbatch, es, bs, sign);   // last pass
+                break;
         }
         Ns *= R;
     }

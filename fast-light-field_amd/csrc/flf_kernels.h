@@ -142,6 +142,7 @@ struct IfftRowPlaneArgs {
     int T, Y, X;
     int wbuf;                    // per-wave LDS buffer (float2): max over planes of fX/2 (even) or fX
     const int2* fdims;           // per frame valid size; outside it the volume is set to 0
+    int prime;                   // some plane's row plan ends with an odd prime radix above 5
 };
 
 namespace flf {
@@ -171,6 +172,8 @@ hipError_t launch_fill(float* p, long long n, float v, hipStream_t s);
 // (v_frames = T).
 hipError_t launch_rl_scale(float* X, const float* Htf, const float* v, int nz, int T, long long npix, int v_frames,
                            hipStream_t s, bool ratio = false);
+// Whether a radix plan ends with an odd prime radix above 5.
+bool plan_has_prime(const RadixPlan& rp);
 // Raise the dynamic-LDS limit of the FFT kernels to max_lds bytes.
 hipError_t configure_kernels(size_t max_lds);
 }  // namespace flf

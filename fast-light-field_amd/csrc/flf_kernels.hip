@@ -179,6 +179,7 @@ __global__ __launch_bounds__(64 * SFFT_WAVES) void k_small_fft(SmallFFTArgs a) {
 // the same pruned 2-D transform column-first on the full plane.)
 // grid (ceil(s2/RB), nu); LDS RB*fX float2.
 // ---------------------------------------------------------------------------
+template <bool PRIME>
 __global__ __launch_bounds__(FFT_THREADS) void k_psf_rows(PsfRowArgs a) {
     extern __shared__ float2 sh[];
     const int u = blockIdx.y;
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_psf_rows(PsfRowArgs a) {
         sh[i] = make_float2(v, 0.f);
     }
     __syncthreads();
-    lds_fft<FFT_MAXB>(sh, a.rpX, nrows, 1, fX, -1.f);
+    lds_fft<FFT_MAXB, PRIME>(sh, a.rpX, nrows, 1, fX, -1.f);
     float2* out = a.tmp + (long long)u * s2 * rfX;
     for (int i = threadIdx.x; i < nrows * rfX; i += blockDim.x) {
         int r = i / rfX, k1 = i - r * rfX;
@@ -208,6 +209,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_psf_rows(PsfRowArgs a) {
 // PSF spectra, stage 2: column FFTs (length fY, input rows >= s2 are zero).
 // grid (ceil(rfX/CB), nu); LDS CB*fY float2 (column-major).
 // ---------------------------------------------------------------------------
+template <bool PRIME>
 __global__ __launch_bounds__(FFT_THREADS) void k_psf_cols(PsfColArgs a) {
     extern __shared__ float2 sh[];
     const int u = blockIdx.y;
@@ -222,7 +224,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_psf_cols(PsfColArgs a) {
         sh[c * fY + y] = v;
     }
     __syncthreads();
-    lds_fft<FFT_MAXB>(sh, a.rpY, a.CB, 1, fY, -1.f);
+    lds_fft<FFT_MAXB, PRIME>(sh, a.rpY, a.CB, 1, fY, -1.f);
     float2* out = a.spec + (long long)u * fY * rfX;
     for (int i = threadIdx.x; i < fY * a.CB; i += blockDim.x) {
         int y = i / a.CB, c = i - y * a.CB;
@@ -873,6 +875,7 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void k_mac_mfma(MacArgs a) {
 // grid (ceil(rfX/CB), nbatch), one wave per column (blockDim = 64*CB); LDS
 // CB*fY + fY (twiddles).
 // ---------------------------------------------------------------------------
+template <bool PRIME>
 __global__ __launch_bounds__(512) void k_ifft_cols(IfftColArgs a) {
     extern __shared__ float2 lds[];
     const int fY = a.rpY.L, rfX = a.rfX, CB = a.CB;
@@ -909,7 +912,7 @@ __global__ __launch_bounds__(512) void k_ifft_cols(IfftColArgs a) {
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (wv < ncol) wave_fft<2048>(C + wv * fY, 1, a.rpY, tw, true, lane);
+    if (wv < ncol) wave_fft<2048, PRIME>(C + wv * fY, 1, a.rpY, tw, true, lane);
     __syncthreads();
     float2* out = a.out + (long long)blockIdx.y * a.out_bstride;
     for (int i = threadIdx.x; i < a.Yout * CB; i += blockDim.x) {
@@ -926,6 +929,7 @@ __global__ __launch_bounds__(512) void k_ifft_cols(IfftColArgs a) {
 // extension and a full-length transform.  The imaginary parts of X[0] and
 // X[L/2] are dropped, as a c2r transform does.  Returns with the result in
 // buf (interleaved real samples for even L, real parts for odd L).
+template <bool PRIME>
 __device__ __forceinline__ void wave_c2r(float2* buf, const float2* __restrict__ row, int L,
                                          const RadixPlan& rph, const float2* twh,
                                          const RadixPlan& rpf, const float2* twf, int lane) {
@@ -960,7 +964,7 @@ __device__ __forceinline__ void wave_c2r(float2* buf, const float2* __restrict__
             }
         }
         wave_sync();
-        wave_fft<1024>(buf, 1, rph, twh, true, lane);
+        wave_fft<1024, PRIME>(buf, 1, rph, twh, true, lane);
     } else {
         const int rfX = L / 2 + 1;
         for (int k0 = lane; k0 < L; k0 += 64 * B) {
@@ -981,7 +985,7 @@ __device__ __forceinline__ void wave_c2r(float2* buf, const float2* __restrict__
             }
         }
         wave_sync();
-        wave_fft<2048>(buf, 1, rpf, twf, true, lane);
+        wave_fft<2048, PRIME>(buf, 1, rpf, twf, true, lane);
     }
 }
 
@@ -996,6 +1000,7 @@ __device__ __forceinline__ float c2r_sample(const float2* buf, int L, int x) {
 // Forward projection, stage 2: per output row, c2r each grid group's row and
 // sum the groups in a fixed order.  One wave per (row, frame);
 // grid (ceil(Y/4), T), block 256.
+template <bool PRIME>
 __global__ __launch_bounds__(256) void k_ifft_rows_sum(IfftRowSumArgs a) {
     extern __shared__ float2 lds[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1018,7 +1023,7 @@ __global__ __launch_bounds__(256) void k_ifft_rows_sum(IfftRowSumArgs a) {
         __syncthreads();
         if (y < a.Y) {
             const float2* row = G.T + (long long)t * G.tstride + (long long)y * G.rfX;
-            wave_c2r(buf, row, L, G.rph, twh, G.rp, twf, lane);
+            wave_c2r<PRIME>(buf, row, L, G.rph, twh, G.rp, twf, lane);
             const float sc = G.scale;
 #pragma unroll
             for (int q = 0; q < XPL; ++q) {
@@ -1056,6 +1061,7 @@ __global__ __launch_bounds__(256) void k_ifft_rows_sum(IfftRowSumArgs a) {
 // X *= v [/ trust]; NaN -> 0 after the update).  One wave per row;
 // grid (ceil(Y/4), T, nplanes), block 256.  Twiddles of every grid group are
 // read from global memory (they stay in the caches).
+template <bool PRIME>
 __global__ __launch_bounds__(256) void k_ifft_rows_plane(IfftRowPlaneArgs a) {
     extern __shared__ float2 lds[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1066,7 +1072,7 @@ __global__ __launch_bounds__(256) void k_ifft_rows_plane(IfftRowPlaneArgs a) {
     float2* buf = lds + wv * a.wbuf;
     if (y >= a.Y) return;                       // no block-level barrier below
     const float2* row = a.Tbuf + pd.T_off + (long long)t * pd.T_tstride + (long long)y * pd.rfX;
-    wave_c2r(buf, row, L, pd.rph, pd.twh, pd.rp, pd.twf, lane);
+    wave_c2r<PRIME>(buf, row, L, pd.rph, pd.twh, pd.rp, pd.twf, lane);
     const float sc = pd.scale;
     const long long o = (((long long)z * a.T + t) * a.Y + y) * a.X;
     const int Yv = a.fdims ? a.fdims[t].x : a.Y, Xv = a.fdims ? a.fdims[t].y : a.X;
@@ -1186,6 +1192,16 @@ __global__ void k_fill(float* p, long long n, float v) {
 // ---------------------------------------------------------------------------
 namespace flf {
 
+// Whether a plan ends with an odd prime radix above 5 (7, 11, ..., 31; the
+// kernels built with that pass are launched only then).  The other radices
+// are 2..6 and 8.
+bool plan_has_prime(const RadixPlan& rp) {
+    if (rp.n == 0) return false;
+    const int r = rp.r[rp.n - 1];
+    return r == 7 || r > 8;
+}
+static bool has_prime(const RadixPlan& rp) { return plan_has_prime(rp); }
+
 size_t small_fft_lds(int rY, int rX) {
     return ((size_t)(rY + 1) * (rX / 2 + 1) + 2 * (size_t)(rY + rX)) * sizeof(float2);
 }
@@ -1196,12 +1212,14 @@ hipError_t launch_small_fft(const SmallFFTArgs& a, int nsrc, int T, hipStream_t 
 }
 hipError_t launch_psf_rows(const PsfRowArgs& a, int nu, hipStream_t s) {
     size_t lds = (size_t)a.RB * a.rpX.L * sizeof(float2);
-    hipLaunchKernelGGL(k_psf_rows, dim3((a.s2 + a.RB - 1) / a.RB, nu), dim3(FFT_THREADS), lds, s, a);
+    if (has_prime(a.rpX)) hipLaunchKernelGGL(k_psf_rows<true>, dim3((a.s2 + a.RB - 1) / a.RB, nu), dim3(FFT_THREADS), lds, s, a);
+    else hipLaunchKernelGGL(k_psf_rows<false>, dim3((a.s2 + a.RB - 1) / a.RB, nu), dim3(FFT_THREADS), lds, s, a);
     return hipGetLastError();
 }
 hipError_t launch_psf_cols(const PsfColArgs& a, int nu, hipStream_t s) {
     size_t lds = (size_t)a.CB * a.rpY.L * sizeof(float2);
-    hipLaunchKernelGGL(k_psf_cols, dim3((a.rfX + a.CB - 1) / a.CB, nu), dim3(FFT_THREADS), lds, s, a);
+    if (has_prime(a.rpY)) hipLaunchKernelGGL(k_psf_cols<true>, dim3((a.rfX + a.CB - 1) / a.CB, nu), dim3(FFT_THREADS), lds, s, a);
+    else hipLaunchKernelGGL(k_psf_cols<false>, dim3((a.rfX + a.CB - 1) / a.CB, nu), dim3(FFT_THREADS), lds, s, a);
     return hipGetLastError();
 }
 hipError_t launch_spec_pack(const SpecPackArgs& a, int nchunks, hipStream_t s) {
@@ -1254,13 +1272,17 @@ hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks,
 size_t ifft_cols_lds(int CB, int fY) { return ((size_t)CB * fY + fY) * sizeof(float2); }
 hipError_t launch_ifft_cols(const IfftColArgs& a, int nbatch, hipStream_t s) {
     size_t lds = ifft_cols_lds(a.CB, a.rpY.L);
-    hipLaunchKernelGGL(k_ifft_cols, dim3((a.rfX + a.CB - 1) / a.CB, nbatch), dim3(64 * a.CB), lds, s, a);
+    if (has_prime(a.rpY)) hipLaunchKernelGGL(k_ifft_cols<true>, dim3((a.rfX + a.CB - 1) / a.CB, nbatch), dim3(64 * a.CB), lds, s, a);
+    else hipLaunchKernelGGL(k_ifft_cols<false>, dim3((a.rfX + a.CB - 1) / a.CB, nbatch), dim3(64 * a.CB), lds, s, a);
     return hipGetLastError();
 }
 size_t ifft_rows_sum_lds(int maxfX) { return ((size_t)maxfX + maxfX / 2 + 1 + 4 * (size_t)maxfX) * sizeof(float2); }
 hipError_t launch_ifft_rows_sum(const IfftRowSumArgs& a, int Y, int T, hipStream_t s) {
     size_t lds = ifft_rows_sum_lds(a.maxfX);
-    hipLaunchKernelGGL(k_ifft_rows_sum, dim3((Y + 3) / 4, T), dim3(256), lds, s, a);
+    bool prime = false;
+    for (int g = 0; g < a.ngroups; ++g) prime = prime || (a.grp[g].active && (has_prime(a.grp[g].rp) || has_prime(a.grp[g].rph)));
+    if (prime) hipLaunchKernelGGL(k_ifft_rows_sum<true>, dim3((Y + 3) / 4, T), dim3(256), lds, s, a);
+    else hipLaunchKernelGGL(k_ifft_rows_sum<false>, dim3((Y + 3) / 4, T), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 // c2r of an even length runs as a half-length complex FFT in place.
@@ -1268,7 +1290,8 @@ int c2r_wave_buffer(int fX) { return (fX & 1) ? fX : fX / 2; }
 size_t ifft_rows_plane_lds(int wbuf) { return 4 * (size_t)wbuf * sizeof(float2); }
 hipError_t launch_ifft_rows_plane(const IfftRowPlaneArgs& a, int nplanes, hipStream_t s) {
     size_t lds = ifft_rows_plane_lds(a.wbuf);
-    hipLaunchKernelGGL(k_ifft_rows_plane, dim3((a.Y + 3) / 4, a.T, nplanes), dim3(256), lds, s, a);
+    if (a.prime) hipLaunchKernelGGL(k_ifft_rows_plane<true>, dim3((a.Y + 3) / 4, a.T, nplanes), dim3(256), lds, s, a);
+    else hipLaunchKernelGGL(k_ifft_rows_plane<false>, dim3((a.Y + 3) / 4, a.T, nplanes), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 hipError_t launch_fit(const float* img, const float* im, long long n_per_frame, int T, int im_frames, double* part,
@@ -1278,8 +1301,11 @@ hipError_t launch_fit(const float* img, const float* im, long long n_per_frame, 
     return hipGetLastError();
 }
 hipError_t configure_kernels(size_t max_lds) {
-    const void* ks[] = {(const void*)k_small_fft, (const void*)k_psf_rows, (const void*)k_psf_cols,
-                        (const void*)k_ifft_cols, (const void*)k_ifft_rows_sum, (const void*)k_ifft_rows_plane,
+    const void* ks[] = {(const void*)k_small_fft, (const void*)k_psf_rows<false>, (const void*)k_psf_cols<false>,
+                        (const void*)k_ifft_cols<false>, (const void*)k_ifft_rows_sum<false>,
+                        (const void*)k_ifft_rows_plane<false>, (const void*)k_psf_rows<true>,
+                        (const void*)k_psf_cols<true>, (const void*)k_ifft_cols<true>,
+                        (const void*)k_ifft_rows_sum<true>, (const void*)k_ifft_rows_plane<true>,
                         (const void*)k_mac<4, 1, 1>, (const void*)k_mac<4, 1, 2>, (const void*)k_mac<4, 1, 3>,
                         (const void*)k_mac<4, 1, 4>, (const void*)k_mac<4, 2, 1>, (const void*)k_mac<4, 2, 2>,
                         (const void*)k_mac<4, 2, 3>, (const void*)k_mac<4, 2, 4>, (const void*)k_mac<2, 1, 1>,

@@ -72,6 +72,14 @@ bool make_radix_plan(int L, RadixPlan& rp, bool wide) {
             m /= R;
         }
     }
+    // One odd prime factor above 5 (a lenslet pitch such as 7, 13 or 19) is
+    // taken as the last pass, the only position where a radix-P butterfly
+    // writes onto its own inputs (see stockham_pass_prime, wave_pass_prime).
+    if (m > 5 && m <= FLF_PRIME_RADIX_MAX && max_prime_factor(m) == m) {
+        if (rp.n >= FLF_MAX_RADIX_PASSES) return false;
+        rp.r[rp.n++] = m;
+        m = 1;
+    }
     return m == 1;
 }
 

@@ -7,6 +7,7 @@
 #define FLF_CHUNK_MAX 256      // items per MAC workgroup
 #define FLF_MAX_FOURIER 16384  // Fourier grid length per axis (keeps phase products in 32 bits)
 #define FLF_MAX_N 31           // largest supported Nnum (s-table = N*N entries)
+#define FLF_PRIME_RADIX_MAX 31 // largest odd prime radix of a transform (a prime factor of Nnum)
 #define FLF_TILE 16            // items per matrix-core tile (one residue class)
 #define FLF_MAX_TILES 24       // tiles per chunk: <= FLF_CHUNK_MAX / FLF_TILE + FLF_MAX_RES
 

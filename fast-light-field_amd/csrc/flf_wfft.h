@@ -83,8 +83,65 @@ __device__ __forceinline__ void wave_pass(float2* buf, int es, int L, int Ns, co
     wave_sync();
 }
 
+// Last pass of an odd prime radix P (5 < P <= FLF_PRIME_MAX; Ns = L / P):
+// butterfly j < M = L/P reads x[j + t M], t < P, and writes y_q to
+// x[j + q M]: the same positions, so each lane takes its butterflies one at
+// a time.  y_q = sum_t x[j + t M] W^(j t) W^(M t q mod L), W = tw[1], the
+// table tw[m] = exp(-2 pi i m / L) (conjugated for the inverse).  Both
+// twiddle sequences are running products of one table entry, re-read from
+// the table every 8 steps (at most 7 roundings), so that only a few table
+// reads are live at a time.
+__device__ __forceinline__ void wave_pass_prime(float2* buf, int es, int L, int P, const float2* __restrict__ tw,
+                                                bool inv, int lane) {
+    const int M = L / P;
+    for (int j = lane; j < M; j += 64) {
+        float2 a[FLF_PRIME_MAX];
+        a[0] = buf[j * es];
+        {
+            float2 w1 = tw[j];                                 // W^j
+            if (inv) w1.y = -w1.y;
+            float2 w = w1;
+#pragma unroll
+            for (int t = 1; t < FLF_PRIME_MAX; ++t) {
+                if (t < P) {
+                    if ((t & 7) == 0) {                        // exact table value W^(j t), j t < L
+                        w = tw[j * t];
+                        if (inv) w.y = -w.y;
+                    }
+                    a[t] = cmul(buf[(j + t * M) * es], w);
+                    w = cmul(w, w1);
+                }
+            }
+        }
+        for (int q = 0; q < P; ++q) {
+            float2 wq = tw[q * M];                             // W_P^q
+            if (inv) wq.y = -wq.y;
+            float2 y = a[0], w = wq;
+            int m = q;                                         // t q mod P
+#pragma unroll
+            for (int t = 1; t < FLF_PRIME_MAX; ++t) {
+                if (t < P) {
+                    if ((t & 7) == 0) {
+                        w = tw[m * M];
+                        if (inv) w.y = -w.y;
+                    }
+                    y = cadd(y, cmul(a[t], w));
+                    w = cmul(w, wq);
+                    m += q;
+                    if (m >= P) m -= P;
+                }
+            }
+            buf[(j + q * M) * es] = y;
+        }
+    }
+    wave_sync();
+}
+
 // In-place transform of one sequence by one wave (inv: unnormalised inverse).
-template <int LMAX>
+// PRIME: the plan may end with an odd prime radix above 5 (kernels for lenslet
+// pitches such as 7, 13, 19 are compiled separately, so that the prime pass's
+// registers do not lower the occupancy of the 5-smooth kernels).
+template <int LMAX, bool PRIME = false>
 __device__ void wave_fft(float2* buf, int es, const RadixPlan& rp, const float2* __restrict__ tw, bool inv,
                          int lane) {
     int Ns = 1;
@@ -96,7 +153,10 @@ __device__ void wave_fft(float2* buf, int es, const RadixPlan& rp, const float2*
             case 4: wave_pass<4, LMAX>(buf, es, rp.L, Ns, tw, inv, lane); break;
             case 2: wave_pass<2, LMAX>(buf, es, rp.L, Ns, tw, inv, lane); break;
             case 3: wave_pass<3, LMAX>(buf, es, rp.L, Ns, tw, inv, lane); break;
-            default: wave_pass<5, LMAX>(buf, es, rp.L, Ns, tw, inv, lane); break;
+            case 5: wave_pass<5, LMAX>(buf, es, rp.L, Ns, tw, inv, lane); break;
+            default:
+                if constexpr (PRIME) wave_pass_prime(buf, es, rp.L, R, tw, inv, lane);   // last pass
+                break;
         }
         Ns *= R;
     }

@@ -345,13 +345,75 @@ def test_group_streams_bit_identical(projector, side_stream):
         assert rel_l2(many[0][t].cpu().numpy(), ref[t]) < 1e-5
 
 
-@pytest.mark.parametrize("N", [7, 19])
-def test_pitch_with_large_prime_factor_rejected(projector, N):
-    """A lenslet pitch whose Fourier grid cannot be 5-smooth is refused at
-    plan time with a clear error, before any kernel runs."""
-    hm = _hm_random(N, [2 * N + 1], seed=N)
-    with pytest.raises(ValueError, match="5-smooth"):       # FLF_EINVAL
-        projector.prepare(hm, 1, 4 * N, 4 * N)
+# Lenslet pitches with an odd prime factor above 5: the full-grid transforms
+# end with a radix-P pass.  The reference self-test shapes (projector.py:
+# 1178-1179) at Nnum = 19, both directions; 77 px PSFs give the even grid
+# 228 = 19 x 12, 21 px PSFs the odd grid 171 = 19 x 9 (full-length c2r rows).
+@pytest.mark.parametrize("shape", [(2, 150, 150), (2, 150, 300), (2, 300, 150)])
+@pytest.mark.parametrize("s2", [77, 21])
+@pytest.mark.parametrize("backward", [False, True])
+def test_selftest_shapes_nnum19(projector, shape, s2, backward):
+    hm = _hm_random(19, [s2], seed=s2)
+    rng = np.random.default_rng(19)
+    x = rng.random(shape).astype(np.float32)
+    print("Nnum 19 case", shape, s2, backward, flush=True)
+    if backward:
+        got = projector.BackwardProjectACC(hm, x, [0])[0]
+        ref = oproj.project_special(x, hm.Hcc(0, True))
+    else:
+        got = projector.ForwardProjectACC(hm, x[None], [0])
+        ref = oproj.project_special(x, hm.Hcc(0, False))
+    print("grid", projector.fshape(0), "rel-L2 %.3e" % rel_l2(got, ref), flush=True)
+    assert rel_l2(got, ref) < 1e-5
+
+
+@pytest.mark.parametrize("N,Y,X,s2", [(7, 49, 56, 15), (13, 65, 78, 27), (11, 60, 70, 23), (21, 84, 105, 43)])
+def test_prime_pitch_small_direct(projector, N, Y, X, s2):
+    """Pitches 7, 11, 13 and 21 (= 3 x 7) against the direct per-offset
+    convolution, several planes of two grids, both directions."""
+    hm = _hm_random(N, [s2, s2 + 2 * N], seed=N)
+    rng = np.random.default_rng(N)
+    x = rng.random((2, 1, Y, X)).astype(np.float32)
+    got = projector.ForwardProjectACC(hm, x, [0, 1])
+    ref = sum(oproj.project_direct(x[z], opsf.expand_quadrant(hm.Hcc(z, False).astype(np.float64))) for z in range(2))
+    print("N %d grids %s %s forward rel-L2 %.3e" % (N, projector.fshape(0), projector.fshape(1), rel_l2(got, ref)),
+          flush=True)
+    assert rel_l2(got, ref) < 1e-5
+    back = projector.BackwardProjectACC(hm, got, [0, 1])
+    for z in range(2):
+        refb = oproj.project_direct(got, opsf.expand_quadrant(hm.Hcc(z, True).astype(np.float64)))
+        assert rel_l2(back[z], refb) < 1e-5
+
+
+def test_nils_geometry_plane(projector):
+    """The reference benchmark's geometry (Nnum = 19, 1273 x 1463 px;
+    benchmarking.ipynb cell 0) with a 229 px ring PSF (grid 1520 x 1710):
+    forward and backward projection of one plane against the FP64 oracle."""
+    from flfhip import synthetic
+    rng = np.random.default_rng(22)
+    Q = synthetic.ring_quadrant(19, 229, 60.0, 2.0, rng)
+    from flfhip.psfmatrix import MemoryHMatrix, adjoint_quadrant
+    hm = MemoryHMatrix([Q], [adjoint_quadrant(Q)])
+    x = synthetic.blob_volume(1, 1273, 1463, 300, seed=9)
+    got = projector.ForwardProjectACC(hm, x, [0])
+    assert projector.fshape(0) == (1520, 1710)
+    ref = oproj.project_special(x[0], Q)
+    e = rel_l2(got, ref)
+    print("Nils geometry forward rel-L2 %.3e" % e, flush=True)
+    assert e < 1e-5
+    back = projector.BackwardProjectACC(hm, got, [0])[0]
+    refb = oproj.project_special(got, hm.Hcc(0, True))
+    eb = rel_l2(back, refb)
+    print("Nils geometry backward rel-L2 %.3e" % eb, flush=True)
+    assert eb < 1e-5
+
+
+def test_pitch_above_31_rejected(projector):
+    """Nnum above 31 is refused at plan time with a clear error, before any
+    kernel runs."""
+    hm = _hm_random(33, [67], seed=33)
+    with pytest.raises(ValueError, match="Nnum"):          # FLF_EINVAL
+        projector.prepare(hm, 1, 4 * 33, 4 * 33)
 
 
 def test_empty_plane_list(projector):
