@@ -78,9 +78,10 @@ def cpu_baseline(w):
     per_volume = per_bin * all_bins * (2 * w["iters"] + 1)
     return {"value": 1.0 / per_volume, "unit": "volumes/s", "cores": 1, "kind": "port",
             "sample": "FP64 numpy oracle (project_special), one thread: forward + backward projection of plane "
-                      "z=%d (PSF %d px) and forward of plane z=%d (PSF %d px) at %dx%d, %.1f s, scaled by Fourier "
+                      "z=%g (PSF %d px) and forward of plane z=%g (PSF %d px) at %dx%d, %.1f s, scaled by Fourier "
                       "bins to %d planes x (2 x %d iterations + 1) projections"
-                      % (w["zs"][zmax], w["sides"][zmax], w["zs"][zmed], w["sides"][zmed], Y, X, t_total,
+                      % (round(w["zs"][zmax], 3), w["sides"][zmax], round(w["zs"][zmed], 3), w["sides"][zmed], Y, X,
+                         t_total,
                          len(w["zs"]), w["iters"])}
 
 
@@ -277,8 +278,9 @@ def _iters_to_target(fits, target):
     return out
 
 
-def oracle_check(proj, hm, im_t, X0_t, iters=1, planes=(0, 20)):
-    """Bounded FP64-oracle comparison of one frame: Prevedel RL restricted to
+def oracle_check(proj, hm, im_t, X0_t, iters=1, planes=(0, 20), start="the learned leg's starting volume"):
+    """Part of the CPU leg (runs only with it, after every timed region).
+    Bounded FP64-oracle comparison of one frame: Prevedel RL restricted to
     two planes (the largest-PSF plane and z = 0) from the given start, on the
     GPU and in the FP64 oracle (numpy, one thread); relative L2 per plane."""
     import torch
@@ -304,7 +306,7 @@ def oracle_check(proj, hm, im_t, X0_t, iters=1, planes=(0, 20)):
     Xg = Xv.cpu().numpy()
     errs = [float(np.linalg.norm(Xg[z, 0] - Xr[k, 0]) / np.linalg.norm(Xr[k, 0])) for k, z in enumerate(planes)]
     return {"frame": 0, "planes": [int(hm.PSFShape(z)[0]) for z in planes], "plane_ids": planes,
-            "iterations": iters, "start": "the learned leg's starting volume", "rel_l2_per_plane": errs,
+            "iterations": iters, "start": start, "rel_l2_per_plane": errs,
             "oracle_s": secs,
             "what": "Prevedel RL of frame 0 restricted to planes %s (PSF sides %s px) on the GPU vs the FP64 numpy "
                     "oracle from the same start" % (planes, [int(hm.PSFShape(z)[0]) for z in planes])}
@@ -509,18 +511,156 @@ def batch_main(args):
         "spectra_precompute_s": t_prep,
     }
     del X1s
-    if not args.no_oracle_check:
-        # the learned leg's starting volume for frame 0 (guess or plain iterate)
-        g0, _, _ = li.initial_volume(im0, li.projector.backward_device(hm, im0))
-        t = time.time()
-        res["oracle_check"] = oracle_check(proj, hm, im0[0], g0[:, 0])
-        log("oracle check %.1fs: %s" % (time.time() - t, res["oracle_check"]["rel_l2_per_plane"]))
     if not args.no_cpu_baseline:
+        # the CPU leg: the FP64 oracle timed on a bounded sample, and (as the
+        # checker, outside every timed GPU region) frame 0 of the learned leg
+        # against the oracle from the same start
         t = time.time()
         wb = dict(hm=hm, Y=Y, X=X, N=w["N"], sides=w["sides"], zs=w["zs"], iters=iters,
                   vol=vol[:, :1].cpu().numpy())
         res["cpu_baseline"] = cpu_baseline(wb)
         log("cpu baseline %.1fs" % (time.time() - t))
+        if not args.no_oracle_check:
+            g0, _, _ = li.initial_volume(im0, li.projector.backward_device(hm, im0))
+            t = time.time()
+            res["cpu_baseline"]["oracle_check"] = oracle_check(proj, hm, im0[0], g0[:, 0])
+            log("oracle check %.1fs: %s" % (time.time() - t, res["cpu_baseline"]["oracle_check"]["rel_l2_per_plane"]))
+    print(json.dumps(res), flush=True)
+
+
+# Published wall times of the reference for this scenario (CuPy GPU and
+# C++/FFTW CPU paths, 4 iterations, F(H) not cached): benchmarking.ipynb:371-375,
+# 435-436 (batch 1, and the linear fits over batch size).
+NILS_PUBLISHED = {"gpu_batch1_s": 19.0, "gpu_fixed_s": 14.01, "gpu_per_frame_s": 4.98,
+                  "cpu_batch1_s": 628.8, "cpu_fixed_s": 565.69, "cpu_per_frame_s": 60.55,
+                  "hardware": "48-SM GPU with 224 GB/s memory; 8-core CPU (suil-bheag)"}
+
+
+def spectrum_bytes(proj, hm):
+    """Compulsory read of the cached half-plane spectra of one projection
+    direction: sum over planes of U * fY * (fX/2 + 1) * 8 B, U = h(h+1)/2
+    lenslet offsets for a square grid (transposes derived), h^2 otherwise."""
+    h = (hm.Nnum + 1) // 2
+    tot = 0.0
+    for z in range(hm.numZ):
+        fy, fx = proj.fshape(z)
+        u = h * (h + 1) // 2 if fy == fx else h * h
+        tot += u * fy * (fx // 2 + 1) * 8
+    return tot
+
+
+def nils_main(args):
+    """The reference's own benchmark (benchmarking.ipynb:14-28, 'new' and
+    'new-batch' of lf_performance_analysis.py:240-250): DeconvRL with
+    Htf=None and 4 iterations of the example Nnum=19 image, on a PSF set
+    synthesised on the GPU at the scenario's optics.  One step = Htf = Ht im,
+    then 4 Prevedel iterations from X = Htf, inputs resident in HBM; batch 1
+    and a batch of 24 copies of the image (the reference's batch runs tile
+    the same image).  value = seconds per image in the batch."""
+    import torch
+    from flfhip.lfdeconv import DeconvRL, deconv_rl_device, _fit_from_sums
+    from flfhip.projector import Projector_hip
+    from flfhip.workloads import setup_nils
+    w = setup_nils(os.path.join(ROOT, "tests", "golden", "nils_lfimage_1273x1463.tif"))
+    hm, Y, X, iters = w["hm"], w["Y"], w["X"], w["iters"]
+    nz = hm.numZ
+    B = args.frames if args.frames > 1 else 24
+    proj = Projector_hip()
+    dev = proj.device
+    s = torch.cuda.current_stream()
+
+    def timed(step, n):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(s)
+        for _ in range(n):
+            step()
+        e1.record(s)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    # first call as a user makes it: numpy image in, numpy volume out, the
+    # spectra computed inside (the reference's batch-1 figure includes F(H))
+    torch.cuda.synchronize()
+    t = time.time()
+    first = DeconvRL(hm, None, iters, None, im=w["im"][None], projector=proj, logPrint=False)
+    torch.cuda.synchronize()
+    first_s = time.time() - t
+    log("nils: first DeconvRL call (spectra + Htf + %d iterations, host arrays) %.2fs" % (iters, first_s))
+    res_frames = {}
+    for T in (1, B):
+        t = time.time()
+        proj.prepare(hm, T, Y, X)
+        torch.cuda.synchronize()
+        prep_s = time.time() - t
+        im = torch.from_numpy(w["im"]).to(dev)[None].expand(T, Y, X).contiguous()
+        Htf = torch.empty((nz, T, Y, X), dtype=torch.float32, device=dev)
+        Xv = torch.empty_like(Htf)
+
+        def step():
+            proj.backward_device(hm, im, out=Htf)
+            Xv.copy_(Htf)
+            deconv_rl_device(proj, hm, Htf, iters, Xv)
+
+        for _ in range(args.warmup):
+            step()
+        ms = timed(step, args.steps)
+        fit = _fit_from_sums(proj.datafit_device(proj.forward_device(hm, Xv), im))
+        res_frames[T] = {"ms_per_step": ms, "s_per_image": ms / 1e3 / T, "prepare_s": prep_s,
+                         "fit_after": float(fit[0])}
+        if T == 1:
+            X1 = Xv.clone()
+            d = float(np.linalg.norm(X1[:, 0].cpu().numpy() - first[:, 0]) / np.linalg.norm(first[:, 0]))
+            res_frames[T]["rel_l2_vs_first_call"] = d
+        else:
+            d = float(torch.linalg.vector_norm((Xv[:, -1] - X1[:, 0]).double()) /
+                      torch.linalg.vector_norm(X1[:, 0].double()))
+            res_frames[T]["rel_l2_last_frame_vs_batch1"] = d
+        log("nils: batch %d: %.1f ms per step, %.2f ms per image, prepare %.2fs" % (T, ms, ms / T, prep_s))
+        del Htf, Xv, im
+    s_dir = spectrum_bytes(proj, hm)
+    per = res_frames[B]["s_per_image"]
+    it_bytes = 2 * s_dir + B * (4 * nz + 2) * Y * X * 4
+    it_s = res_frames[B]["ms_per_step"] / 1e3 / (iters + 0.5)
+    res = {
+        "metric": "s per image: 4-iteration Richardson-Lucy of the reference's benchmark image (DeconvRL, Htf=None)",
+        "value": per, "unit": "s/image", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res_frames[B]["ms_per_step"], "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": per / NILS_PUBLISHED["gpu_per_frame_s"], "dtype": "f32",
+        "data": "real camera image (the reference's example data), PSF synthesised on the GPU at the scenario optics",
+        "config": {"workload": "nils: %d copies of the 1273x1463 Nnum=19 image, %d planes z=-60..60 um, PSF %d..%d "
+                               "px, %d Prevedel iterations from Htf" % (B, nz, min(w["sides"]), max(w["sides"]),
+                                                                         iters),
+                   "frames": B, "grids": sorted(set(tuple(proj.fshape(z)) for z in range(nz)))},
+        "batch1": res_frames[1], "batch": res_frames[B],
+        "first_call_s": first_s, "psf_synthesis_s": w["psf_s"],
+        "rl_iteration": {"ms": it_s * 1e3, "frames": B, "algorithmic_bytes": it_bytes,
+                         "achieved_GBps": it_bytes / it_s / 1e9, "frac": it_bytes / it_s / PEAK_HBM},
+        "reference_published": NILS_PUBLISHED,
+        "speedup_vs_published": {"gpu_per_frame": NILS_PUBLISHED["gpu_per_frame_s"] / per,
+                                 "gpu_batch1_first_call": NILS_PUBLISHED["gpu_batch1_s"] / first_s,
+                                 "cpu_per_frame": NILS_PUBLISHED["cpu_per_frame_s"] / per},
+    }
+    if not args.no_cpu_baseline:
+        # the CPU leg: the FP64 oracle timed on a bounded sample and scaled
+        # to one image (Htf + 2 projections per iteration), and as the
+        # checker, one iteration on two planes from X = Htf against it
+        t = time.time()
+        wb = dict(hm=hm, Y=Y, X=X, N=w["N"], sides=w["sides"], zs=w["zs"], iters=iters,
+                  vol=[w["im"][None]] * nz)
+        cb = cpu_baseline(wb)
+        cb["value"], cb["unit"] = 1.0 / cb["value"], "s/image"
+        res["cpu_baseline"] = cb
+        log("cpu baseline %.1fs: %.0f s/image" % (time.time() - t, cb["value"]))
+        if not args.no_oracle_check:
+            t = time.time()
+            im1 = torch.from_numpy(w["im"]).to(dev)[None].contiguous()
+            proj.prepare(hm, 1, Y, X)
+            Htf1 = proj.backward_device(hm, im1)
+            cb["oracle_check"] = oracle_check(proj, hm, im1[0], Htf1[:, 0], iters=1, planes=(0, nz // 2),
+                                              start="X = Htf (DeconvRL's default Xguess)")
+            log("nils oracle check %.1fs: %s" % (time.time() - t, cb["oracle_check"]["rel_l2_per_plane"]))
     print(json.dumps(res), flush=True)
 
 
@@ -529,7 +669,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rl1455x32", choices=sorted(WORKLOADS) + sorted(RAGGED) + sorted(BATCHES))
+    ap.add_argument("--workload", default="rl1455x32", choices=sorted(WORKLOADS) + sorted(RAGGED) + sorted(BATCHES) + ["nils"])
     ap.add_argument("--buckets", type=int, default=2, help="ragged workloads: canvas sizes (at most)")
     ap.add_argument("--graph", action="store_true", help="ragged workloads: replay iterations as HIP graphs")
     ap.add_argument("--weights", default=None, help="learned-initialiser weights (default: the packaged file)")
@@ -555,6 +695,8 @@ def main():
         return ragged_main(args)
     if args.workload in BATCHES:
         return batch_main(args)
+    if args.workload == "nils":
+        return nils_main(args)
 
     import torch
     from flfhip.lfdeconv import deconv_rl_device, _fit_from_sums

@@ -171,6 +171,19 @@ def blob_volumes_gpu(nz, Y, X, T, nblobs_range, seed, device, background=0.01, s
     return vol
 
 
+def _frames(proj, hMatrix, T, Y, X, vseed, peaks, gen, nblobs):
+    """Seeded blob volumes (nz, T, Y, X), their images Poisson(scale * H vol)
+    at the given peak counts (T,), and the per-frame scale."""
+    dev = proj.device
+    vol = blob_volumes_gpu(hMatrix.numZ, Y, X, T, nblobs, vseed, dev)
+    proj.prepare(hMatrix, T, Y, X)
+    clean = proj.forward_device(hMatrix, vol)
+    pk = torch.tensor(peaks, dtype=torch.float32, device=dev)
+    scale = pk / clean.reshape(T, -1).amax(dim=1)
+    im = torch.poisson(clean.clamp_min(0) * scale[:, None, None], generator=gen)
+    return vol, im, scale
+
+
 def make_pairs(proj, hMatrix, n, L, seed, peak=(500.0, 5000.0), iters=8, batch=16, nblobs=(50, 300), log=None):
     """n training pairs at L x L lenslets: (images (n, Y, X), targets
     (n, nz, L, L)) on the GPU, targets = lenslet means of plain RL after
@@ -189,21 +202,81 @@ def make_pairs(proj, hMatrix, n, L, seed, peak=(500.0, 5000.0), iters=8, batch=1
     rng = np.random.default_rng(seed)
     for b0 in range(0, n, batch):
         T = min(batch, n - b0)
-        vol = blob_volumes_gpu(nz, Y, X, T, nblobs, seed * 100003 + b0, dev)
-        proj.prepare(hMatrix, T, Y, X)
-        clean = proj.forward_device(hMatrix, vol)
-        peaks = torch.tensor(rng.uniform(peak[0], peak[1], T), dtype=torch.float32, device=dev)
-        scale = peaks / clean.reshape(T, -1).amax(dim=1)
-        im = torch.poisson(clean.clamp_min(0) * scale[:, None, None], generator=gen)
+        vol, im, _ = _frames(proj, hMatrix, T, Y, X, seed * 100003 + b0, rng.uniform(peak[0], peak[1], T), gen,
+                             nblobs)
         Htf = proj.backward_device(hMatrix, im)
         Xv = torch.ones_like(Htf)
         deconv_rl_device(proj, hMatrix, Htf, iters, Xv, im=im)
         imgs[b0:b0 + T] = im
         tgts[b0:b0 + T] = lenslet_means(Xv, N)
-        del vol, clean, Htf, Xv
+        del vol, im, Htf, Xv
         if log and (b0 // batch) % 16 == 0:
             log("pairs %d / %d" % (b0 + T, n))
     return imgs, tgts
+
+
+def evaluate_heldout(net, proj, hMatrix, n, L, seed, iters=8, batch=16, peak=(500.0, 5000.0), nblobs=(50, 300),
+                     max_iter=32):
+    """Held-out evaluation of a trained network on the n frames that
+    make_pairs(..., seed, batch=batch) would draw (same volumes, peaks and
+    noise): per frame, plain Prevedel RL from a uniform start for `iters`
+    iterations (its fit is the target), then the guarded learned start and
+    RL until the fit reaches that target.  Returns per-frame arrays and a
+    summary: guard acceptance, learned iterations to the target (the plain
+    leg needs `iters`; a rejected guess counts its plain iterate as one),
+    and the relative L2 of both results against the ground-truth volume."""
+    from .lfdeconv import deconv_rl_device, _fit_from_sums
+    if n % batch:
+        raise ValueError("the frame count must be a multiple of the batch")
+    Y = X = L * hMatrix.Nnum
+    dev = proj.device
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    rng = np.random.default_rng(seed)
+    li = LearnedInit(net, proj, hMatrix)
+    acc, n_it, gt_p, gt_l, fit_t, fit_l = [], [], [], [], [], []
+
+    def rel_truth(Xv, vol, scale):
+        d = (Xv.double() / scale.double()[None, :, None, None] - vol.double()).reshape(vol.shape[0], vol.shape[1], -1)
+        return (torch.linalg.vector_norm(d, dim=(0, 2)) /
+                torch.linalg.vector_norm(vol.double().reshape(vol.shape[0], vol.shape[1], -1), dim=(0, 2)))
+
+    for b0 in range(0, n, batch):
+        T = min(batch, n - b0)
+        vol, im, scale = _frames(proj, hMatrix, T, Y, X, seed * 100003 + b0, rng.uniform(peak[0], peak[1], T), gen,
+                                 nblobs)
+        Htf = proj.backward_device(hMatrix, im)
+        Xp = torch.ones_like(Htf)
+        deconv_rl_device(proj, hMatrix, Htf, iters, Xp, im=im)
+        target = _fit_from_sums(proj.datafit_device(proj.forward_device(hMatrix, Xp), im))
+        fits = []
+        Xl, _, info = li.deconvolve(im, Htf, max_iter, targetFit=target, fitOut=fits)
+        F = np.array([np.asarray(f, np.float64) for f in fits])          # (iterates, T)
+        for t in range(T):
+            k = np.nonzero(F[:, t] <= target[t])[0]
+            n_it.append((int(k[0]) if len(k) else -1) + (0 if info["accepted"][t] else 1))
+        acc.extend(bool(a) for a in info["accepted"])
+        fit_t.extend(float(v) for v in target)
+        fit_l.extend(float(v) for v in F[-1])
+        gt_p.extend(rel_truth(Xp, vol, scale).tolist())
+        gt_l.extend(rel_truth(Xl, vol, scale).tolist())
+        del vol, im, Htf, Xp, Xl
+    n_it = np.array(n_it)
+    gt_p, gt_l = np.array(gt_p), np.array(gt_l)
+    summ = lambda a: {"min": float(np.min(a)), "median": float(np.median(a)), "mean": float(np.mean(a)),
+                      "max": float(np.max(a))}
+    summary = {
+        "frames": n, "lenslets": L, "seed": seed, "plain_iterations": iters,
+        "accepted_fraction": float(np.mean(acc)),
+        "learned_iterations": summ(n_it[n_it >= 0]) if np.any(n_it >= 0) else None,
+        "learned_iterations_histogram": {str(int(k)): int(v) for k, v in zip(*np.unique(n_it, return_counts=True))},
+        "not_reached": int(np.sum(n_it < 0)),
+        "iterations_saved_mean": float(iters - np.mean(n_it[n_it >= 0])) if np.any(n_it >= 0) else None,
+        "gt_rel_l2_plain": summ(gt_p), "gt_rel_l2_learned": summ(gt_l),
+        "learned_better_gt_fraction": float(np.mean(gt_l < gt_p)),
+    }
+    return {"accepted": acc, "iterations": n_it.tolist(), "target_fit": fit_t, "learned_fit": fit_l,
+            "gt_plain": gt_p.tolist(), "gt_learned": gt_l.tolist()}, summary
 
 
 def dihedral(x, k):

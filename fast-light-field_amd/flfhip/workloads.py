@@ -12,6 +12,12 @@ drifting 0.5 px per frame (synthetic.drifting_blob_frames).
     name       frames  Nnum  lenslets  planes      PSF sides  blobs/frame  peak  RL iterations
     ragged64   64      15    20..60    z=-15..15   15..255    50..300      1000  8
 
+The reference's own benchmark scenario (NILS; benchmarking.ipynb:14-28,
+the 'nils-image' and 'nils-matrix' options of lf_performance_analysis.py:141,
+161): one 1273x1463 light-field camera image of Nnum=19 lenslets, the PSF of
+M 22.222, NA 0.5, ML pitch 125 um, f_ML 3125 um, n 1.33, 520 nm at 25 planes
+z=-60..60 um (5 um apart), 4 RL iterations from Htf (DeconvRL with Htf=None).
+
 Batches of full-size frames (BATCHES): T frames of one size reconstructed
 together, each a seeded blob volume with its own blob count, on the PSF set
 of a base workload (so that a network trained for it applies).
@@ -37,6 +43,12 @@ BATCHES = {
 }
 
 
+# (M, NA, MLPitch, Nnum, OSR, n, fml, lambda, zmin, zmax, zspacing) of
+# GeneratePSF, image shape, RL iterations
+NILS = {"optics": (22.222, 0.5, 125e-6, 19, 3, 1.33, 3125e-6, 520e-9, -60e-6, 60e-6, 5e-6),
+        "shape": (1273, 1463), "iters": 4}
+
+
 RAGGED = {
     # name: (frames, Nnum, L_min, L_max, zmax, s_min, s_max, blobs_min, blobs_max, peak, iters, seed)
     "ragged64": (64, 15, 20, 60, 15, 15, 255, 50, 300, 1000.0, 8, 3),
@@ -48,8 +60,12 @@ def _log(*a):
 
 
 def psf_set(name):
-    """(HMatrix, zs, sides) of a workload."""
-    N, L, zmax, smin, smax, nblobs, peak, iters, seed = WORKLOADS[name]
+    """(HMatrix, zs, sides) of a workload (WORKLOADS, or the shared PSF set of
+    a RAGGED time series)."""
+    if name in RAGGED:
+        T, N, lmin, lmax, zmax, smin, smax, bmin, bmax, peak, iters, seed = RAGGED[name]
+    else:
+        N, L, zmax, smin, smax, nblobs, peak, iters, seed = WORKLOADS[name]
     zs = list(range(-zmax, zmax + 1))
     sides = synthetic.psf_sides(zs, smin, smax)
     t = time.time()
@@ -78,11 +94,7 @@ def setup_ragged(name, volume=True):
     volume) the frames' blob volumes.  Frame t's lenslet grid and blob count
     come from a generator seeded with seed + t."""
     T, N, lmin, lmax, zmax, smin, smax, bmin, bmax, peak, iters, seed = RAGGED[name]
-    zs = list(range(-zmax, zmax + 1))
-    sides = synthetic.psf_sides(zs, smin, smax)
-    t0 = time.time()
-    hm = synthetic.make_psf_set(N, zs, sides, seed=seed)
-    _log("psf set: %d planes, sides %d..%d, %.1fs" % (len(zs), min(sides), max(sides), time.time() - t0))
+    hm, zs, sides = psf_set(name)
     shapes, nblobs = [], []
     for t in range(T):
         rng = synthetic.np.random.default_rng(seed + t)
@@ -109,3 +121,25 @@ def setup_batch(name):
     Y = X = N * L
     return dict(name=name, base=base, N=N, Y=Y, X=X, zs=zs, sides=sides, hm=hm, frames=T, nblobs=(bmin, bmax),
                 peak=peak, iters=iters, seed=seed)
+
+
+def setup_nils(image_path):
+    """The reference's benchmark scenario: the camera image (float32, as the
+    reference reads it) and its PSF set synthesised on the GPU
+    (psfgen.GeneratePSF at the scenario's optics; seconds, where the
+    reference's CPU generation takes tens of minutes)."""
+    import numpy as np
+    import torch
+    from . import lfimage, psfgen
+    im = lfimage.LoadLightFieldTiff(image_path)
+    if im.shape != NILS["shape"]:
+        raise ValueError("%s: expected a %s image, found %s" % (image_path, NILS["shape"], im.shape))
+    torch.cuda.synchronize()
+    t = time.time()
+    hm, info = psfgen.GeneratePSF(*NILS["optics"])
+    torch.cuda.synchronize()
+    t_psf = time.time() - t
+    sides = [int(hm.PSFShape(z)[0]) for z in range(hm.numZ)]
+    _log("nils: PSF set of %d planes, sides %d..%d, synthesised in %.1fs" % (hm.numZ, min(sides), max(sides), t_psf))
+    return dict(name="nils", N=NILS["optics"][3], Y=im.shape[0], X=im.shape[1], im=im.astype(np.float32), hm=hm,
+                zs=[z * 1e6 for z in info["zs"]], sides=sides, iters=NILS["iters"], psf_s=t_psf, psf_info=info)

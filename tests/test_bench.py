@@ -47,3 +47,19 @@ def test_iterations_to_target():
     fits = [[0.5, 0.2], [0.3, 0.15], [0.1, 0.14]]
     assert bench._iters_to_target(fits, [0.3, 0.2]) == [1, 0]
     assert bench._iters_to_target(fits, [0.05, 0.14]) == [-1, 2]
+
+
+def test_nils_scenario_inputs():
+    """The reference benchmark's scenario (benchmarking.ipynb:14-28): the
+    example image fixture reads as a 1273 x 1463 float32 image, and the
+    optics give 25 planes z = -60..60 um."""
+    sys.path.insert(0, os.path.join(ROOT, "fast-light-field_amd"))
+    import numpy as np
+    from flfhip import lfimage, psfgen
+    from flfhip.workloads import NILS
+    im = lfimage.LoadLightFieldTiff(os.path.join(ROOT, "tests", "golden", "nils_lfimage_1273x1463.tif"))
+    assert im.shape == NILS["shape"] and im.dtype == np.float32
+    assert float(im.min()) > 0 and float(im.max()) == 65535.0
+    zs = psfgen.plane_depths(*NILS["optics"][8:])
+    assert len(zs) == 25 and abs(zs[0] + 60e-6) < 1e-12 and abs(zs[-1] - 60e-6) < 1e-12
+    assert NILS["optics"][3] == 19 and NILS["iters"] == 4
