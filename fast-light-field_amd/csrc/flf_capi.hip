@@ -76,6 +76,7 @@ struct flf_ctx {
     int ipt = 2;                      // MAC items per lane pair's thread: 2 = lane pairs share two items
                                       // and split the spectra (FLF_ITEMS_PER_THREAD=1 selects one item
                                       // per thread); plan time
+    int radix_order = 0;              // pass order of every FFT plan (make_radix_plan); plan time
     int fb_max = 2;                   // frames per MAC workgroup at T >= 2 (FLF_MAC_FRAMES overrides)
     int fwd_split = 0;                // forward plane ranges, 0 = automatic (FLF_FWD_SPLIT overrides)
     int bwd_planes = 1;               // planes per backward MAC workgroup
@@ -603,10 +604,12 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
             Group& g = ctx->groups.back();
             std::string e = build_grid_tables(nnum, fY, fX, g.gt, ctx->ipt);
             if (!e.empty()) return fail(ctx, FLF_EINVAL, e);
-            if (!make_radix_plan(fY, g.rpY) || !make_radix_plan(fX, g.rpX) ||
-                !make_radix_plan(g.gt.rY, g.rpYr) || !make_radix_plan(g.gt.rX, g.rpXr) ||
-                !make_radix_plan(fY, g.rpYw, true) || !make_radix_plan(fX, g.rpXw, true) ||
-                (fX % 2 == 0 && (!make_radix_plan(fX / 2, g.rpXh) || !make_radix_plan(fX / 2, g.rpXhw, true))))
+            const int ro = ctx->radix_order;
+            if (!make_radix_plan(fY, g.rpY, false, ro) || !make_radix_plan(fX, g.rpX, false, ro) ||
+                !make_radix_plan(g.gt.rY, g.rpYr, false, ro) || !make_radix_plan(g.gt.rX, g.rpXr, false, ro) ||
+                !make_radix_plan(fY, g.rpYw, true, ro) || !make_radix_plan(fX, g.rpXw, true, ro) ||
+                (fX % 2 == 0 &&
+                 (!make_radix_plan(fX / 2, g.rpXh, false, ro) || !make_radix_plan(fX / 2, g.rpXhw, true, ro))))
                 return fail(ctx, FLF_EINVAL, "Fourier grid length has a prime factor other than 2, 3, 5 and one odd "
                                              "prime up to 31");
             gi = (int)ctx->groups.size() - 1;
@@ -803,6 +806,12 @@ int flf_set_option(flf_ctx* ctx, const char* name, int value) {
         ctx->bwd_planes = value;
     } else if (n == "group_streams") {
         ctx->group_streams = value != 0;
+    } else if (n == "radix_order") {             // takes effect at the next flf_plan
+        if (value < 0 || value >= FLF_RADIX_ORDERS) return fail(ctx, FLF_EINVAL, "radix_order must be 0..3");
+        ctx->radix_order = value;
+    } else if (n == "mac_ipt") {                 // takes effect at the next flf_plan
+        if (value != 1 && value != 2) return fail(ctx, FLF_EINVAL, "mac_ipt must be 1 or 2");
+        ctx->ipt = value;
     } else if (n == "ifft_col_batch") {
         if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");
         if (value < 1 || value > 8) return fail(ctx, FLF_EINVAL, "ifft_col_batch must be 1..8");
@@ -824,6 +833,8 @@ int flf_get_option(const flf_ctx* ctx, const char* name, int* value) {
     else if (n == "mac_mfma") *value = ctx->mfma ? 1 : 0;
     else if (n == "bwd_planes") *value = ctx->bwd_planes;
     else if (n == "group_streams") *value = ctx->group_streams ? 1 : 0;
+    else if (n == "radix_order") *value = ctx->radix_order;
+    else if (n == "mac_ipt") *value = ctx->ipt;
     else if (n == "ifft_col_batch") *value = ctx->groups.empty() ? 0 : ctx->groups[0].CBi;
     else return FLF_EINVAL;
     return FLF_OK;
@@ -1181,6 +1192,15 @@ int flf_profile_read(flf_ctx* ctx, double* mac_ms, long long* mac_launches, doub
     *mac_ms = ms;
     *mac_launches = n;
     *mac_bytes = by;
+    return FLF_OK;
+}
+
+int flf_radix_plan(int L, int wide, int order, int* radices, int* npass) {
+    if (!radices || !npass || L < 1) return FLF_EINVAL;
+    RadixPlan rp;
+    if (!make_radix_plan(L, rp, wide != 0, order)) return FLF_EINVAL;
+    for (int i = 0; i < rp.n; ++i) radices[i] = rp.r[i];
+    *npass = rp.n;
     return FLF_OK;
 }
 

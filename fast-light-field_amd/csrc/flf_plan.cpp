@@ -51,7 +51,7 @@ int fourier_len(int s1, int s2, int N) {
     return N * smooth5_at_least(red);
 }
 
-bool make_radix_plan(int L, RadixPlan& rp, bool wide) {
+bool make_radix_plan(int L, RadixPlan& rp, bool wide, int order) {
     rp.L = L;
     rp.n = 0;
     int m = L;
@@ -59,19 +59,25 @@ bool make_radix_plan(int L, RadixPlan& rp, bool wide) {
     // composite radices 8 and 6 first, one pass (an LDS round trip) fewer at
     // most lengths used here.  Radices 9 and 10 were tried: their butterflies
     // raised the register count of every wave-FFT kernel past the occupancy
-    // its LDS allows, with spills.
+    // its LDS allows, with spills.  Orders 2 and 3 change the radices of
+    // wave-level plans only: the block-level kernels size their butterfly
+    // registers from the plan's radices (fft_batch_cap).
     const int narrow[4] = {4, 2, 3, 5};
-    const int wider[6] = {8, 6, 4, 5, 3, 2};
-    const int* order = wide ? wider : narrow;
-    const int nord = wide ? 6 : 4;
-    for (int k = 0; k < nord; ++k) {
-        int R = order[k];
+    const int wider[4][6] = {{8, 6, 4, 5, 3, 2}, {8, 6, 4, 5, 3, 2}, {6, 4, 5, 3, 2, 0}, {5, 3, 8, 6, 4, 2}};
+    if (order < 0 || order >= FLF_RADIX_ORDERS) return false;
+    const int* list = wide ? wider[order] : narrow;
+    const int nlist = wide ? 6 : 4;
+    for (int k = 0; k < nlist; ++k) {
+        const int R = list[k];
+        if (R == 0) break;
         while (m % R == 0) {
             if (rp.n >= FLF_MAX_RADIX_PASSES) return false;
             rp.r[rp.n++] = R;
             m /= R;
         }
     }
+    if (order == 1)
+        for (int i = 0, j = rp.n - 1; i < j; ++i, --j) std::swap(rp.r[i], rp.r[j]);
     // One odd prime factor above 5 (a lenslet pitch such as 7, 13 or 19) is
     // taken as the last pass, the only position where a radix-P butterfly
     // writes onto its own inputs (see stockham_pass_prime, wave_pass_prime).

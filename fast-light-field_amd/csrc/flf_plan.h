@@ -16,7 +16,12 @@ int smooth5_at_least(int n);
 int fourier_len(int s1, int s2, int N);
 
 // Radix plan (4s first, then 2, 3, 5).  Returns false if L is not 5-smooth.
-bool make_radix_plan(int L, RadixPlan& rp, bool wide = false);
+// Pass orders of a plan (the autotuner's radix_order): 0 largest radices
+// first; 1 the same passes reversed; 2 (wave-level plans) without radix 8;
+// 3 (wave-level plans) the odd radices first.  An odd prime factor above 5
+// is always the last pass.
+constexpr int FLF_RADIX_ORDERS = 4;
+bool make_radix_plan(int L, RadixPlan& rp, bool wide = false, int order = 0);
 
 // Index tables for one Fourier grid (fY, fX).
 struct GridTables {

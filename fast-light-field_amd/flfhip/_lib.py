@@ -39,6 +39,7 @@ SIGNATURES = {
                                _c.c_void_p]),
     "flf_fourier_len": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int]),
     "flf_grid_check": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.POINTER(_c.c_longlong)]),
+    "flf_radix_plan": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int)]),
     "flf_set_frames": (_c.c_int, [_c.c_void_p, _c.c_int]),
     "flf_forward_pair": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_void_p, _c.POINTER(_c.c_int),
                                     _c.c_int, _c.c_void_p]),

@@ -166,9 +166,16 @@ class Projector_hip(object):
         changes whenever that state is freed or rewritten."""
         return (self.plan_generation, self._frames, self._dims)
 
+    # options that the library applies at its next plan (flf_set_option)
+    PLAN_OPTIONS = ("radix_order", "mac_ipt")
+
     def set_option(self, name, value):
-        """Launch-shape option of the library (flf_set_option)."""
+        """Launch-shape option of the library (flf_set_option).  A plan
+        option (PLAN_OPTIONS) takes effect from the next prepare(), which
+        then re-plans and recomputes the spectra."""
         self._check(self.lib.flf_set_option(self._ctx, name.encode(), int(value)), "flf_set_option(%s)" % name)
+        if name in self.PLAN_OPTIONS:
+            self._key = None
 
     def get_option(self, name):
         v = ctypes.c_int()

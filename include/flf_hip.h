@@ -110,6 +110,12 @@ int flf_set_frame_dims(flf_ctx* ctx, const int* dims, int T, void* stream);
  *                    run side by side on library-owned streams, forked from
  *                    and joined back to the caller's stream by events (one
  *                    stream while the caller's stream is captured)
+ * Plan options, applied by the next flf_plan (the FFTW_MEASURE-style
+ * choices of py_light_field.cpp:25-31):
+ *   "radix_order"    0..3 pass order of every FFT: 0 largest radices first,
+ *                    1 reversed, 2 no radix 8 (wave-level transforms), 3 odd
+ *                    radices first (wave-level); a prime pass stays last
+ *   "mac_ipt"        1 or 2 items per MAC thread (2: lane pairs share two)
  * Results do not depend on them beyond FP32 rounding (group_streams: not at
  * all).  flfhip.autotune picks them by timing. */
 int flf_set_option(flf_ctx* ctx, const char* name, int value);
@@ -225,9 +231,13 @@ int flf_profile_read(flf_ctx* ctx, double* mac_ms, long long* mac_launches, doub
  * bin is written exactly once and that every pair shares its residues;
  * stats[6] = {workgroups,
  * items, max residues per orbit, stored spectra per plane, slots per item,
- * max items per workgroup}.  Returns FLF_EINVAL if the tables are invalid. */
+ * max items per workgroup}.  Returns FLF_EINVAL if the tables are invalid.
+ * flf_radix_plan: the passes of a length-L transform for plan option
+ * radix_order = order (wide = 1: a wave-level transform) into radices[16];
+ * *npass receives their count.  FLF_EINVAL if L has no such plan. */
 int flf_fourier_len(int s1, int s2, int nnum);
 int flf_grid_check(int nnum, int fY, int fX, long long* stats);
+int flf_radix_plan(int L, int wide, int order, int* radices, int* npass);
 
 /* ---- Learned initialiser (no reference counterpart: the reference starts
  * RL from Htf or a caller-supplied Xguess, lfdeconv.py:97-104) ----

@@ -164,3 +164,48 @@ def test_option_entry_points_reject_null():
     assert lib.flf_set_option(None, b"fwd_split", 1) == -1
     v = ctypes.c_int()
     assert lib.flf_get_option(None, b"fwd_split", ctypes.byref(v)) == -1
+
+
+def _plan(lib, L, wide, order):
+    r = (ctypes.c_int * 16)()
+    n = ctypes.c_int()
+    rc = lib.flf_radix_plan(L, wide, order, r, ctypes.byref(n))
+    return rc, list(r[:n.value])
+
+
+@pytest.mark.parametrize("L", [1800, 1620, 1500, 900, 120, 225, 66, 99, 1520, 1710, 1539, 65, 78, 56, 2048, 7])
+def test_radix_plan_orders(lib, L):
+    """Every pass order of a plan multiplies out to L, keeps an odd prime
+    factor above 5 as the last pass, uses the radices its kernels hold
+    (2..5 block-level; also 8 and 6 wave-level), and differs from the
+    default order as documented: 1 reverses the 5-smooth passes, 2 takes no
+    radix 8, 3 takes the odd radices first; block-level plans change only
+    under order 1."""
+    import math
+    for wide in (0, 1):
+        base = None
+        for order in range(4):
+            rc, r = _plan(lib, L, wide, order)
+            assert rc == 0, (L, wide, order)
+            assert math.prod(r) == L
+            smooth = [x for x in r if x in (2, 3, 4, 5, 6, 8)]
+            primes = [x for x in r if x not in (2, 3, 4, 5, 6, 8)]
+            assert len(primes) <= 1 and r[len(smooth):] == primes          # the prime pass is last
+            assert set(smooth) <= ({2, 3, 4, 5, 6, 8} if wide else {2, 3, 4, 5})
+            if order == 0:
+                base = smooth
+            elif order == 1:
+                assert smooth == base[::-1]
+            elif not wide:
+                assert smooth == base
+            elif order == 2:
+                assert 8 not in smooth
+            else:
+                odd = [x for x in smooth if x in (3, 5)]
+                assert smooth[:len(odd)] == odd
+    assert _plan(lib, L, 1, 4)[0] == _lib.FLF_EINVAL
+
+
+def test_radix_plan_rejects_unsupported(lib):
+    assert _plan(lib, 7 * 11, 1, 0)[0] == _lib.FLF_EINVAL       # two primes above 5
+    assert _plan(lib, 37, 0, 0)[0] == _lib.FLF_EINVAL           # prime above 31

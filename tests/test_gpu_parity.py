@@ -385,6 +385,34 @@ def test_prime_pitch_small_direct(projector, N, Y, X, s2):
         assert rel_l2(back[z], refb) < 1e-5
 
 
+@pytest.mark.parametrize("order", [0, 1, 2, 3])
+@pytest.mark.parametrize("N,Y,X,s2s", [(15, 90, 150, [31, 61]), (13, 65, 78, [27, 53])])
+def test_plan_options_radix_order_and_ipt(projector, order, N, Y, X, s2s):
+    """Every FFT pass order (radix_order 0..3) with one and two items per
+    MAC thread (mac_ipt), on grids of lengths 120..225 (radices 8, 6, 5, 3,
+    2) and of Nnum 13 (prime pass last): forward and backward projection of
+    two planes against the FP64 oracle.  Own projectors: the plan options
+    stay off the session's projector."""
+    from flfhip.projector import Projector_hip
+    hm = _hm_random(N, s2s, seed=N + order)
+    rng = np.random.default_rng(order)
+    x = rng.random((2, 1, Y, X)).astype(np.float32)
+    ref = sum(oproj.project_special(x[z], hm.Hcc(z, False)) for z in range(2))
+    for ipt in (1, 2):
+        p = Projector_hip()
+        p.set_option("radix_order", order)
+        p.set_option("mac_ipt", ipt)
+        got = p.ForwardProjectACC(hm, x, [0, 1])
+        assert p.get_option("radix_order") == order and p.get_option("mac_ipt") == ipt
+        e = rel_l2(got, ref)
+        back = p.BackwardProjectACC(hm, got, [0, 1])
+        eb = max(rel_l2(back[z], oproj.project_special(got, hm.Hcc(z, True))) for z in range(2))
+        print("radix_order %d ipt %d grids %s %s: forward %.2e backward %.2e"
+              % (order, ipt, p.fshape(0), p.fshape(1), e, eb), flush=True)
+        assert e < 1e-5 and eb < 1e-5
+        del p
+
+
 def test_nils_geometry_plane(projector):
     """The reference benchmark's geometry (Nnum = 19, 1273 x 1463 px;
     benchmarking.ipynb cell 0) with a 229 px ring PSF (grid 1520 x 1710):
