@@ -101,6 +101,25 @@ def test_heldout_evaluation_packaged_network(projector):
     assert np.isfinite(per["gt_learned"]).all() and np.isfinite(per["gt_plain"]).all()
 
 
+def test_packaged_31_plane_network_heldout(projector):
+    """The packaged 31-plane network (PSF set of the ragged time series,
+    PSF 15..255 px; trained on 20k 600x600 pairs of seed 4 against
+    8-iteration targets): on the first 32 of its 512 held-out frames (seed
+    5) the guard keeps the guess and RL from it reaches the 8-iteration
+    plain fit in at most 4 iterations."""
+    from flfhip import workloads
+    from flfhip.lensletnet import LensletNet, default_weights_path, evaluate_heldout
+    hm, zs, sides = workloads.psf_set("ragged64")
+    path = default_weights_path(hm.Nnum, hm.numZ, "ragged64")
+    assert os.path.exists(path), path
+    net = LensletNet.load(path)
+    per, summ = evaluate_heldout(net, projector, hm, 32, 40, seed=5, iters=8, batch=32)
+    print(summ, flush=True)
+    assert summ["accepted_fraction"] >= 0.9
+    assert summ["not_reached"] == 0 and max(per["iterations"]) <= 4
+    assert summ["iterations_saved_mean"] >= 4.0
+
+
 def test_guess_beside_htf_matches_sequential(projector):
     """Htf computed on the current stream while the network's guess runs on a
     second one: the same Htf, guess and RL result as one after the other,
