@@ -89,6 +89,11 @@ struct flf_ctx {
     // backward and RL iteration unchanged; the MAC launches already fill
     // the GPU).
     bool group_streams = false;
+    // Grouped launches (option "grouped_launch", on by default): the small
+    // FFTs, the MAC and the column transforms of all active grid groups of a
+    // projection each go out as one dispatch, every group's arguments by
+    // value in the kernel arguments.
+    bool grouped = true;
     std::vector<hipStream_t> gstreams;
     std::vector<hipEvent_t> gevents;  // [0] fork, [1 + i] join of gstreams[i]
     long long bytes = 0;
@@ -417,20 +422,137 @@ int run_small_fft_image(flf_ctx* ctx, Group& g, const float* img, hipStream_t st
     return FLF_OK;
 }
 
-// Per group (active ones only), on the group's stream: small FFTs of the
-// image, the per-plane MAC and the column transforms into the group's slice
-// of d_T_bwd.
+// The work of one grid group in one projection, as the kernels take it:
+// its small FFTs (nsrc source planes), its MAC (nranges plane ranges) and
+// its column transforms (nbatch transforms).
+struct GroupWork {
+    size_t gi;
+    SmallFFTArgs sa;
+    int nsrc;
+    MacArgs ma;
+    int nranges;
+    IfftColArgs ca;
+    int nbatch;
+};
+
+// One group's launches on st (the path of group streams, of a single group,
+// and of graphs captured before their argument tables exist).
+int launch_group_work(flf_ctx* ctx, const GroupWork& w, hipStream_t st) {
+    Group& g = ctx->groups[w.gi];
+    HIPCHK(launch_small_fft(w.sa, w.nsrc, ctx->T, st));
+    int rc = run_mac(ctx, w.ma, g, w.nranges, st);
+    if (rc) return rc;
+    HIPCHK(launch_ifft_cols(w.ca, w.nbatch, st));
+    return FLF_OK;
+}
+
+// All groups' stages as one dispatch each on st.  done = false when this is
+// not possible (fewer than two groups, the matrix-core MAC, mixed item
+// shapes, LDS): the caller then launches group by group.
+int launch_grouped(flf_ctx* ctx, const std::vector<GroupWork>& ws, hipStream_t st, bool& done) {
+    done = false;
+    if (!ctx->grouped || ws.size() < 2 || ws.size() > FLF_MAX_GROUPS || ctx->mfma) return FLF_OK;
+    const GridTables& g0 = ctx->groups[ws[0].gi].gt;
+    int fb = 4, CB = 8;
+    bool prime = false;
+    for (const GroupWork& w : ws) {
+        const Group& g = ctx->groups[w.gi];
+        if (g.gt.slots != g0.slots || g.gt.ipt != g0.ipt) return FLF_OK;
+        fb = std::min(fb, mac_frames(ctx, g.gt));
+        CB = std::min(CB, g.CBi);
+        prime = prime || plan_has_prime(w.ca.rpY);
+    }
+    SmallFFTGrouped sg{};
+    MacGrouped mg{};
+    IfftColGrouped cg{};
+    sg.gg.ngroups = mg.gg.ngroups = cg.gg.ngroups = (int)ws.size();
+    size_t lds_s = 0, lds_m = 0, lds_c = 0;
+    double bytes = 0;
+    for (size_t k = 0; k < ws.size(); ++k) {
+        const GroupWork& w = ws[k];
+        const Group& g = ctx->groups[w.gi];
+        sg.a[k] = w.sa;
+        sg.gg.start[k + 1] = sg.gg.start[k] + w.nsrc;
+        sg.gg.nchunks[k] = sg.gg.nranges[k] = 1;
+        lds_s = std::max(lds_s, small_fft_lds(w.sa.rpY.L, w.sa.rpX.L));
+        mg.a[k] = w.ma;
+        const int nch = (int)g.gt.chunks.size();
+        mg.gg.start[k + 1] = mg.gg.start[k] + (nch * w.nranges + 7) / 8 * 8;   // XCD-aligned (k_mac_grouped)
+        mg.gg.nchunks[k] = nch;
+        mg.gg.nranges[k] = w.nranges;
+        lds_m = std::max(lds_m, mac_lds(g.gt.slots, g.gt.nu, ctx->N, fb));
+        cg.a[k] = w.ca;
+        cg.a[k].CB = CB;
+        const int ncb = (w.ca.rfX + CB - 1) / CB;
+        cg.gg.start[k + 1] = cg.gg.start[k] + ncb * w.nbatch;
+        cg.gg.nchunks[k] = ncb;
+        cg.gg.nranges[k] = w.nbatch;
+        lds_c = std::max(lds_c, ifft_cols_lds(CB, w.ca.rpY.L));
+        int np = 0;
+        for (unsigned char e : g.enable_host) np += e ? 1 : 0;
+        bytes += (double)np * g.gt.nu * g.gt.fY * g.gt.rfX * sizeof(float2);
+    }
+    if (lds_s > ctx->max_lds || lds_m > ctx->max_lds || lds_c > ctx->max_lds) return FLF_OK;
+    HIPCHK(launch_small_fft_grouped(sg, ctx->N, ctx->T, lds_s, st));
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (ctx->prof) {
+        ea = take_event(ctx);
+        eb = take_event(ctx);
+        if (ea && eb) HIPCHK(hipEventRecord(ea, st));
+    }
+    HIPCHK(launch_mac_grouped(mg, g0.slots, g0.ipt, fb, (ctx->T + fb - 1) / fb, lds_m, st));
+    if (ctx->prof && ea && eb) {
+        HIPCHK(hipEventRecord(eb, st));
+        ctx->prof_recs.push_back({ea, eb, bytes});
+    }
+    HIPCHK(launch_ifft_cols_grouped(cg, CB, prime, lds_c, st));
+    done = true;
+    return FLF_OK;
+}
+
+// The groups' work of one projection: grouped dispatches on the caller's
+// stream when possible, else per group on the group streams (fk).
+int run_group_work(flf_ctx* ctx, const std::vector<GroupWork>& ws, const GroupFork& fk) {
+    if (!fk.on) {
+        bool done = false;
+        int rc = launch_grouped(ctx, ws, fk.st, done);
+        if (rc || done) return rc;
+    }
+    for (size_t k = 0; k < ws.size(); ++k) {
+        int rc = launch_group_work(ctx, ws[k], fk.stream((int)k));
+        if (rc) return rc;
+    }
+    return FLF_OK;
+}
+
+// Per active group: small FFTs of the image, the per-plane MAC and the
+// column transforms into the group's slice of d_T_bwd.
 int run_backward_groups(flf_ctx* ctx, const float* img, const std::vector<char>& active, const GroupFork& fk) {
     const int N = ctx->N, T = ctx->T;
-    int k = 0;
+    std::vector<GroupWork> ws;
     for (size_t gi = 0; gi < ctx->groups.size(); ++gi) {
         if (!active[gi]) continue;
         Group& g = ctx->groups[gi];
-        const hipStream_t gs = fk.stream(k++);
-        int rc = run_small_fft_image(ctx, g, img, gs);
-        if (rc) return rc;
         const GridTables& gt = g.gt;
-        MacArgs m{};
+        GroupWork w{};
+        w.gi = gi;
+        SmallFFTArgs& a = w.sa;
+        a.src = img;
+        a.src_off = ctx->d_zeros;
+        a.src_tstride = (long long)ctx->Y * ctx->X;
+        a.src_pitch = ctx->X;
+        a.Y = ctx->Y; a.X = ctx->X; a.N = N;
+        a.rpY = g.rpYr; a.rpX = g.rpXr;
+        a.twY = g.twRY; a.twX = g.twRX;
+        a.S = g.d_S_bwd;
+        a.S_off = ctx->d_zeros;
+        a.chunks = g.d_chunks; a.nchunks = (int)gt.chunks.size(); a.NH = gt.slots;
+        a.fY = gt.fY; a.fX = gt.fX;
+        a.S_tstride = s_frame_stride(ctx, g);
+        a.fdims = frame_dims(ctx);
+        a.tsplit = T;
+        w.nsrc = 1;
+        MacArgs& m = w.ma;
         m.G = g.d_G[1];
         m.chunks = g.d_chunks; m.items = g.d_items; m.u_bu = g.d_bu; m.u_au = g.d_au; m.uoff = g.d_uoff;
         m.tiles = g.d_tiles;
@@ -449,9 +571,8 @@ int run_backward_groups(flf_ctx* ctx, const float* img, const std::vector<char>&
         // the matrix-core kernel keeps one plane per workgroup.
         m.per_plane = 1;
         const int bp = ctx->mfma ? 1 : std::max(1, ctx->bwd_planes);
-        rc = run_mac(ctx, m, g, (m.nP + bp - 1) / bp, gs);
-        if (rc) return rc;
-        IfftColArgs ic{};
+        w.nranges = (m.nP + bp - 1) / bp;
+        IfftColArgs& ic = w.ca;
         ic.A = g.d_A_bwd;
         ic.A_bstride = (long long)gt.fY * gt.rfX;
         ic.nsum = 1;
@@ -460,9 +581,10 @@ int run_backward_groups(flf_ctx* ctx, const float* img, const std::vector<char>&
         ic.twY = g.twY;
         ic.out = ctx->d_T_bwd + g.Tbwd_off;
         ic.out_bstride = (long long)ctx->Y * gt.rfX;
-        HIPCHK(launch_ifft_cols(ic, m.nP * T, gs));
+        w.nbatch = m.nP * T;
+        ws.push_back(w);
     }
-    return FLF_OK;
+    return run_group_work(ctx, ws, fk);
 }
 
 int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, const float* trust, int mode,
@@ -806,6 +928,8 @@ int flf_set_option(flf_ctx* ctx, const char* name, int value) {
         ctx->bwd_planes = value;
     } else if (n == "group_streams") {
         ctx->group_streams = value != 0;
+    } else if (n == "grouped_launch") {
+        ctx->grouped = value != 0;
     } else if (n == "radix_order") {             // takes effect at the next flf_plan
         if (value < 0 || value >= FLF_RADIX_ORDERS) return fail(ctx, FLF_EINVAL, "radix_order must be 0..3");
         ctx->radix_order = value;
@@ -833,6 +957,7 @@ int flf_get_option(const flf_ctx* ctx, const char* name, int* value) {
     else if (n == "mac_mfma") *value = ctx->mfma ? 1 : 0;
     else if (n == "bwd_planes") *value = ctx->bwd_planes;
     else if (n == "group_streams") *value = ctx->group_streams ? 1 : 0;
+    else if (n == "grouped_launch") *value = ctx->grouped ? 1 : 0;
     else if (n == "radix_order") *value = ctx->radix_order;
     else if (n == "mac_ipt") *value = ctx->ipt;
     else if (n == "ifft_col_batch") *value = ctx->groups.empty() ? 0 : ctx->groups[0].CBi;
@@ -986,21 +1111,21 @@ int run_forward(flf_ctx* ctx, const float* vol, const float* vol2, float* img, c
     return FLF_OK;
 }
 
-// Per active group, on the group's stream: small FFTs of the volume planes,
-// the MAC into the group's partial accumulators and the column transforms
-// into the group's d_T_fwd (summed over groups by the row transform).
+// Per active group: small FFTs of the volume planes, the MAC into the
+// group's partial accumulators and the column transforms into the group's
+// d_T_fwd (summed over groups by the row transform).
 int run_forward_groups(flf_ctx* ctx, const float* vol, const float* vol2, const IfftRowSumArgs& rs,
                        const GroupFork& fk) {
     const int N = ctx->N, T = ctx->T;
-    int k = 0;
+    std::vector<GroupWork> ws;
     for (size_t gi = 0; gi < ctx->groups.size(); ++gi) {
         if (!rs.grp[gi].active) continue;
         Group& g = ctx->groups[gi];
         const GridTables& gt = g.gt;
-        const hipStream_t st = fk.stream(k++);
-        int rc = FLF_OK;
         const int nP = (int)g.planes.size();
-        SmallFFTArgs a{};
+        GroupWork w{};
+        w.gi = gi;
+        SmallFFTArgs& a = w.sa;
         a.src = vol;
         // two sources: both hold T/2 frames, so the plane offsets are those of T/2 frames
         a.src_off = vol2 ? g.srcoff_T[T / 2 - 1] : g.d_srcoff_fwd;
@@ -1017,8 +1142,8 @@ int run_forward_groups(flf_ctx* ctx, const float* vol, const float* vol2, const 
         a.fY = gt.fY; a.fX = gt.fX;
         a.S_tstride = s_frame_stride(ctx, g);
         a.fdims = frame_dims(ctx);
-        HIPCHK(launch_small_fft(a, nP, T, st));
-        MacArgs m{};
+        w.nsrc = nP;
+        MacArgs& m = w.ma;
         m.G = g.d_G[0];
         m.chunks = g.d_chunks; m.items = g.d_items; m.u_bu = g.d_bu; m.u_au = g.d_au; m.uoff = g.d_uoff;
         m.tiles = g.d_tiles;
@@ -1034,22 +1159,21 @@ int run_forward_groups(flf_ctx* ctx, const float* vol, const float* vol2, const 
         // Split the plane sum into ranges so that the launch has enough
         // workgroups to keep the tail short; the column transform adds the
         // partial accumulators in a fixed order.
-        const int sp = fwd_split(ctx, g, nP);
-        rc = run_mac(ctx, m, g, sp, st);
-        if (rc) return rc;
-        IfftColArgs ic{};
+        w.nranges = fwd_split(ctx, g, nP);
+        IfftColArgs& ic = w.ca;
         ic.A = g.d_A_fwd;
         ic.A_bstride = (long long)gt.fY * gt.rfX;
-        ic.nsum = sp;
+        ic.nsum = w.nranges;
         ic.A_sstride = m.A_pstride;
         ic.rfX = gt.rfX; ic.CB = g.CBi; ic.Yout = ctx->Y;
         ic.rpY = g.rpYw;
         ic.twY = g.twY;
         ic.out = g.d_T_fwd;
         ic.out_bstride = (long long)ctx->Y * gt.rfX;
-        HIPCHK(launch_ifft_cols(ic, T, st));
+        w.nbatch = T;
+        ws.push_back(w);
     }
-    return FLF_OK;
+    return run_group_work(ctx, ws, fk);
 }
 }  // namespace
 

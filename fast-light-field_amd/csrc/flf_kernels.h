@@ -13,6 +13,18 @@ struct PlaneMask {
     __host__ __device__ bool on(int p) const { return (w[p >> 5] >> (p & 31)) & 1u; }
 };
 
+// Block ranges of a grouped launch (one dispatch for every active grid group
+// of a projection).  start[g] is group g's first flattened block,
+// start[ngroups] the total; nchunks/nranges split a group's blocks (MAC:
+// chunks x plane ranges, each group's range padded to a multiple of 8;
+// column transform: column blocks x batches).
+struct GroupedGrid {
+    int ngroups;
+    int start[FLF_MAX_GROUPS + 1];
+    int nchunks[FLF_MAX_GROUPS];
+    int nranges[FLF_MAX_GROUPS];
+};
+
 struct SmallFFTArgs {
     const float* src;            // source base
     const long long* src_off;    // per source plane z: element offset (device array)
@@ -96,6 +108,22 @@ struct IfftColArgs {
     long long out_bstride;
 };
 
+// Every active grid group's arguments of one stage, by value in the kernel
+// arguments (pointers read from kernel arguments are known to be global, so
+// the kernels load through them with global rather than flat loads).
+struct SmallFFTGrouped {
+    SmallFFTArgs a[FLF_MAX_GROUPS];
+    GroupedGrid gg;
+};
+struct MacGrouped {
+    MacArgs a[FLF_MAX_GROUPS];
+    GroupedGrid gg;
+};
+struct IfftColGrouped {
+    IfftColArgs a[FLF_MAX_GROUPS];
+    GroupedGrid gg;
+};
+
 struct IfftRowGroup {
     const float2* T;             // (T, Y, rfX)
     long long tstride;
@@ -152,6 +180,10 @@ size_t ifft_rows_sum_lds(int maxfX);
 size_t ifft_rows_plane_lds(int wbuf);
 int c2r_wave_buffer(int fX);
 hipError_t launch_small_fft(const SmallFFTArgs& a, int nsrc, int T, hipStream_t s);
+// Grouped launches (the groups' arguments by value).
+hipError_t launch_small_fft_grouped(const SmallFFTGrouped& ga, int N, int T, size_t lds, hipStream_t s);
+hipError_t launch_mac_grouped(const MacGrouped& ga, int slots, int ipt, int fb, int nfb, size_t lds, hipStream_t s);
+hipError_t launch_ifft_cols_grouped(const IfftColGrouped& ga, int CB, bool prime, size_t lds, hipStream_t s);
 hipError_t launch_psf_rows(const PsfRowArgs& a, int nu, hipStream_t s);
 hipError_t launch_psf_cols(const PsfColArgs& a, int nu, hipStream_t s);
 hipError_t launch_spec_pack(const SpecPackArgs& a, int nchunks, hipStream_t s);

@@ -68,11 +68,35 @@ __device__ __forceinline__ int xcd_block(int bid, int n) {
     return bid;
 #endif
 }
-__global__ __launch_bounds__(64 * SFFT_WAVES) void k_small_fft(SmallFFTArgs a) {
+// Grouped launches: the grid groups of one projection in one dispatch.  A
+// flattened block index b falls in group g for start[g] <= b < start[g + 1];
+// the group is found with static indices only (selects on kernel-argument
+// values); its arguments are then read from the kernel arguments with g as
+// a scalar index.
+struct GroupPick {
+    int g, rem, n, nch, nr;                       // n: blocks of the group (start[g + 1] - start[g])
+};
+__device__ __forceinline__ GroupPick pick_group(const GroupedGrid& gg, int b) {
+    GroupPick p{0, b, gg.start[1], gg.nchunks[0], gg.nranges[0]};
+    int s0 = 0;
+#pragma unroll
+    for (int k = 1; k < FLF_MAX_GROUPS; ++k)
+        if (k < gg.ngroups && b >= gg.start[k]) {
+            p.g = k;
+            s0 = gg.start[k];
+            p.n = gg.start[k + 1] - gg.start[k];
+            p.nch = gg.nchunks[k];
+            p.nr = gg.nranges[k];
+        }
+    p.rem = b - s0;
+    return p;
+}
+
+// One (lenslet offset, frame, source plane) of a grid group; src0/src2/
+// tsplit: the source buffers of the call.
+__device__ __forceinline__ void small_fft_body(const SmallFFTArgs& a, const float* src0, const float* src2,
+                                               int tsplit, int off, int t, int z) {
     extern __shared__ float2 lds[];
-    const int off = xcd_block(blockIdx.x, a.N * a.N);
-    const int t = blockIdx.y;
-    const int z = blockIdx.z;
     const int N = a.N;
     const int b = off / N, aa = off % N;
     const int rY = a.rpY.L, rX = a.rpX.L, rXh = rX / 2 + 1;
@@ -84,8 +108,8 @@ __global__ __launch_bounds__(64 * SFFT_WAVES) void k_small_fft(SmallFFTArgs a) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < rY; i += blockDim.x) twY[i] = a.twY[i];
     for (int i = threadIdx.x; i < rX; i += blockDim.x) twX[i] = a.twX[i];
-    const bool second = a.src2 && t >= a.tsplit;  // two-source calls: frames from two buffers
-    const float* src = (second ? a.src2 : a.src) + a.src_off[z] + (long long)(second ? t - a.tsplit : t) * a.src_tstride;
+    const bool second = src2 && t >= tsplit;      // two-source calls: frames from two buffers
+    const float* src = (second ? src2 : src0) + a.src_off[z] + (long long)(second ? t - tsplit : t) * a.src_tstride;
     // frame t's valid region (frames of one batch may be smaller than the plan)
     const int Yv = a.fdims ? a.fdims[t].x : a.Y, Xv = a.fdims ? a.fdims[t].y : a.X;
     const int Ys = max(0, (Yv - b + N - 1) / N);
@@ -171,6 +195,17 @@ __global__ __launch_bounds__(64 * SFFT_WAVES) void k_small_fft(SmallFFTArgs a) {
         v = cmul(v, cmul(phY[k0], phX[k1]));
         S[((long long)w * NN + off) * NH + j] = v;
     }
+}
+
+__global__ __launch_bounds__(64 * SFFT_WAVES) void k_small_fft(SmallFFTArgs a) {
+    small_fft_body(a, a.src, a.src2, a.tsplit, xcd_block(blockIdx.x, a.N * a.N), blockIdx.y, blockIdx.z);
+}
+// grid (N*N, T, sum over groups of their source planes): z flattened over
+// (group, plane).
+__global__ __launch_bounds__(64 * SFFT_WAVES) void k_small_fft_grouped(SmallFFTGrouped ga) {
+    const GroupPick p = pick_group(ga.gg, blockIdx.z);
+    const SmallFFTArgs& a = ga.a[p.g];
+    small_fft_body(a, a.src, a.src2, a.tsplit, xcd_block(blockIdx.x, a.N * a.N), blockIdx.y, p.rem);
 }
 
 // ---------------------------------------------------------------------------
@@ -450,14 +485,14 @@ __device__ __forceinline__ void mac_tables(const MacArgs& a, const ChunkDesc* ch
 // residues and splits the spectra between its lanes.  FB frames per
 // workgroup: every stored spectrum value read from HBM serves FB frames
 // (grid z = frame blocks).
+// One workgroup: chunk cidx, plane range `range` of nranges, frame block
+// fblock.
 template <int SLOTS, int IPT, int FB>
-__global__ __launch_bounds__(FLF_CHUNK_MAX) __attribute__((amdgpu_waves_per_eu(IPT * FB >= 3 ? 2 : (IPT * FB == 2 ? 3 : FLF_MAC_WAVES))))
-void k_mac(MacArgs a) {
+__device__ __forceinline__ void mac_body(const MacArgs& a, int cidx, int range, int nranges, int fblock) {
     extern __shared__ float4 pm_tab[];            // [FB][NRES][rs: nu*NPT + pad], then staging
     constexpr int NRES = SLOTS == 4 ? 8 : 4;      // residues per orbit (canonical list)
     constexpr int NH = NRES / 2;                  // residues stored by the small-FFT kernel
     constexpr int UNR = IPT >= 2 ? FLF_MAC_UNR2 : FLF_MAC_UNR;   // spectra whose loads are issued together
-    const int cidx = xcd_block(blockIdx.x, gridDim.x);
     const ChunkDesc* ch = a.chunks + cidx;
     const long long item_base = ch->item_base;
     const int n_items = ch->n_items;
@@ -473,7 +508,7 @@ void k_mac(MacArgs a) {
     const bool valid = slot * IPT < n_items;
     const int tsafe = valid ? slot : 0;          // idle lanes read item 0 (in bounds) and never write
     const int N = a.N, NN = N * N, fourN = 4 * N;
-    const int t0 = blockIdx.z * FB;
+    const int t0 = fblock * FB;
     const int nf = min(FB, a.T - t0);            // frames of this block (block-uniform)
     const int nu = a.nu;
     // Residue rows are a.rs = nu*NPT + pad entries apart, rs = 1 (mod 16):
@@ -510,9 +545,9 @@ void k_mac(MacArgs a) {
             for (int j = 0; j < 4; ++j) tot[f][q][j] = make_float2(0.f, 0.f);
 
     // Planes of this workgroup: an even share of the group's nP planes for
-    // grid y = plane ranges (gridDim.y = nP gives one plane per workgroup).
-    const int pb = (int)(((long long)blockIdx.y * a.nP) / gridDim.y);
-    const int pe = (int)(((long long)(blockIdx.y + 1) * a.nP) / gridDim.y);
+    // nranges plane ranges (nranges = nP gives one plane per workgroup).
+    const int pb = (int)(((long long)range * a.nP) / nranges);
+    const int pe = (int)(((long long)(range + 1) * a.nP) / nranges);
     const long long ustride = (long long)n_items * SLOTS;
     const int nul = (nu + USTEP - 1) / USTEP;     // spectra per lane and plane
     const int nb = (nul + UNR - 1) / UNR;         // spectrum batches per plane
@@ -655,7 +690,28 @@ void k_mac(MacArgs a) {
         }
     }
 
-    if (valid && !a.per_plane) store(blockIdx.y);
+    if (valid && !a.per_plane) store(range);
+}
+
+template <int SLOTS, int IPT, int FB>
+__global__ __launch_bounds__(FLF_CHUNK_MAX) __attribute__((amdgpu_waves_per_eu(IPT * FB >= 3 ? 2 : (IPT * FB == 2 ? 3 : FLF_MAC_WAVES))))
+void k_mac(MacArgs a) {
+    mac_body<SLOTS, IPT, FB>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, gridDim.y, blockIdx.z);
+}
+
+// All active grid groups of one projection: grid x flattened over (group,
+// plane range, chunk), z = frame blocks.  Each group's range starts at a
+// multiple of 8 (padded with idle workgroups), and the XCD remap applies
+// within the group: every XCD then takes an eighth of every group, whose
+// workgroups cost about the same, where one remap over the whole grid would
+// give some XCDs only a cheap group's workgroups and leave them idle early.
+template <int SLOTS, int IPT, int FB>
+__global__ __launch_bounds__(FLF_CHUNK_MAX) __attribute__((amdgpu_waves_per_eu(IPT * FB >= 3 ? 2 : (IPT * FB == 2 ? 3 : FLF_MAC_WAVES))))
+void k_mac_grouped(MacGrouped ga) {
+    const GroupPick p = pick_group(ga.gg, blockIdx.x);
+    const int w = xcd_block(p.rem, p.n);
+    if (w >= p.nch * p.nr) return;                // padding
+    mac_body<SLOTS, IPT, FB>(ga.a[p.g], w % p.nch, w / p.nch, p.nr, blockIdx.z);
 }
 
 // Matrix-core variant for FB >= 2 frames per workgroup.  The items of a chunk
@@ -876,14 +932,14 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void k_mac_mfma(MacArgs a) {
 // CB*fY + fY (twiddles).
 // ---------------------------------------------------------------------------
 template <bool PRIME>
-__global__ __launch_bounds__(512) void k_ifft_cols(IfftColArgs a) {
+__device__ __forceinline__ void ifft_cols_body(const IfftColArgs& a, int colblk, int batch) {
     extern __shared__ float2 lds[];
     const int fY = a.rpY.L, rfX = a.rfX, CB = a.CB;
     float2* C = lds;                 // [CB][fY]
     float2* tw = C + CB * fY;        // [fY]
-    const int c0 = blockIdx.x * CB;
+    const int c0 = colblk * CB;
     const int ncol = min(CB, rfX - c0);
-    const float2* in = a.A + (long long)blockIdx.y * a.A_bstride;
+    const float2* in = a.A + (long long)batch * a.A_bstride;
     for (int i = threadIdx.x; i < fY; i += blockDim.x) tw[i] = a.twY[i];
     constexpr int B = 8;                        // loads in flight per thread
     for (int i0 = threadIdx.x; i0 < fY * CB; i0 += blockDim.x * B) {
@@ -914,11 +970,22 @@ __global__ __launch_bounds__(512) void k_ifft_cols(IfftColArgs a) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (wv < ncol) wave_fft<2048, PRIME>(C + wv * fY, 1, a.rpY, tw, true, lane);
     __syncthreads();
-    float2* out = a.out + (long long)blockIdx.y * a.out_bstride;
+    float2* out = a.out + (long long)batch * a.out_bstride;
     for (int i = threadIdx.x; i < a.Yout * CB; i += blockDim.x) {
         const int y = i / CB, cc = i - y * CB;
         if (cc < ncol) out[(long long)y * rfX + c0 + cc] = C[cc * fY + y];
     }
+}
+template <bool PRIME>
+__global__ __launch_bounds__(512) void k_ifft_cols(IfftColArgs a) {
+    ifft_cols_body<PRIME>(a, blockIdx.x, blockIdx.y);
+}
+// All active grid groups: grid x flattened over (group, batch, column
+// block), nchunks[g] = column blocks of group g; one CB for every group.
+template <bool PRIME>
+__global__ __launch_bounds__(512) void k_ifft_cols_grouped(IfftColGrouped ga) {
+    const GroupPick p = pick_group(ga.gg, blockIdx.x);
+    ifft_cols_body<PRIME>(ga.a[p.g], p.rem % p.nch, p.rem / p.nch);
 }
 
 // c2r of one row by one wave.  row: rfX = L/2+1 half-spectrum values (global);
@@ -1210,6 +1277,12 @@ hipError_t launch_small_fft(const SmallFFTArgs& a, int nsrc, int T, hipStream_t 
     hipLaunchKernelGGL(k_small_fft, dim3(a.N * a.N, T, nsrc), dim3(64 * SFFT_WAVES), lds, s, a);
     return hipGetLastError();
 }
+hipError_t launch_small_fft_grouped(const SmallFFTGrouped& ga, int N, int T, size_t lds, hipStream_t s) {
+    if (ga.gg.ngroups < 1 || ga.gg.ngroups > FLF_MAX_GROUPS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_small_fft_grouped, dim3(N * N, T, ga.gg.start[ga.gg.ngroups]), dim3(64 * SFFT_WAVES), lds, s,
+                       ga);
+    return hipGetLastError();
+}
 hipError_t launch_psf_rows(const PsfRowArgs& a, int nu, hipStream_t s) {
     size_t lds = (size_t)a.RB * a.rpX.L * sizeof(float2);
     if (has_prime(a.rpX)) hipLaunchKernelGGL(k_psf_rows<true>, dim3((a.s2 + a.RB - 1) / a.RB, nu), dim3(FFT_THREADS), lds, s, a);
@@ -1269,7 +1342,37 @@ hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks,
     }
     return hipGetLastError();
 }
+template <int SLOTS, int IPT>
+static void launch_mac_grouped_fb(const MacGrouped& ga, int fb, dim3 grid, dim3 block, size_t lds, hipStream_t s) {
+    if (fb == 1) hipLaunchKernelGGL((k_mac_grouped<SLOTS, IPT, 1>), grid, block, lds, s, ga);
+    else if (fb == 2) hipLaunchKernelGGL((k_mac_grouped<SLOTS, IPT, 2>), grid, block, lds, s, ga);
+    else if (fb == 3) hipLaunchKernelGGL((k_mac_grouped<SLOTS, IPT, 3>), grid, block, lds, s, ga);
+    else hipLaunchKernelGGL((k_mac_grouped<SLOTS, IPT, 4>), grid, block, lds, s, ga);
+}
+hipError_t launch_mac_grouped(const MacGrouped& ga, int slots, int ipt, int fb, int nfb, size_t lds, hipStream_t s) {
+    const GroupedGrid& gg = ga.gg;
+    if (fb < 1 || fb > 4 || (ipt != 1 && ipt != 2) || (slots != 2 && slots != 4) || nfb < 1 || gg.ngroups < 1 ||
+        gg.ngroups > FLF_MAX_GROUPS)
+        return hipErrorInvalidValue;
+    dim3 grid(gg.start[gg.ngroups], 1, nfb);
+    dim3 block(FLF_CHUNK_MAX);
+    if (slots == 4) {
+        if (ipt == 1) launch_mac_grouped_fb<4, 1>(ga, fb, grid, block, lds, s);
+        else launch_mac_grouped_fb<4, 2>(ga, fb, grid, block, lds, s);
+    } else {
+        if (ipt == 1) launch_mac_grouped_fb<2, 1>(ga, fb, grid, block, lds, s);
+        else launch_mac_grouped_fb<2, 2>(ga, fb, grid, block, lds, s);
+    }
+    return hipGetLastError();
+}
 size_t ifft_cols_lds(int CB, int fY) { return ((size_t)CB * fY + fY) * sizeof(float2); }
+hipError_t launch_ifft_cols_grouped(const IfftColGrouped& ga, int CB, bool prime, size_t lds, hipStream_t s) {
+    const GroupedGrid& gg = ga.gg;
+    if (CB < 1 || CB > 8 || gg.ngroups < 1 || gg.ngroups > FLF_MAX_GROUPS) return hipErrorInvalidValue;
+    if (prime) hipLaunchKernelGGL(k_ifft_cols_grouped<true>, dim3(gg.start[gg.ngroups]), dim3(64 * CB), lds, s, ga);
+    else hipLaunchKernelGGL(k_ifft_cols_grouped<false>, dim3(gg.start[gg.ngroups]), dim3(64 * CB), lds, s, ga);
+    return hipGetLastError();
+}
 hipError_t launch_ifft_cols(const IfftColArgs& a, int nbatch, hipStream_t s) {
     size_t lds = ifft_cols_lds(a.CB, a.rpY.L);
     if (has_prime(a.rpY)) hipLaunchKernelGGL(k_ifft_cols<true>, dim3((a.rfX + a.CB - 1) / a.CB, nbatch), dim3(64 * a.CB), lds, s, a);
@@ -1313,7 +1416,17 @@ hipError_t configure_kernels(size_t max_lds) {
                         (const void*)k_mac<2, 2, 1>, (const void*)k_mac<2, 2, 2>, (const void*)k_mac<2, 2, 3>,
                         (const void*)k_mac<2, 2, 4>,
                         (const void*)k_mac_mfma<4, 2>, (const void*)k_mac_mfma<4, 4>,
-                        (const void*)k_mac_mfma<2, 2>, (const void*)k_mac_mfma<2, 4>};
+                        (const void*)k_mac_mfma<2, 2>, (const void*)k_mac_mfma<2, 4>,
+                        (const void*)k_small_fft_grouped, (const void*)k_ifft_cols_grouped<false>,
+                        (const void*)k_ifft_cols_grouped<true>,
+                        (const void*)k_mac_grouped<4, 1, 1>, (const void*)k_mac_grouped<4, 1, 2>,
+                        (const void*)k_mac_grouped<4, 1, 3>, (const void*)k_mac_grouped<4, 1, 4>,
+                        (const void*)k_mac_grouped<4, 2, 1>, (const void*)k_mac_grouped<4, 2, 2>,
+                        (const void*)k_mac_grouped<4, 2, 3>, (const void*)k_mac_grouped<4, 2, 4>,
+                        (const void*)k_mac_grouped<2, 1, 1>, (const void*)k_mac_grouped<2, 1, 2>,
+                        (const void*)k_mac_grouped<2, 1, 3>, (const void*)k_mac_grouped<2, 1, 4>,
+                        (const void*)k_mac_grouped<2, 2, 1>, (const void*)k_mac_grouped<2, 2, 2>,
+                        (const void*)k_mac_grouped<2, 2, 3>, (const void*)k_mac_grouped<2, 2, 4>};
     for (const void* k : ks) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
         if (e != hipSuccess) return e;

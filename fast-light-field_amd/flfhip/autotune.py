@@ -6,7 +6,8 @@ CalibrateBlockFactors, projector.py:628-675).
 Plan options come first, each candidate a fresh plan (spectra recomputed):
 the pass order of every FFT (``radix_order``: largest radices first,
 reversed, without radix 8, odd radices first) and the items per MAC thread
-(``mac_ipt``).  Then the launch shapes: the forward MAC's plane ranges
+(``mac_ipt``).  Then the launch shapes: one dispatch per stage for all grid
+groups or one per group (``grouped_launch``), the forward MAC's plane ranges
 (``fwd_split``), the frames per MAC workgroup (``mac_frames``, only when a
 call holds several frames), the columns per column-transform workgroup
 (``ifft_col_batch``) and the planes per back-projection MAC workgroup
@@ -21,6 +22,7 @@ PLAN_CANDIDATES = {
 }
 
 CANDIDATES = {
+    "grouped_launch": [1, 0],
     "fwd_split": [1, 2, 3, 4],
     "mac_frames": [1, 2, 3, 4],
     "ifft_col_batch": [2, 4, 8],

@@ -106,6 +106,10 @@ int flf_set_frame_dims(flf_ctx* ctx, const int* dims, int T, void* stream);
  *   "mac_mfma"       0/1: matrix-core MAC for >= 2 frames per workgroup
  *   "bwd_planes"     1..8 planes per back-projection MAC workgroup
  *   "ifft_col_batch" 1..8 columns per column-transform workgroup (after plan)
+ *   "grouped_launch" 0/1 (default 1): the small FFTs, the multiply-accumulate
+ *                    and the column transforms of all Fourier-grid groups of
+ *                    a projection go out as one dispatch each (bit-identical
+ *                    results); not with group_streams or mac_mfma
  *   "group_streams"  0/1 (default 0): the Fourier-grid groups of a projection
  *                    run side by side on library-owned streams, forked from
  *                    and joined back to the caller's stream by events (one
