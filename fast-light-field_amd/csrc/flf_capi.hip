@@ -458,6 +458,8 @@ int launch_grouped(flf_ctx* ctx, const std::vector<GroupWork>& ws, hipStream_t s
     for (const GroupWork& w : ws) {
         const Group& g = ctx->groups[w.gi];
         if (g.gt.slots != g0.slots || g.gt.ipt != g0.ipt) return FLF_OK;
+        // the grouped kernels hold the register-held transforms only
+        if (small_fft_big(g.gt.rY, g.gt.rX) || wave_big(g.gt.fY)) return FLF_OK;
         fb = std::min(fb, mac_frames(ctx, g.gt));
         CB = std::min(CB, g.CBi);
         prime = prime || plan_has_prime(w.ca.rpY);
@@ -619,12 +621,13 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
     if (rj) return rj;
     // every plane's row transform (and RL update) in one launch on st
     std::vector<int> ids;
-    int wbuf = 0, prime = 0;
+    int wbuf = 0, prime = 0, big = 0;
     for (int z = 0; z < ctx->nz; ++z)
         if (sel[z]) {
             ids.push_back(z);
             const Group& gz = ctx->groups[ctx->group_of[z]];
             wbuf = std::max(wbuf, c2r_wave_buffer(gz.gt.fX));
+            if (wave_big(gz.gt.fX)) big = 1;
             if (plan_has_prime(gz.rpXw) || plan_has_prime(gz.rpXhw)) prime = 1;
         }
     if (ids.empty()) return FLF_OK;
@@ -636,6 +639,7 @@ int run_backward(flf_ctx* ctx, const float* img, float* vol, const float* Htf, c
     r.mode = mode; r.T = ctx->T; r.Y = ctx->Y; r.X = ctx->X;
     r.wbuf = wbuf;
     r.prime = prime;
+    r.big = big;
     r.fdims = frame_dims(ctx);
     HIPCHK(launch_ifft_rows_plane(r, (int)ids.size(), st));
     return FLF_OK;
@@ -756,11 +760,13 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
         while (g.CBi > 1 && ifft_cols_lds(g.CBi, gt.fY) > ctx->max_lds) --g.CBi;
         if (g.RB < 1 || g.CBp < 1 || ifft_cols_lds(g.CBi, gt.fY) > ctx->max_lds)
             return fail(ctx, FLF_EINVAL, "Fourier grid too large for the LDS FFT");
-        // Wave-level transforms hold at most 2048 (columns, odd rows), 1024
-        // (half-length rows) and 128 (reduced grid) points.
-        if (gt.fY > 2048 || gt.fX > 2048 || gt.rY > 128 || gt.rX > 128)
+        // Wave-level transforms: up to 2048 points in registers, up to 4096
+        // out of place (two LDS buffers); reduced grids up to 256.
+        if (gt.fY > 4096 || gt.fX > 4096 || gt.rY > 256 || gt.rX > 256)
             return fail(ctx, FLF_EINVAL, "Fourier grid " + std::to_string(gt.fY) + "x" + std::to_string(gt.fX) +
-                                             " exceeds the transform sizes this library supports");
+                                             " exceeds the transform sizes this library supports (4096, reduced 256)");
+        if (ifft_rows_plane_lds(c2r_wave_buffer(gt.fX), wave_big(gt.fX)) > ctx->max_lds)
+            return fail(ctx, FLF_EINVAL, "image row too long for the row transform");
         if (small_fft_lds(gt.rY, gt.rX) > ctx->max_lds)
             return fail(ctx, FLF_EINVAL, "reduced grid too large for the small-FFT kernel");
         {
@@ -831,7 +837,9 @@ int flf_plan(flf_ctx* ctx, int nnum, int Y, int X, int T, int nz, const int* s2)
             rd.twh = g.twXh;
         }
         Tbwd_total += (long long)nP * Y * gt.rfX;
-        if (ifft_rows_sum_lds(gt.fX) > ctx->max_lds || X > 2048)
+        // (rows of a grid up to 2048 points hold X < 2048 samples; longer
+        // ones take the out-of-place kernels, which hold up to 4096)
+        if (ifft_rows_sum_lds(gt.fX) > ctx->max_lds || X > 4096)
             return fail(ctx, FLF_EINVAL, "image row too long for the row transform");
     }
     int rc;

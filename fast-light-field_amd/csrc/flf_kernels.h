@@ -171,13 +171,16 @@ struct IfftRowPlaneArgs {
     int wbuf;                    // per-wave LDS buffer (float2): max over planes of fX/2 (even) or fX
     const int2* fdims;           // per frame valid size; outside it the volume is set to 0
     int prime;                   // some plane's row plan ends with an odd prime radix above 5
+    int big;                     // some plane's rows are longer than 2048 (out-of-place transforms)
 };
 
 namespace flf {
 size_t small_fft_lds(int rY, int rX);
 size_t ifft_cols_lds(int CB, int fY);
 size_t ifft_rows_sum_lds(int maxfX);
-size_t ifft_rows_plane_lds(int wbuf);
+size_t ifft_rows_plane_lds(int wbuf, bool big);
+bool wave_big(int L);
+bool small_fft_big(int rY, int rX);
 int c2r_wave_buffer(int fX);
 hipError_t launch_small_fft(const SmallFFTArgs& a, int nsrc, int T, hipStream_t s);
 // Grouped launches (the groups' arguments by value).

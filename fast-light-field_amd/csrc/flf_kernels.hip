@@ -94,8 +94,10 @@ __device__ __forceinline__ GroupPick pick_group(const GroupedGrid& gg, int b) {
 
 // One (lenslet offset, frame, source plane) of a grid group; src0/src2/
 // tsplit: the source buffers of the call.
+template <bool BIG>
 __device__ __forceinline__ void small_fft_body(const SmallFFTArgs& a, const float* src0, const float* src2,
                                                int tsplit, int off, int t, int z) {
+    constexpr int RMAX = BIG ? 256 : 128;          // reduced grid length (BIG: 129..256)
     extern __shared__ float2 lds[];
     const int N = a.N;
     const int b = off / N, aa = off % N;
@@ -134,7 +136,7 @@ __device__ __forceinline__ void small_fft_body(const SmallFFTArgs& a, const floa
             }
         }
         wave_sync();
-        wave_fft_ns<128, 2>(P, 1, 2 * rXh, np, a.rpX, twX, false, lane);
+        wave_fft_ns<RMAX, 2>(P, 1, 2 * rXh, np, a.rpX, twX, false, lane);
         // Separate: row ya = (Z + Zc)/2, row yb = (Z - Zc)/(2i), Zc = conj Z(-k);
         // all reads before the in-place writes.
         float2 ev[2][2], od[2][2];
@@ -167,7 +169,7 @@ __device__ __forceinline__ void small_fft_body(const SmallFFTArgs& a, const floa
     }
     __syncthreads();
     for (int c = 2 * wv; c < rXh; c += 2 * SFFT_WAVES)
-        wave_fft_ns<128, 2>(H + c, rXh, 1, min(2, rXh - c), a.rpY, twY, false, lane);
+        wave_fft_ns<RMAX, 2>(H + c, rXh, 1, min(2, rXh - c), a.rpY, twY, false, lane);
     // Sub-sampling phases of this offset: exp(-2 pi i b kt0 / fY) and
     // exp(-2 pi i a kt1 / fX), kt the centred residue (0 for r/2).
     __syncthreads();
@@ -197,15 +199,16 @@ __device__ __forceinline__ void small_fft_body(const SmallFFTArgs& a, const floa
     }
 }
 
+template <bool BIG>
 __global__ __launch_bounds__(64 * SFFT_WAVES) void k_small_fft(SmallFFTArgs a) {
-    small_fft_body(a, a.src, a.src2, a.tsplit, xcd_block(blockIdx.x, a.N * a.N), blockIdx.y, blockIdx.z);
+    small_fft_body<BIG>(a, a.src, a.src2, a.tsplit, xcd_block(blockIdx.x, a.N * a.N), blockIdx.y, blockIdx.z);
 }
 // grid (N*N, T, sum over groups of their source planes): z flattened over
 // (group, plane).
 __global__ __launch_bounds__(64 * SFFT_WAVES) void k_small_fft_grouped(SmallFFTGrouped ga) {
     const GroupPick p = pick_group(ga.gg, blockIdx.z);
     const SmallFFTArgs& a = ga.a[p.g];
-    small_fft_body(a, a.src, a.src2, a.tsplit, xcd_block(blockIdx.x, a.N * a.N), blockIdx.y, p.rem);
+    small_fft_body<false>(a, a.src, a.src2, a.tsplit, xcd_block(blockIdx.x, a.N * a.N), blockIdx.y, p.rem);
 }
 
 // ---------------------------------------------------------------------------
@@ -931,12 +934,13 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void k_mac_mfma(MacArgs a) {
 // grid (ceil(rfX/CB), nbatch), one wave per column (blockDim = 64*CB); LDS
 // CB*fY + fY (twiddles).
 // ---------------------------------------------------------------------------
-template <bool PRIME>
+template <bool PRIME, bool BIG>
 __device__ __forceinline__ void ifft_cols_body(const IfftColArgs& a, int colblk, int batch) {
     extern __shared__ float2 lds[];
     const int fY = a.rpY.L, rfX = a.rfX, CB = a.CB;
     float2* C = lds;                 // [CB][fY]
-    float2* tw = C + CB * fY;        // [fY]
+    float2* C2 = C + CB * fY;        // BIG: [CB][fY], the second buffer of the out-of-place passes
+    float2* tw = BIG ? C2 + CB * fY : C2;   // [fY]
     const int c0 = colblk * CB;
     const int ncol = min(CB, rfX - c0);
     const float2* in = a.A + (long long)batch * a.A_bstride;
@@ -968,38 +972,44 @@ __device__ __forceinline__ void ifft_cols_body(const IfftColArgs& a, int colblk,
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (wv < ncol) wave_fft<2048, PRIME>(C + wv * fY, 1, a.rpY, tw, true, lane);
+    if constexpr (BIG) {
+        if (wv < ncol) wave_fft_pp<PRIME>(C + wv * fY, C2 + wv * fY, a.rpY, tw, true, lane);
+    } else {
+        if (wv < ncol) wave_fft<2048, PRIME>(C + wv * fY, 1, a.rpY, tw, true, lane);
+    }
     __syncthreads();
+    const float2* Res = (BIG && (a.rpY.n & 1)) ? C2 : C;   // after an odd pass count: the second buffer
     float2* out = a.out + (long long)batch * a.out_bstride;
     for (int i = threadIdx.x; i < a.Yout * CB; i += blockDim.x) {
         const int y = i / CB, cc = i - y * CB;
-        if (cc < ncol) out[(long long)y * rfX + c0 + cc] = C[cc * fY + y];
+        if (cc < ncol) out[(long long)y * rfX + c0 + cc] = Res[cc * fY + y];
     }
 }
-template <bool PRIME>
+template <bool PRIME, bool BIG>
 __global__ __launch_bounds__(512) void k_ifft_cols(IfftColArgs a) {
-    ifft_cols_body<PRIME>(a, blockIdx.x, blockIdx.y);
+    ifft_cols_body<PRIME, BIG>(a, blockIdx.x, blockIdx.y);
 }
 // All active grid groups: grid x flattened over (group, batch, column
 // block), nchunks[g] = column blocks of group g; one CB for every group.
 template <bool PRIME>
 __global__ __launch_bounds__(512) void k_ifft_cols_grouped(IfftColGrouped ga) {
     const GroupPick p = pick_group(ga.gg, blockIdx.x);
-    ifft_cols_body<PRIME>(ga.a[p.g], p.rem % p.nch, p.rem / p.nch);
+    ifft_cols_body<PRIME, false>(ga.a[p.g], p.rem % p.nch, p.rem / p.nch);
 }
 
 // c2r of one row by one wave.  row: rfX = L/2+1 half-spectrum values (global);
-// buf: LDS work area (L/2 entries for even L, L for odd L).  For even L the
+// buf: LDS work area (L/2 entries for even L, L for odd L; BIG, for L above
+// 2048: buf2 is a second such area for the out-of-place passes).  For even L the
 // length-L/2 complex transform of Z[k] = E[k] + i O[k],
 //   E = X[k] + conj(X[L/2-k]),  O = (X[k] - conj(X[L/2-k])) exp(+2 pi i k/L),
 // gives x[2m] = Re z[m], x[2m+1] = Im z[m]; odd L uses the Hermitian
 // extension and a full-length transform.  The imaginary parts of X[0] and
 // X[L/2] are dropped, as a c2r transform does.  Returns with the result in
 // buf (interleaved real samples for even L, real parts for odd L).
-template <bool PRIME>
-__device__ __forceinline__ void wave_c2r(float2* buf, const float2* __restrict__ row, int L,
-                                         const RadixPlan& rph, const float2* twh,
-                                         const RadixPlan& rpf, const float2* twf, int lane) {
+template <bool PRIME, bool BIG>
+__device__ __forceinline__ float2* wave_c2r(float2* buf, float2* buf2, const float2* __restrict__ row, int L,
+                                            const RadixPlan& rph, const float2* twh,
+                                            const RadixPlan& rpf, const float2* twf, int lane) {
     // Loads are issued C2R_BATCH per lane at a time, so that their latencies
     // overlap instead of adding up.
     constexpr int B = 8;
@@ -1031,7 +1041,9 @@ __device__ __forceinline__ void wave_c2r(float2* buf, const float2* __restrict__
             }
         }
         wave_sync();
+        if constexpr (BIG) return wave_fft_pp<PRIME>(buf, buf2, rph, twh, true, lane);
         wave_fft<1024, PRIME>(buf, 1, rph, twh, true, lane);
+        return buf;
     } else {
         const int rfX = L / 2 + 1;
         for (int k0 = lane; k0 < L; k0 += 64 * B) {
@@ -1052,7 +1064,9 @@ __device__ __forceinline__ void wave_c2r(float2* buf, const float2* __restrict__
             }
         }
         wave_sync();
+        if constexpr (BIG) return wave_fft_pp<PRIME>(buf, buf2, rpf, twf, true, lane);
         wave_fft<2048, PRIME>(buf, 1, rpf, twf, true, lane);
+        return buf;
     }
 }
 
@@ -1067,15 +1081,17 @@ __device__ __forceinline__ float c2r_sample(const float2* buf, int L, int x) {
 // Forward projection, stage 2: per output row, c2r each grid group's row and
 // sum the groups in a fixed order.  One wave per (row, frame);
 // grid (ceil(Y/4), T), block 256.
-template <bool PRIME>
+template <bool PRIME, bool BIG>
 __global__ __launch_bounds__(256) void k_ifft_rows_sum(IfftRowSumArgs a) {
     extern __shared__ float2 lds[];
+    constexpr int RPB = BIG ? 2 : 4;            // rows (waves) per workgroup
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int y = blockIdx.x * 4 + wv, t = blockIdx.y;
+    const int y = blockIdx.x * RPB + wv, t = blockIdx.y;
     float2* twf = lds;                          // [maxfX]
     float2* twh = twf + a.maxfX;                // [maxfX/2]
-    float2* buf = twh + a.maxfX / 2 + 1 + wv * a.maxfX;
-    constexpr int XPL = 2048 / 64;              // output samples per lane (X <= 2048)
+    float2* buf = twh + a.maxfX / 2 + 1 + wv * (BIG ? 2 : 1) * a.maxfX;
+    float2* buf2 = buf + a.maxfX;               // BIG only
+    constexpr int XPL = (BIG ? 4096 : 2048) / 64;   // output samples per lane (X <= 2048, BIG: 4096)
     float acc[XPL];
 #pragma unroll
     for (int q = 0; q < XPL; ++q) acc[q] = 0.f;
@@ -1090,12 +1106,12 @@ __global__ __launch_bounds__(256) void k_ifft_rows_sum(IfftRowSumArgs a) {
         __syncthreads();
         if (y < a.Y) {
             const float2* row = G.T + (long long)t * G.tstride + (long long)y * G.rfX;
-            wave_c2r<PRIME>(buf, row, L, G.rph, twh, G.rp, twf, lane);
+            const float2* res = wave_c2r<PRIME, BIG>(buf, buf2, row, L, G.rph, twh, G.rp, twf, lane);
             const float sc = G.scale;
 #pragma unroll
             for (int q = 0; q < XPL; ++q) {
                 const int x = lane + 64 * q;
-                if (x < a.X) acc[q] += c2r_sample(buf, L, x) * sc;
+                if (x < a.X) acc[q] += c2r_sample(res, L, x) * sc;
             }
             wave_sync();
         }
@@ -1128,7 +1144,7 @@ __global__ __launch_bounds__(256) void k_ifft_rows_sum(IfftRowSumArgs a) {
 // X *= v [/ trust]; NaN -> 0 after the update).  One wave per row;
 // grid (ceil(Y/4), T, nplanes), block 256.  Twiddles of every grid group are
 // read from global memory (they stay in the caches).
-template <bool PRIME>
+template <bool PRIME, bool BIG>
 __global__ __launch_bounds__(256) void k_ifft_rows_plane(IfftRowPlaneArgs a) {
     extern __shared__ float2 lds[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1136,10 +1152,11 @@ __global__ __launch_bounds__(256) void k_ifft_rows_plane(IfftRowPlaneArgs a) {
     const int z = a.plane_ids[blockIdx.z];      // kernel argument (scalar load)
     const RowPlaneDesc& pd = a.pd[z];           // read in place (no private copy)
     const int L = pd.rp.L;
-    float2* buf = lds + wv * a.wbuf;
+    float2* buf = lds + wv * (BIG ? 2 : 1) * a.wbuf;
     if (y >= a.Y) return;                       // no block-level barrier below
     const float2* row = a.Tbuf + pd.T_off + (long long)t * pd.T_tstride + (long long)y * pd.rfX;
-    wave_c2r<PRIME>(buf, row, L, pd.rph, pd.twh, pd.rp, pd.twf, lane);
+    const float2* res = wave_c2r<PRIME, BIG>(buf, buf + a.wbuf, row, L, pd.rph, pd.twh, pd.rp, pd.twf, lane);
+    buf = const_cast<float2*>(res);
     const float sc = pd.scale;
     const long long o = (((long long)z * a.T + t) * a.Y + y) * a.X;
     const int Yv = a.fdims ? a.fdims[t].x : a.Y, Xv = a.fdims ? a.fdims[t].y : a.X;
@@ -1272,9 +1289,15 @@ static bool has_prime(const RadixPlan& rp) { return plan_has_prime(rp); }
 size_t small_fft_lds(int rY, int rX) {
     return ((size_t)(rY + 1) * (rX / 2 + 1) + 2 * (size_t)(rY + rX)) * sizeof(float2);
 }
+// Reduced grids above 128 points take the 256-point wave transforms.
+bool small_fft_big(int rY, int rX) { return rY > 128 || rX > 128; }
 hipError_t launch_small_fft(const SmallFFTArgs& a, int nsrc, int T, hipStream_t s) {
     size_t lds = small_fft_lds(a.rpY.L, a.rpX.L);
-    hipLaunchKernelGGL(k_small_fft, dim3(a.N * a.N, T, nsrc), dim3(64 * SFFT_WAVES), lds, s, a);
+    if (a.rpY.L > 256 || a.rpX.L > 256) return hipErrorInvalidValue;
+    if (small_fft_big(a.rpY.L, a.rpX.L))
+        hipLaunchKernelGGL(k_small_fft<true>, dim3(a.N * a.N, T, nsrc), dim3(64 * SFFT_WAVES), lds, s, a);
+    else
+        hipLaunchKernelGGL(k_small_fft<false>, dim3(a.N * a.N, T, nsrc), dim3(64 * SFFT_WAVES), lds, s, a);
     return hipGetLastError();
 }
 hipError_t launch_small_fft_grouped(const SmallFFTGrouped& ga, int N, int T, size_t lds, hipStream_t s) {
@@ -1365,7 +1388,11 @@ hipError_t launch_mac_grouped(const MacGrouped& ga, int slots, int ipt, int fb, 
     }
     return hipGetLastError();
 }
-size_t ifft_cols_lds(int CB, int fY) { return ((size_t)CB * fY + fY) * sizeof(float2); }
+// Transforms longer than 2048 points run out of place (two LDS buffers).
+bool wave_big(int L) { return L > 2048; }
+size_t ifft_cols_lds(int CB, int fY) {
+    return ((size_t)CB * fY * (wave_big(fY) ? 2 : 1) + fY) * sizeof(float2);
+}
 hipError_t launch_ifft_cols_grouped(const IfftColGrouped& ga, int CB, bool prime, size_t lds, hipStream_t s) {
     const GroupedGrid& gg = ga.gg;
     if (CB < 1 || CB > 8 || gg.ngroups < 1 || gg.ngroups > FLF_MAX_GROUPS) return hipErrorInvalidValue;
@@ -1375,26 +1402,51 @@ hipError_t launch_ifft_cols_grouped(const IfftColGrouped& ga, int CB, bool prime
 }
 hipError_t launch_ifft_cols(const IfftColArgs& a, int nbatch, hipStream_t s) {
     size_t lds = ifft_cols_lds(a.CB, a.rpY.L);
-    if (has_prime(a.rpY)) hipLaunchKernelGGL(k_ifft_cols<true>, dim3((a.rfX + a.CB - 1) / a.CB, nbatch), dim3(64 * a.CB), lds, s, a);
-    else hipLaunchKernelGGL(k_ifft_cols<false>, dim3((a.rfX + a.CB - 1) / a.CB, nbatch), dim3(64 * a.CB), lds, s, a);
+    const dim3 grid((a.rfX + a.CB - 1) / a.CB, nbatch), block(64 * a.CB);
+    const bool big = wave_big(a.rpY.L);
+    if (has_prime(a.rpY)) {
+        if (big) hipLaunchKernelGGL((k_ifft_cols<true, true>), grid, block, lds, s, a);
+        else hipLaunchKernelGGL((k_ifft_cols<true, false>), grid, block, lds, s, a);
+    } else {
+        if (big) hipLaunchKernelGGL((k_ifft_cols<false, true>), grid, block, lds, s, a);
+        else hipLaunchKernelGGL((k_ifft_cols<false, false>), grid, block, lds, s, a);
+    }
     return hipGetLastError();
 }
-size_t ifft_rows_sum_lds(int maxfX) { return ((size_t)maxfX + maxfX / 2 + 1 + 4 * (size_t)maxfX) * sizeof(float2); }
+// Twiddles (full and half length), then per wave one buffer (two for rows
+// longer than 2048, which take two waves per workgroup instead of four).
+size_t ifft_rows_sum_lds(int maxfX) {
+    const bool big = wave_big(maxfX);
+    return ((size_t)maxfX + maxfX / 2 + 1 + (big ? 2 * 2 : 4) * (size_t)maxfX) * sizeof(float2);
+}
 hipError_t launch_ifft_rows_sum(const IfftRowSumArgs& a, int Y, int T, hipStream_t s) {
     size_t lds = ifft_rows_sum_lds(a.maxfX);
     bool prime = false;
     for (int g = 0; g < a.ngroups; ++g) prime = prime || (a.grp[g].active && (has_prime(a.grp[g].rp) || has_prime(a.grp[g].rph)));
-    if (prime) hipLaunchKernelGGL(k_ifft_rows_sum<true>, dim3((Y + 3) / 4, T), dim3(256), lds, s, a);
-    else hipLaunchKernelGGL(k_ifft_rows_sum<false>, dim3((Y + 3) / 4, T), dim3(256), lds, s, a);
+    if (wave_big(a.maxfX)) {
+        const dim3 grid((Y + 1) / 2, T), block(128);
+        if (prime) hipLaunchKernelGGL((k_ifft_rows_sum<true, true>), grid, block, lds, s, a);
+        else hipLaunchKernelGGL((k_ifft_rows_sum<false, true>), grid, block, lds, s, a);
+    } else {
+        const dim3 grid((Y + 3) / 4, T), block(256);
+        if (prime) hipLaunchKernelGGL((k_ifft_rows_sum<true, false>), grid, block, lds, s, a);
+        else hipLaunchKernelGGL((k_ifft_rows_sum<false, false>), grid, block, lds, s, a);
+    }
     return hipGetLastError();
 }
 // c2r of an even length runs as a half-length complex FFT in place.
 int c2r_wave_buffer(int fX) { return (fX & 1) ? fX : fX / 2; }
-size_t ifft_rows_plane_lds(int wbuf) { return 4 * (size_t)wbuf * sizeof(float2); }
+size_t ifft_rows_plane_lds(int wbuf, bool big) { return 4 * (size_t)wbuf * (big ? 2 : 1) * sizeof(float2); }
 hipError_t launch_ifft_rows_plane(const IfftRowPlaneArgs& a, int nplanes, hipStream_t s) {
-    size_t lds = ifft_rows_plane_lds(a.wbuf);
-    if (a.prime) hipLaunchKernelGGL(k_ifft_rows_plane<true>, dim3((a.Y + 3) / 4, a.T, nplanes), dim3(256), lds, s, a);
-    else hipLaunchKernelGGL(k_ifft_rows_plane<false>, dim3((a.Y + 3) / 4, a.T, nplanes), dim3(256), lds, s, a);
+    size_t lds = ifft_rows_plane_lds(a.wbuf, a.big != 0);
+    const dim3 grid((a.Y + 3) / 4, a.T, nplanes), block(256);
+    if (a.prime) {
+        if (a.big) hipLaunchKernelGGL((k_ifft_rows_plane<true, true>), grid, block, lds, s, a);
+        else hipLaunchKernelGGL((k_ifft_rows_plane<true, false>), grid, block, lds, s, a);
+    } else {
+        if (a.big) hipLaunchKernelGGL((k_ifft_rows_plane<false, true>), grid, block, lds, s, a);
+        else hipLaunchKernelGGL((k_ifft_rows_plane<false, false>), grid, block, lds, s, a);
+    }
     return hipGetLastError();
 }
 hipError_t launch_fit(const float* img, const float* im, long long n_per_frame, int T, int im_frames, double* part,
@@ -1404,11 +1456,15 @@ hipError_t launch_fit(const float* img, const float* im, long long n_per_frame, 
     return hipGetLastError();
 }
 hipError_t configure_kernels(size_t max_lds) {
-    const void* ks[] = {(const void*)k_small_fft, (const void*)k_psf_rows<false>, (const void*)k_psf_cols<false>,
-                        (const void*)k_ifft_cols<false>, (const void*)k_ifft_rows_sum<false>,
-                        (const void*)k_ifft_rows_plane<false>, (const void*)k_psf_rows<true>,
-                        (const void*)k_psf_cols<true>, (const void*)k_ifft_cols<true>,
-                        (const void*)k_ifft_rows_sum<true>, (const void*)k_ifft_rows_plane<true>,
+    const void* ks[] = {(const void*)k_small_fft<false>, (const void*)k_small_fft<true>,
+                        (const void*)k_psf_rows<false>, (const void*)k_psf_cols<false>,
+                        (const void*)k_psf_rows<true>, (const void*)k_psf_cols<true>,
+                        (const void*)k_ifft_cols<false, false>, (const void*)k_ifft_cols<true, false>,
+                        (const void*)k_ifft_cols<false, true>, (const void*)k_ifft_cols<true, true>,
+                        (const void*)k_ifft_rows_sum<false, false>, (const void*)k_ifft_rows_sum<true, false>,
+                        (const void*)k_ifft_rows_sum<false, true>, (const void*)k_ifft_rows_sum<true, true>,
+                        (const void*)k_ifft_rows_plane<false, false>, (const void*)k_ifft_rows_plane<true, false>,
+                        (const void*)k_ifft_rows_plane<false, true>, (const void*)k_ifft_rows_plane<true, true>,
                         (const void*)k_mac<4, 1, 1>, (const void*)k_mac<4, 1, 2>, (const void*)k_mac<4, 1, 3>,
                         (const void*)k_mac<4, 1, 4>, (const void*)k_mac<4, 2, 1>, (const void*)k_mac<4, 2, 2>,
                         (const void*)k_mac<4, 2, 3>, (const void*)k_mac<4, 2, 4>, (const void*)k_mac<2, 1, 1>,

@@ -162,6 +162,109 @@ __device__ void wave_fft(float2* buf, int es, const RadixPlan& rp, const float2*
     }
 }
 
+// Out-of-place forms for transforms longer than the register-held passes
+// cover (grids above 2048 points): each pass reads one LDS buffer and writes
+// the other, and a lane takes its butterflies one at a time, so no register
+// array grows with the length.  Contiguous sequences (element stride 1).
+template <int R>
+__device__ __forceinline__ void wave_pass_oop(const float2* src, float2* dst, int L, int Ns,
+                                              const float2* __restrict__ tw, bool inv, int lane) {
+    const int M = L / R;
+    const int tstep = L / (Ns * R);
+    const float rNs = 1.0f / (float)Ns;
+    const float sign = inv ? 1.f : -1.f;
+    for (int j = lane; j < M; j += 64) {
+        int k;
+        const int g = div_small(j, Ns, rNs, k);
+        float2 a[FLF_RADIX_MAX];
+#pragma unroll
+        for (int t = 0; t < R; ++t) a[t] = src[j + t * M];
+        if (k != 0) twiddle_powers<R>(a, tw, k * tstep, inv);
+        dft_small<R>(a, sign);
+        const int o = g * Ns * R + k;
+#pragma unroll
+        for (int t = 0; t < R; ++t) dst[o + t * Ns] = a[t];
+    }
+    wave_sync();
+}
+
+// The prime last pass (Ns = L / P) out of place: y_q of butterfly j goes to
+// dst[j + q M], the positions its inputs held in src.
+__device__ __forceinline__ void wave_pass_prime_oop(const float2* src, float2* dst, int L, int P,
+                                                    const float2* __restrict__ tw, bool inv, int lane) {
+    const int M = L / P;
+    for (int j = lane; j < M; j += 64) {
+        float2 a[FLF_PRIME_MAX];
+        a[0] = src[j];
+        {
+            float2 w1 = tw[j];
+            if (inv) w1.y = -w1.y;
+            float2 w = w1;
+#pragma unroll
+            for (int t = 1; t < FLF_PRIME_MAX; ++t) {
+                if (t < P) {
+                    if ((t & 7) == 0) {
+                        w = tw[j * t];
+                        if (inv) w.y = -w.y;
+                    }
+                    a[t] = cmul(src[j + t * M], w);
+                    w = cmul(w, w1);
+                }
+            }
+        }
+        for (int q = 0; q < P; ++q) {
+            float2 wq = tw[q * M];
+            if (inv) wq.y = -wq.y;
+            float2 y = a[0], w = wq;
+            int m = q;
+#pragma unroll
+            for (int t = 1; t < FLF_PRIME_MAX; ++t) {
+                if (t < P) {
+                    if ((t & 7) == 0) {
+                        w = tw[m * M];
+                        if (inv) w.y = -w.y;
+                    }
+                    y = cadd(y, cmul(a[t], w));
+                    w = cmul(w, wq);
+                    m += q;
+                    if (m >= P) m -= P;
+                }
+            }
+            dst[j + q * M] = y;
+        }
+    }
+    wave_sync();
+}
+
+// Whole transform between buffers a and b (length rp.L each); returns the
+// buffer that holds the result (a after an even number of passes).
+template <bool PRIME>
+__device__ float2* wave_fft_pp(float2* a, float2* b, const RadixPlan& rp, const float2* __restrict__ tw, bool inv,
+                               int lane) {
+    int Ns = 1;
+    float2* src = a;
+    float2* dst = b;
+    for (int p = 0; p < rp.n; ++p) {
+        const int R = rp.r[p];
+        switch (R) {
+            case 8: wave_pass_oop<8>(src, dst, rp.L, Ns, tw, inv, lane); break;
+            case 6: wave_pass_oop<6>(src, dst, rp.L, Ns, tw, inv, lane); break;
+            case 4: wave_pass_oop<4>(src, dst, rp.L, Ns, tw, inv, lane); break;
+            case 2: wave_pass_oop<2>(src, dst, rp.L, Ns, tw, inv, lane); break;
+            case 3: wave_pass_oop<3>(src, dst, rp.L, Ns, tw, inv, lane); break;
+            case 5: wave_pass_oop<5>(src, dst, rp.L, Ns, tw, inv, lane); break;
+            default:
+                if constexpr (PRIME) wave_pass_prime_oop(src, dst, rp.L, R, tw, inv, lane);   // last pass
+                break;
+        }
+        Ns *= R;
+        float2* t = src;
+        src = dst;
+        dst = t;
+    }
+    return src;
+}
+
 // NS sequences per call (sequence s at buf + s * ss, element stride es),
 // nseq <= NS of them present: the butterflies of all sequences share the
 // wave's lanes, which a single short sequence fills only partly.

@@ -464,11 +464,11 @@ def test_empty_plane_list(projector):
 
 
 def test_largest_grid_adjoint_and_linearity(projector):
-    """The largest Fourier grid the kernels take (1920 = 15 x 128: a 1575 px
-    image with a 345 px PSF): back projection through Ht = adjoint_quadrant(H)
-    is the adjoint of forward projection, <H x, y> = <x, Ht y>, and forward
-    projection is linear; one lenslet row more needs a 135-point reduced grid
-    and is refused at plan time.  The PSFs are rings (radially symmetric, as
+    """The largest Fourier grid of the register-held transforms (1920 = 15 x
+    128: a 1575 px image with a 345 px PSF): back projection through Ht =
+    adjoint_quadrant(H) is the adjoint of forward projection, <H x, y> =
+    <x, Ht y>, and forward projection is linear; an image that needs a
+    reduced grid above 256 points is refused at plan time.  The PSFs are rings (radially symmetric, as
     physical light-field PSFs are): only then does the adjoint set keep the
     8-fold symmetry that quadrant storage of Ht assumes (for an arbitrary
     quadrant it does not, in the reference's storage as here)."""
@@ -496,7 +496,49 @@ def test_largest_grid_adjoint_and_linearity(projector):
     ref = 2.0 * Hx.double() + 3.0 * projector.forward_device(hm, x2).double()
     assert float(torch.linalg.vector_norm(lin.double() - ref) / torch.linalg.vector_norm(ref)) < 1e-6
     with pytest.raises(ValueError, match="exceeds"):
-        projector.prepare(hm, 1, Y + 15, X + 15)
+        projector.prepare(hm, 1, 3750, 3750)        # (3750 + 344) / 15 -> 274 > 256
+
+
+@pytest.mark.parametrize("N,Y,X,s2,grid", [(15, 150, 2000, 345, (540, 2400)), (15, 2000, 150, 345, (2400, 540)),
+                                           (19, 190, 2200, 171, (380, 2375)), (15, 1800, 1800, 345, (2160, 2160))])
+def test_large_grids_vs_oracle(projector, N, Y, X, s2, grid):
+    """Fourier grids above 2048 points (out-of-place wave transforms in the
+    column and row transforms) and reduced grids above 128 (the 256-point
+    small FFT): rows of 2400 (even) and 2375 (odd, Nnum 19: with the prime
+    pass), columns of 2400, and a square 2160 grid; forward and backward
+    projection of one ring-PSF plane within 1e-5 of the FP64 oracle."""
+    from flfhip.psfmatrix import MemoryHMatrix, adjoint_quadrant
+    rng = np.random.default_rng(N + Y)
+    Q = synthetic.ring_quadrant(N, s2, s2 / 4.0, 1.5, rng).astype(np.float32)
+    hm = MemoryHMatrix([Q], [adjoint_quadrant(Q)])
+    x = synthetic.blob_volume(1, Y, X, 60, seed=N + X)
+    got = projector.ForwardProjectACC(hm, x, [0])
+    assert projector.fshape(0) == grid
+    ref = oproj.project_special(x[0], Q)
+    e = rel_l2(got, ref)
+    back = projector.BackwardProjectACC(hm, got, [0])[0]
+    eb = rel_l2(back, oproj.project_special(got, hm.Hcc(0, True)))
+    print("N %d %dx%d s2 %d grid %s: forward %.2e backward %.2e" % (N, Y, X, s2, grid, e, eb), flush=True)
+    assert e < 1e-5 and eb < 1e-5
+
+
+def test_camera_frame_2048(projector):
+    """A 2048 x 2048 camera frame at Nnum 15 with a 345 px PSF (grid 2400 x
+    2400, reduced 160): forward projection of one plane against the FP64
+    oracle (about 20 s of CPU) and an RL iteration that stays finite."""
+    from flfhip.lfdeconv import DeconvRL
+    from flfhip.psfmatrix import MemoryHMatrix, adjoint_quadrant
+    rng = np.random.default_rng(2048)
+    Q = synthetic.ring_quadrant(15, 345, 90.0, 2.0, rng).astype(np.float32)
+    hm = MemoryHMatrix([Q], [adjoint_quadrant(Q)])
+    x = synthetic.blob_volume(1, 2048, 2048, 400, seed=3)
+    got = projector.ForwardProjectACC(hm, x, [0])
+    assert projector.fshape(0) == (2400, 2400)
+    e = rel_l2(got, oproj.project_special(x[0], Q))
+    print("2048^2 frame: forward rel-L2 %.2e" % e, flush=True)
+    assert e < 1e-5
+    X1 = DeconvRL(hm, None, 1, None, im=got + 0.01, projector=projector, logPrint=False)
+    assert np.isfinite(X1).all() and X1.shape == (1, 1, 2048, 2048)
 
 
 def test_full_size_plane_vs_oracle(projector):
