@@ -80,8 +80,8 @@ struct flf_ctx {
     int fb_max = 2;                   // frames per MAC workgroup at T >= 2 (FLF_MAC_FRAMES overrides)
     int fwd_split = 0;                // forward plane ranges, 0 = automatic (FLF_FWD_SPLIT overrides)
     int bwd_planes = 1;               // planes per backward MAC workgroup
-    bool mfma = false;                // matrix-core MAC for >= 2 frames per workgroup (FLF_MAC_MFMA=1 enables;
-                                      // measured slower than the vector kernel, see the kernel's comment)
+    bool mfma = false;                // matrix-core MAC (k_mac_tc, 8 frames per workgroup) for calls of >= 2
+                                      // frames (option mac_mfma, FLF_MAC_MFMA=1; chosen by the autotuner)
     // Fourier-grid groups of one projection side by side on streams of
     // their own (default priority, non-blocking), forked from and joined
     // back to the caller's stream by events; option "group_streams", off by
@@ -383,7 +383,9 @@ int run_mac(flf_ctx* ctx, const MacArgs& m, const Group& g, int nplanes_grid, hi
         eb = take_event(ctx);
         if (ea && eb) HIPCHK(hipEventRecord(ea, st));
     }
-    HIPCHK(launch_mac(m, gt.slots, gt.ipt, mac_frames(ctx, gt), (int)gt.chunks.size(), nplanes_grid, ctx->mfma, st));
+    // the matrix-core kernel (8 frames per workgroup) for calls of several frames
+    const bool tc = ctx->mfma && m.T >= 2 && m.tiles && mac_tc_lds(gt.slots, m.N) <= ctx->max_lds;
+    HIPCHK(launch_mac(m, gt.slots, gt.ipt, mac_frames(ctx, gt), (int)gt.chunks.size(), nplanes_grid, tc, st));
     if (ctx->prof && ea && eb) {
         HIPCHK(hipEventRecord(eb, st));
         int np = 0;
@@ -569,10 +571,9 @@ int run_backward_groups(flf_ctx* ctx, const float* img, const std::vector<char>&
         m.T = T;
         m.rs = mac_row_stride(gt.slots, gt.nu);
         // Each workgroup takes bwd_planes consecutive planes (loads run on
-        // across the plane boundary) and stores every plane's accumulator;
-        // the matrix-core kernel keeps one plane per workgroup.
+        // across the plane boundary) and stores every plane's accumulator.
         m.per_plane = 1;
-        const int bp = ctx->mfma ? 1 : std::max(1, ctx->bwd_planes);
+        const int bp = std::max(1, ctx->bwd_planes);
         w.nranges = (m.nP + bp - 1) / bp;
         IfftColArgs& ic = w.ca;
         ic.A = g.d_A_bwd;

@@ -191,6 +191,7 @@ hipError_t launch_psf_rows(const PsfRowArgs& a, int nu, hipStream_t s);
 hipError_t launch_psf_cols(const PsfColArgs& a, int nu, hipStream_t s);
 hipError_t launch_spec_pack(const SpecPackArgs& a, int nchunks, hipStream_t s);
 int mac_row_stride(int slots, int nu);
+size_t mac_tc_lds(int slots, int N);
 size_t mac_lds(int slots, int nu, int N, int fb);
 hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks, int nplanes_grid, bool mfma,
                       hipStream_t s);

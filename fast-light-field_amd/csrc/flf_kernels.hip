@@ -717,173 +717,247 @@ void k_mac_grouped(MacGrouped ga) {
     mac_body<SLOTS, IPT, FB>(ga.a[p.g], w % p.nch, w / p.nch, p.nr, blockIdx.z);
 }
 
-// Matrix-core variant for FB >= 2 frames per workgroup.  The items of a chunk
-// are grouped into tiles of <= 16 items of one residue class (TileDesc); for a
-// tile and a stored spectrum u the MAC of all its items and frames is the
-// product  acc[item][(f, j, re/im)] += sum_k g[item][k] W_u[k][(f, j, re/im)]
-// with k = (slot i, Re/Im g_i) and W_u made of the same P/M table entries as
-// the vector kernel (P for Re g, sg*M for Im g).  It runs on
-// v_mfma_f32_16x16x4_f32: lane l supplies A[l % 16][l / 16] (one float of the
-// item's slot-pair float4, read straight from the cached spectra),
-// B[l / 16][l % 16] (one float of the P/M table in LDS), and holds
-// D[4 (l / 16) + r][l % 16].  The two lanes of a column pair (re, im) swap
-// values to apply the complex epilogue phases.  A workgroup has MFMA_WAVES
-// waves; wave w owns tiles w, w + MFMA_WAVES, ...; FP32 operands and
-// accumulation throughout.  Opt-in (FLF_MAC_MFMA=1): measured 2.2x slower
-// than the vector kernel at 4 frames on MI355X, with 5.8 LDS bank-conflict
-// cycles per B-operand read (the 64 lanes gather P/M entries of up to 16
-// table rows that share banks), so the vector kernel stays the default.
-constexpr int MFMA_WAVES = 8;
-template <int SLOTS, int FB>
-__global__ __launch_bounds__(64 * MFMA_WAVES) void k_mac_mfma(MacArgs a) {
-    extern __shared__ float4 pm_tab[];            // [FB][NRES][rs], then staging
-    constexpr int NPT = SLOTS == 4 ? 4 : 2;
+// Matrix-core multiply-accumulate, TC_FB = 8 frames per workgroup
+// (v_mfma_f32_16x16x4_f32: FP32 operands and accumulation, as the vector
+// kernel).  The pair type of source slot i at output slot j is i XOR j
+// (kQuadPT, kPairPT), so with the P/M table of one residue rho as the B
+// operand,
+//   out_j[item][(f, c)] = sum_{u, pt, r} A_j[item][(u, pt, r)] B_rho[(u, pt, r)][(f, c)],
+//   B_rho[(u, pt, Re)][(f, c)] = P_c,   B_rho[(u, pt, Im)][(f, c)] = M_c,
+//   A_j[item][(u, pt, r)] = component r of the stored value of slot pt XOR j
+//                           (times the M sign of (j, i) for r = Im),
+// one B per residue serves every output slot and every item of that
+// residue; the per-slot permutation is a choice between two registers of
+// the A operand.  Items come in tiles of <= 16 of one residue class
+// (TileDesc), so each slot's residue is uniform within a tile.  B lives in
+// LDS in the MFMA's operand order (lane l reads float l of a 64-float block:
+// no bank conflicts), built per batch of TC_UB stored spectra from the
+// cached small-FFT values; lane l supplies A[l % 16][l / 16] (its tile row's
+// item, read from the cached spectra: each value once per frame block) and
+// holds D[4 (l / 16) + r][l % 16] (items 4 (l / 16) + r, column (f, c) =
+// l % 16).  Wave w owns tiles w, w + TC_WAVES, ...
+constexpr int TC_FB = 8;
+constexpr int TC_WAVES = 8;
+constexpr int TC_UB = 12;
+template <int SLOTS>
+__global__ __launch_bounds__(64 * TC_WAVES) void k_mac_tc(MacArgs a) {
+    extern __shared__ float4 tc_lds[];
     constexpr int NRES = SLOTS == 4 ? 8 : 4;
     constexpr int NH = NRES / 2;
-    constexpr int CPF = 2 * SLOTS;                // output columns per frame: (j, re/im)
-    constexpr int NT = (FB * CPF + 15) / 16;      // 16-column tiles
-    constexpr int KS = SLOTS / 2;                 // 4-deep k steps per spectrum (slot pairs)
-    constexpr int MAXT = (FLF_MAX_TILES + MFMA_WAVES - 1) / MFMA_WAVES;   // tiles per wave
+    constexpr int NPT = SLOTS == 4 ? 4 : 2;
+    constexpr int KS = SLOTS / 2;                 // 4-deep k steps per stored spectrum
+    constexpr int NUSE = SLOTS == 4 ? 8 : 4;
+    constexpr int MAXT = (FLF_MAX_TILES + TC_WAVES - 1) / TC_WAVES;
+    float* Bt = reinterpret_cast<float*>(tc_lds);                          // [TC_UB][NRES][KS][64]
+    float2* psi = reinterpret_cast<float2*>(Bt + TC_UB * NRES * KS * 64);  // [NRES][2]
+    float2* s_stage = psi + 2 * NRES;                                      // [TC_FB][N*N][NH]
     const int cidx = xcd_block(blockIdx.x, gridDim.x);
     const ChunkDesc* ch = a.chunks + cidx;
     const long long item_base = ch->item_base;
     const int n_items = ch->n_items;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int g = lane >> 4, ln = lane & 15;
-    const int N = a.N, NN = N * N, fourN = 4 * N;
-    const int t0 = blockIdx.z * FB;
-    const int nf = min(FB, a.T - t0);
+    const int row = lane & 15, kq = lane >> 4;
+    const int col = row, f_col = col >> 1, c_col = col & 1;
+    const int N = a.N, NN = N * N, c = N / 2, fourN = 4 * N;
+    const int t0 = blockIdx.z * TC_FB;
+    const int nf = min(TC_FB, a.T - t0);
     const int nu = a.nu;
-    const int rs = a.rs;
-    const int tab = NRES * rs;
-    float2* s_stage = reinterpret_cast<float2*>(pm_tab + FB * tab);   // [FB][NN][NH]
-    float2* psi = s_stage + FB * NN * NH;
-    const float* pmf = reinterpret_cast<const float*>(pm_tab);
     const TileDesc* tiles = a.tiles + ch->tile_base;
     const int ntiles = ch->n_tiles;
+    const long long ustride = (long long)n_items * SLOTS;
 
-    // This wave's tiles q: chunk tile wv + MFMA_WAVES q.  Per tile: A offset of the
-    // lane's item (floats, within a spectrum block) or -1, and the B table
-    // offsets (floats, per k step and column tile; bit 30 = negate, -1 = 0).
-    int aoff[MAXT];
-    int boff[MAXT][KS][NT];
+    int it_of[MAXT];                              // this lane's A-row item (chunk-relative), -1 if none
 #pragma unroll
     for (int q = 0; q < MAXT; ++q) {
-        const int tq = wv + MFMA_WAVES * q;
-        aoff[q] = -1;
-#pragma unroll
-        for (int s = 0; s < KS; ++s)
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt) boff[q][s][nt] = -1;
+        const int tq = wv + TC_WAVES * q;
+        it_of[q] = -1;
         if (tq < ntiles) {
             const TileDesc td = tiles[tq];
-            if (ln < td.count) aoff[q] = (td.start + ln) * 4 + g;
-#pragma unroll
-            for (int s = 0; s < KS; ++s) {
-                const int i = 2 * s + (g >> 1), part = g & 1;
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    const int gc = nt * 16 + ln;
-                    const int f = gc / CPF, jj = (gc % CPF) >> 1, cc = gc & 1;
-                    if (f < FB) {
-                        const int pt = SLOTS == 4 ? kQuadPT[jj][i] : kPairPT[jj][i];
-                        const int sg = SLOTS == 4 ? kQuadSG[jj][i] : 1;
-                        const int off = (f * tab + td.res[jj] * rs + pt) * 4 + part * 2 + cc;
-                        boff[q][s][nt] = off | ((part && sg < 0) ? (1 << 30) : 0);
-                    }
-                }
-            }
+            if (row < td.count) it_of[q] = td.start + row;
         }
     }
-
-    v4f tot[MAXT][NT];
+    v4f tot[MAXT][SLOTS];
 #pragma unroll
     for (int q = 0; q < MAXT; ++q)
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) tot[q][nt] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < SLOTS; ++j) tot[q][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
-    const float* Gf = reinterpret_cast<const float*>(a.G);
-    auto load_a = [&](float (&av)[MAXT][KS], int p, int u) {
-        const float* blk = Gf + (item_base * SLOTS * nu * a.nP + ((long long)p * nu + u) * n_items * SLOTS) * 2;
+    // Store: lane (f, c) writes component c of its items' slot-j bins.
+    auto store = [&](int slice) {
+        if (f_col >= nf) return;
+        float* A = reinterpret_cast<float*>(a.A + (long long)(t0 + f_col) * a.A_tstride +
+                                            (long long)slice * a.A_pstride);
 #pragma unroll
-        for (int q = 0; q < MAXT; ++q)
+        for (int q = 0; q < MAXT; ++q) {
+            const int tq = wv + TC_WAVES * q;
+            if (tq >= ntiles) continue;                                   // wave-uniform
+            const TileDesc td = tiles[tq];
 #pragma unroll
-            for (int s = 0; s < KS; ++s) av[q][s] = aoff[q] >= 0 ? blk[s * n_items * 4 + aoff[q]] : 0.f;
+            for (int r = 0; r < 4; ++r) {
+                const int ir = 4 * kq + r;
+                if (ir < td.count) {
+                    const ItemDesc& it = a.items[item_base + td.start + ir];
+#pragma unroll
+                    for (int j = 0; j < SLOTS; ++j)
+                        if ((it.writemask >> j) & 1) A[2LL * it.pos[j] + c_col] = tot[q][j][r];
+                }
+            }
+        }
     };
 
     const int pb = (int)(((long long)blockIdx.y * a.nP) / gridDim.y);
     const int pe = (int)(((long long)(blockIdx.y + 1) * a.nP) / gridDim.y);
     for (int p = pb; p < pe; ++p) {
         if (!a.enable.on(p)) continue;
-        mac_tables<SLOTS>(a, ch, cidx, p, t0, nf, pm_tab, s_stage, psi, rs);
-        v4f acc[MAXT][NT];
+        const int s2 = a.s2[p];
+        __syncthreads();                          // the previous plane is done with Bt and psi
+        if (tid < 2 * NRES) {                     // flip factors per residue: exp(-2 pi i (s2-1) kt / f)
+            const int rho = tid >> 1;
+            const bool isY = (tid & 1) == 0;
+            const int k = isY ? ch->res0[rho] : ch->res1[rho];
+            const int r = isY ? a.rY : a.rX;
+            const int f = isY ? a.fY : a.fX;
+            const int kt = (2 * k < r) ? k : ((2 * k > r) ? k - r : 0);
+            int m = (int)(((long long)(s2 - 1) * kt) % f);
+            if (m < 0) m += f;
+            float sn, cs;
+            sincospif(-2.0f * (float)m / (float)f, &sn, &cs);
+            psi[tid] = make_float2(cs, sn);
+        }
+        v4f acc[MAXT][SLOTS];
 #pragma unroll
         for (int q = 0; q < MAXT; ++q)
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) acc[q][nt] = v4f{0.f, 0.f, 0.f, 0.f};
-        // Ring of PD spectra in flight: buffer d holds spectrum u0 + d; after
-        // its products, the loads of spectrum u0 + d + PD are issued into it.
-        constexpr int PD = 4;
-        float av[PD][MAXT][KS];
-#pragma unroll
-        for (int d = 0; d < PD; ++d) load_a(av[d], p, min(d, nu - 1));
-        auto step = [&](float (&cur)[MAXT][KS], int u) {
-            const int ub = u * NPT * 4;
-#pragma unroll
-            for (int q = 0; q < MAXT; ++q) {
-                if (wv + MFMA_WAVES * q >= ntiles) continue;                // wave-uniform: tile absent
-#pragma unroll
-                for (int s = 0; s < KS; ++s)
-#pragma unroll
-                    for (int nt = 0; nt < NT; ++nt) {
-                        const int bo = boff[q][s][nt];
-                        float bv = 0.f;
-                        if (bo >= 0) {
-                            bv = pmf[(bo & ((1 << 30) - 1)) + ub];
-                            if (bo & (1 << 30)) bv = -bv;
-                        }
-                        acc[q][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur[q][s], bv, acc[q][nt], 0, 0, 0);
-                    }
-            }
-        };
-        for (int u0 = 0; u0 < nu; u0 += PD) {
-#pragma unroll
-            for (int d = 0; d < PD; ++d) {
-                const int u = u0 + d;
-                if (u < nu) step(av[d], u);
-                if (u + PD < nu) load_a(av[d], p, u + PD);
+            for (int j = 0; j < SLOTS; ++j) acc[q][j] = v4f{0.f, 0.f, 0.f, 0.f};
+        const float2* Gp = a.G + item_base * SLOTS * nu * a.nP + ((long long)p * nu) * ustride;
+        {
+            // the chunk's small-FFT block of every frame, staged once per plane
+            const int blk = NN * NH;
+            const float2* Sp = a.S + a.S_off[p] + (long long)cidx * blk;
+            for (int i = tid; i < nf * blk; i += blockDim.x) {
+                const int f = i / blk, e = i - f * blk;
+                s_stage[i] = Sp[(long long)(t0 + f) * a.S_tstride + e];
             }
         }
-        // Epilogue of plane p: outer phase and 'same'-crop shift, per item and
-        // output slot (the same for every frame).
-        const int s2 = a.s2[p];
+        for (int u0 = 0; u0 < nu; u0 += TC_UB) {
+            const int ubn = min(TC_UB, nu - u0);
+            __syncthreads();                      // psi written; the previous batch's products are done
+            // B of spectra u0 .. u0 + ubn - 1: one (P, M) pair per (spectrum, residue, pair type, frame)
+            for (int e = tid; e < TC_UB * NRES * NPT * TC_FB; e += blockDim.x) {
+                const int f = e % TC_FB;
+                int rest = e / TC_FB;
+                const int pt = rest % NPT;
+                rest /= NPT;
+                const int rho = rest % NRES;
+                const int ub = rest / NRES;
+                float4 pm = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (ub < ubn && f < nf) {
+                    const int u = u0 + ub;
+                    const int rb0 = rho < NH ? rho : rho - NH;
+                    const float2* sf = s_stage + f * NN * NH;
+                    float2 sv[2];
+#pragma unroll
+                    for (int w = 0; w < 2; ++w) {
+                        const int off = a.uoff[u * NUSE + (w == 0 ? kPairPlain[pt] : kPairConj[pt])];
+                        float2 r = make_float2(0.f, 0.f);
+                        if (off < NN) {
+                            const int bo = off / N, ao = off - bo * N;
+                            r = sf[off * NH + rb0];
+                            if (rho >= NH) r = cconj(r);
+                            if (bo > c) r = cmul(r, psi[2 * rho]);
+                            if (ao > c) r = cmul(r, psi[2 * rho + 1]);
+                        }
+                        sv[w] = r;
+                    }
+                    const float2 P = cadd(sv[0], sv[1]);
+                    const float2 d = csub(sv[0], sv[1]);
+                    pm = make_float4(P.x, P.y, -d.y, d.x);                    // M = i * d
+                }
+                float* b = Bt + ((ub * NRES + rho) * KS + (pt >> 1)) * 64 + 2 * (pt & 1) * 16 + 2 * f;
+                b[0] = pm.x;                      // k row 2 (pt & 1): Re g -> P
+                b[1] = pm.y;
+                b[16] = pm.z;                     // k row 2 (pt & 1) + 1: Im g -> M
+                b[17] = pm.w;
+            }
+            __syncthreads();
+            // Spectra outer, tiles inner: the loads of spectrum ub + 1 for all
+            // of this wave's tiles are in flight during the products of ub.
+            int res_q[MAXT][SLOTS];
+#pragma unroll
+            for (int q = 0; q < MAXT; ++q) {
+                const int tq = min(wv + TC_WAVES * q, ntiles - 1);
+                const TileDesc td = tiles[tq];
+#pragma unroll
+                for (int j = 0; j < SLOTS; ++j) res_q[q][j] = td.res[j];
+            }
+            auto load_u = [&](float2 (&gq)[MAXT][4], int ub) {
+#pragma unroll
+                for (int q = 0; q < MAXT; ++q) {
+                    const int it = it_of[q];
+                    load_g<SLOTS>(gq[q], Gp + (long long)(u0 + ub) * ustride, it >= 0 ? it : 0, n_items);
+                }
+            };
+            float2 gA[MAXT][4], gB[MAXT][4];
+            load_u(gA, 0);
+            auto products = [&](float2 (&gq)[MAXT][4], int ub) {
+                const float* bu = Bt + ub * NRES * KS * 64 + lane;
+#pragma unroll
+                for (int q = 0; q < MAXT; ++q) {
+                    if (wv + TC_WAVES * q >= ntiles) continue;                // wave-uniform
+                    const bool live = it_of[q] >= 0;
+#pragma unroll
+                    for (int j = 0; j < SLOTS; ++j) {
+#pragma unroll
+                        for (int s = 0; s < KS; ++s) {
+                            const int i0 = (2 * s) ^ j, i1 = (2 * s + 1) ^ j;
+                            const float sg0 = SLOTS == 4 ? (float)kQuadSG[j][i0] : 1.0f;
+                            const float sg1 = SLOTS == 4 ? (float)kQuadSG[j][i1] : 1.0f;
+                            const float2 gi = (kq >> 1) ? gq[q][i1] : gq[q][i0];
+                            const float sg = (kq >> 1) ? sg1 : sg0;
+                            const float av = live ? ((kq & 1) ? sg * gi.y : gi.x) : 0.f;
+                            const float bv = bu[(res_q[q][j] * KS + s) * 64];
+                            acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[q][j], 0, 0, 0);
+                        }
+                    }
+                }
+            };
+            for (int ub = 0; ub < ubn; ub += 2) {
+                if (ub + 1 < ubn) load_u(gB, ub + 1);
+                products(gA, ub);
+                if (ub + 1 < ubn) {
+                    if (ub + 2 < ubn) load_u(gA, ub + 2);
+                    products(gB, ub + 1);
+                }
+            }
+        }
+        // Epilogue of plane p: outer phase and 'same'-crop shift per item and
+        // slot (the same for every frame); (re, im) of a column pair meet by
+        // one lane swap.
         const int sig = s2 - 2 + N;
         const int c2 = (s2 - 1) / 2;
 #pragma unroll
         for (int q = 0; q < MAXT; ++q) {
-            const int tq = wv + MFMA_WAVES * q;
-            if (tq >= ntiles) continue;                                  // wave-uniform
+            const int tq = wv + TC_WAVES * q;
+            if (tq >= ntiles) continue;                                       // wave-uniform
             const TileDesc td = tiles[tq];
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                const int gc = nt * 16 + ln;
-                const int f = gc / CPF, jj = (gc % CPF) >> 1, cc = gc & 1;
+            for (int j = 0; j < SLOTS; ++j) {
                 float mine[4], other[4];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    mine[r] = acc[q][nt][r];
+                    mine[r] = acc[q][j][r];
                     other[r] = __shfl_xor(mine[r], 1);
                 }
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int ir = 4 * g + r;
-                    if (f < FB && ir < td.count) {
+                    const int ir = 4 * kq + r;
+                    if (ir < td.count) {
                         const ItemDesc& it = a.items[item_base + td.start + ir];
                         const int m2 = it.m2[0], n2 = it.n2[0];
-                        const int outer = jj == 0 ? sig * (m2 + n2)
-                                        : jj == 1 ? sig * (n2 - m2)
-                                        : jj == 2 ? sig * (n2 + m2) : sig * (m2 - n2);
-                        const int pos = it.pos[jj];
+                        const int outer = j == 0 ? sig * (m2 + n2)
+                                        : j == 1 ? sig * (n2 - m2)
+                                        : j == 2 ? sig * (n2 + m2) : sig * (m2 - n2);
+                        const int pos = it.pos[j];
                         const int k0 = pos / a.rfX, k1 = pos - k0 * a.rfX;
                         const float2 po = phase_frac_i(outer, fourN);
                         const int e0 = (c2 * k0) % a.fY;
@@ -892,39 +966,22 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void k_mac_mfma(MacArgs a) {
                         fr -= rintf(fr);
                         float sn, cs;
                         sincospif(2.0f * fr, &sn, &cs);
-                        const float2 z = cc == 0 ? make_float2(mine[r], other[r]) : make_float2(other[r], mine[r]);
+                        const float2 z = c_col == 0 ? make_float2(mine[r], other[r]) : make_float2(other[r], mine[r]);
                         const float2 v = cmul(cmul(z, po), make_float2(cs, sn));
-                        tot[q][nt][r] += cc == 0 ? v.x : v.y;
+                        tot[q][j][r] += c_col == 0 ? v.x : v.y;
                     }
                 }
             }
         }
-    }
-
-    // Store: lane (column f, j, re/im) writes its component of every item of
-    // the tile whose slot j owns the bin.
+        if (a.per_plane) {                        // back projection: one output per plane
+            store(p);
 #pragma unroll
-    for (int q = 0; q < MAXT; ++q) {
-        const int tq = wv + MFMA_WAVES * q;
-        if (tq >= ntiles) continue;
-        const TileDesc td = tiles[tq];
+            for (int q = 0; q < MAXT; ++q)
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-            const int gc = nt * 16 + ln;
-            const int f = gc / CPF, jj = (gc % CPF) >> 1, cc = gc & 1;
-            if (f >= nf) continue;
-            float* A = reinterpret_cast<float*>(a.A + (long long)(t0 + f) * a.A_tstride +
-                                                (long long)blockIdx.y * a.A_pstride);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int ir = 4 * g + r;
-                if (ir < td.count) {
-                    const ItemDesc& it = a.items[item_base + td.start + ir];
-                    if ((it.writemask >> jj) & 1) A[2LL * it.pos[jj] + cc] = tot[q][nt][r];
-                }
-            }
+                for (int j = 0; j < SLOTS; ++j) tot[q][j] = v4f{0.f, 0.f, 0.f, 0.f};
         }
     }
+    if (!a.per_plane) store(blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -1322,6 +1379,11 @@ hipError_t launch_spec_pack(const SpecPackArgs& a, int nchunks, hipStream_t s) {
     hipLaunchKernelGGL(k_spec_pack, dim3(nchunks, a.nu), dim3(FLF_CHUNK_MAX), 0, s, a);
     return hipGetLastError();
 }
+size_t mac_tc_lds(int slots, int N) {
+    const int nres = slots == 4 ? 8 : 4;
+    return (size_t)TC_UB * nres * (slots / 2) * 64 * sizeof(float) + 2 * (size_t)nres * sizeof(float2) +
+           (size_t)TC_FB * N * N * (nres / 2) * sizeof(float2);
+}
 int mac_row_stride(int slots, int nu) {
     const int n = nu * (slots == 4 ? 4 : 2);
     return n + ((1 - n % 16) + 16) % 16;
@@ -1344,15 +1406,12 @@ hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks,
         return hipErrorInvalidValue;
     const size_t lds = mac_lds(slots, a.nu, a.N, fb);
     dim3 grid(nchunks, nplanes_grid, (a.T + fb - 1) / fb);
-    if (mfma && (fb == 2 || fb == 4) && ipt == 1 && a.tiles) {
-        dim3 block(64 * MFMA_WAVES);
-        if (slots == 4) {
-            if (fb == 2) hipLaunchKernelGGL((k_mac_mfma<4, 2>), grid, block, lds, s, a);
-            else hipLaunchKernelGGL((k_mac_mfma<4, 4>), grid, block, lds, s, a);
-        } else {
-            if (fb == 2) hipLaunchKernelGGL((k_mac_mfma<2, 2>), grid, block, lds, s, a);
-            else hipLaunchKernelGGL((k_mac_mfma<2, 4>), grid, block, lds, s, a);
-        }
+    if (mfma && a.tiles) {
+        // matrix-core kernel: TC_FB frames per workgroup
+        dim3 tgrid(nchunks, nplanes_grid, (a.T + TC_FB - 1) / TC_FB);
+        const size_t tlds = mac_tc_lds(slots, a.N);
+        if (slots == 4) hipLaunchKernelGGL((k_mac_tc<4>), tgrid, dim3(64 * TC_WAVES), tlds, s, a);
+        else hipLaunchKernelGGL((k_mac_tc<2>), tgrid, dim3(64 * TC_WAVES), tlds, s, a);
         return hipGetLastError();
     }
     dim3 block(FLF_CHUNK_MAX);                   // ipt 2: lane pairs share two items
@@ -1471,8 +1530,7 @@ hipError_t configure_kernels(size_t max_lds) {
                         (const void*)k_mac<2, 1, 2>, (const void*)k_mac<2, 1, 3>, (const void*)k_mac<2, 1, 4>,
                         (const void*)k_mac<2, 2, 1>, (const void*)k_mac<2, 2, 2>, (const void*)k_mac<2, 2, 3>,
                         (const void*)k_mac<2, 2, 4>,
-                        (const void*)k_mac_mfma<4, 2>, (const void*)k_mac_mfma<4, 4>,
-                        (const void*)k_mac_mfma<2, 2>, (const void*)k_mac_mfma<2, 4>,
+                        (const void*)k_mac_tc<4>, (const void*)k_mac_tc<2>,
                         (const void*)k_small_fft_grouped, (const void*)k_ifft_cols_grouped<false>,
                         (const void*)k_ifft_cols_grouped<true>,
                         (const void*)k_mac_grouped<4, 1, 1>, (const void*)k_mac_grouped<4, 1, 2>,

@@ -6,7 +6,8 @@ CalibrateBlockFactors, projector.py:628-675).
 Plan options come first, each candidate a fresh plan (spectra recomputed):
 the pass order of every FFT (``radix_order``: largest radices first,
 reversed, without radix 8, odd radices first) and the items per MAC thread
-(``mac_ipt``).  Then the launch shapes: one dispatch per stage for all grid
+(``mac_ipt``).  Then the launch shapes: the matrix-core MAC (``mac_mfma``,
+8 frames per workgroup; only when a call holds several frames), one dispatch per stage for all grid
 groups or one per group (``grouped_launch``), the forward MAC's plane ranges
 (``fwd_split``), the frames per MAC workgroup (``mac_frames``, only when a
 call holds several frames), the columns per column-transform workgroup
@@ -22,6 +23,7 @@ PLAN_CANDIDATES = {
 }
 
 CANDIDATES = {
+    "mac_mfma": [0, 1],
     "grouped_launch": [1, 0],
     "fwd_split": [1, 2, 3, 4],
     "mac_frames": [1, 2, 3, 4],
@@ -86,7 +88,7 @@ def tune(projector, hMatrix, frames, Y, X, reps=3, log=None, plan_options=True):
         for name, values in PLAN_CANDIDATES.items():
             search(name, values, True)
     for name, values in CANDIDATES.items():
-        if name == "mac_frames" and frames < 2:
+        if name in ("mac_frames", "mac_mfma") and frames < 2:
             continue
         search(name, values, False)
     del vol, img

@@ -103,7 +103,8 @@ int flf_set_frame_dims(flf_ctx* ctx, const int* dims, int T, void* stream);
  * tuning knobs).  Names and values:
  *   "fwd_split"      0 (automatic) or 1..4 plane ranges of the forward MAC
  *   "mac_frames"     1..4 frames per MAC workgroup (upper bound)
- *   "mac_mfma"       0/1: matrix-core MAC for >= 2 frames per workgroup
+ *   "mac_mfma"       0/1: matrix-core MAC (FP32 MFMA, 8 frames per
+ *                    workgroup) for calls of >= 2 frames
  *   "bwd_planes"     1..8 planes per back-projection MAC workgroup
  *   "ifft_col_batch" 1..8 columns per column-transform workgroup (after plan)
  *   "grouped_launch" 0/1 (default 1): the small FFTs, the multiply-accumulate

@@ -94,33 +94,61 @@ def test_frame_capacity(projector):
             assert rel_l2(bwd[1, t], ref_b) < 1e-5, (T, t)
 
 
-def test_multiframe_matrix_core_kernel():
-    """The opt-in matrix-core MAC (FLF_MAC_MFMA=1, read when a projector is
-    created) against the oracle on 2 and 3 frames, square and non-square."""
-    import os
+@pytest.mark.parametrize("T,Y,X,ranges", [(3, 150, 150, 1), (8, 150, 150, 2), (10, 105, 120, 3), (2, 150, 300, 1),
+                                          (9, 90, 150, 2)])
+def test_multiframe_matrix_core_kernel(T, Y, X, ranges):
+    """The matrix-core MAC (k_mac_tc, 8 frames per workgroup; option
+    mac_mfma, also FLF_MAC_MFMA=1 when a projector is created) against the
+    oracle: partial and full frame blocks (2, 3, 8, 9, 10 frames), square
+    and non-square grids, 1 to 3 forward plane ranges and 2 planes per
+    back-projection workgroup."""
     from flfhip.projector import Projector_hip
-    old = os.environ.get("FLF_MAC_MFMA")
-    os.environ["FLF_MAC_MFMA"] = "1"
-    try:
-        proj = Projector_hip()
-    finally:
-        if old is None:
-            del os.environ["FLF_MAC_MFMA"]
-        else:
-            os.environ["FLF_MAC_MFMA"] = old
-    for T, Y, X in [(3, 150, 150), (2, 150, 300)]:
-        hm = _hm_random(15, [31, 45], seed=T + Y + 1)
-        rng = np.random.default_rng(7)
-        vol = rng.random((2, T, Y, X)).astype(np.float32)
-        img = rng.random((T, Y, X)).astype(np.float32)
-        print("matrix-core multiframe", T, Y, X, flush=True)
-        fwd = proj.ForwardProjectACC(hm, vol, [0, 1])
-        bwd = proj.BackwardProjectACC(hm, img, [0, 1])
+    proj = Projector_hip()
+    proj.set_option("mac_mfma", 1)
+    proj.set_option("fwd_split", ranges)
+    proj.set_option("bwd_planes", 2)
+    hm = _hm_random(15, [31, 45, 61], seed=T + Y + 1)
+    rng = np.random.default_rng(7)
+    vol = rng.random((3, T, Y, X)).astype(np.float32)
+    img = rng.random((T, Y, X)).astype(np.float32)
+    print("matrix-core multiframe", T, Y, X, ranges, flush=True)
+    fwd = proj.ForwardProjectACC(hm, vol, [0, 1, 2])
+    bwd = proj.BackwardProjectACC(hm, img, [0, 1, 2])
+    for t in range(T):
+        ref_f = sum(oproj.project_special(vol[z, t][None], hm.Hcc(z, False))[0] for z in range(3))
+        assert rel_l2(fwd[t], ref_f) < 1e-5, t
+        for z in range(3):
+            ref_b = oproj.project_special(img[t][None], hm.Hcc(z, True))[0]
+            assert rel_l2(bwd[z, t], ref_b) < 1e-5, (t, z)
+
+
+def test_matrix_core_full_size_matches_vector_kernel():
+    """At the 1455^2, 41-plane geometry with 8 and 10 frames, the
+    matrix-core MAC's forward and backward projections equal the vector
+    kernel's (FP32, other summation order) to 1e-6 relative L2 per frame."""
+    import torch
+    from flfhip import workloads
+    from flfhip.projector import Projector_hip
+    hm, zs, sides = workloads.psf_set("rl1455")
+    Y = X = 1455
+    pv, pm = Projector_hip(), Projector_hip()
+    pm.set_option("mac_mfma", 1)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    for T in (8, 10):
+        vol = torch.rand((41, T, Y, X), generator=g, device="cuda")
+        img = torch.rand((T, Y, X), generator=g, device="cuda")
+        for p in (pv, pm):
+            p.prepare(hm, T, Y, X)
+        fv, fm = pv.forward_device(hm, vol), pm.forward_device(hm, vol)
+        bv, bm = pv.backward_device(hm, img), pm.backward_device(hm, img)
         for t in range(T):
-            ref_f = sum(oproj.project_special(vol[z, t][None], hm.Hcc(z, False))[0] for z in range(2))
-            assert rel_l2(fwd[t], ref_f) < 1e-5, t
-            ref_b = oproj.project_special(img[t][None], hm.Hcc(1, True))[0]
-            assert rel_l2(bwd[1, t], ref_b) < 1e-5, t
+            ef = float(torch.linalg.vector_norm((fm[t] - fv[t]).double()) / torch.linalg.vector_norm(fv[t].double()))
+            eb = float(torch.linalg.vector_norm((bm[:, t] - bv[:, t]).double()) /
+                       torch.linalg.vector_norm(bv[:, t].double()))
+            assert ef < 1e-6 and eb < 1e-6, (T, t, ef, eb)
+        print("T %d: matrix-core vs vector kernel within 1e-6 on every frame" % T, flush=True)
+        del vol, img, fv, fm, bv, bm
 
 
 def test_autotune_and_options_keep_parity(projector):
