@@ -136,3 +136,38 @@ def test_set_frame_dims_rejects(projector, case):
     arr = (ctypes.c_int * 2)(60, 60)
     assert lib.flf_set_frame_dims(ctx, arr, 1, None) == _lib.FLF_EINVAL     # 1 size for 2 active frames
     assert lib.flf_set_frame_dims(ctx, None, 2, None) == 0
+
+
+def test_ragged_series_frames_full_size_vs_oracle(projector):
+    """The uneven time series at its real frame sizes: the first four frames
+    of the 64-frame series (lenslet grids drawn from 20..60, i.e. 300..900
+    px, by the series' own seeds), the series' ring PSFs of planes z = -15
+    (255 px) and z = 0, two canvas buckets (the 570 px frame alone, the
+    other three on a 795 px canvas); Htf and one Prevedel iteration per
+    frame against the FP64 oracle at the frame's own size (1e-4)."""
+    from flfhip import workloads
+    from flfhip.psfmatrix import MemoryHMatrix
+    from flfhip.ragged import RaggedBatch
+    w = workloads.setup_ragged("ragged64", volume=False)
+    shapes = w["shapes"][:4]
+    full = w["hm"]
+    planes = [0, w["zs"].index(0)]
+    hm = MemoryHMatrix([full.Hcc(z, False) for z in planes], [full.Hcc(z, True) for z in planes])
+    Qf = [hm.Hcc(z, False) for z in range(2)]
+    Qb = [hm.Hcc(z, True) for z in range(2)]
+    fwd = lambda v: oproj.forward_project(v, Qf)
+    bwd = lambda i: oproj.backward_project(i, Qb)
+    ims = []
+    for t, (y, x) in enumerate(shapes):
+        v = synthetic.blob_volume(2, y, x, 120, seed=300 + t)
+        ims.append(synthetic.poisson_image(fwd(v.astype(np.float64)), 1000.0, seed=t)[0])
+    rb = RaggedBatch(hm, shapes, nbuckets=2, spectrum_weight=0.5)
+    print("series frames", shapes, "buckets", [(b[0], len(b[1])) for b in rb.buckets], flush=True)
+    assert len(rb.buckets) == 2
+    got = rb.deconv(ims, 1)
+    for t, (y, x) in enumerate(shapes):
+        ref = orl.deconv_rl(fwd, bwd, None, 1, im=ims[t][None].astype(np.float64))[:, 0]
+        e = rel_l2(got[t], ref)
+        print("frame %d %dx%d: RL (Htf + 1 iteration) rel-L2 %.2e" % (t, y, x, e), flush=True)
+        assert got[t].shape == (2, y, x)
+        assert e < 1e-4
