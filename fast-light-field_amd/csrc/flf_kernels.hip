@@ -394,12 +394,57 @@ __device__ __forceinline__ void mac_spectrum(float2 (&acc)[FB][IPT][4], const fl
 #ifndef FLF_MAC_UNR2
 #define FLF_MAC_UNR2 1
 #endif
+#ifndef FLF_MAC_UNR2_FB1
+#define FLF_MAC_UNR2_FB1 2
+#endif
 #ifndef FLF_MAC_WAVES
 #define FLF_MAC_WAVES 4
 #endif
 
+// Wave-uniform read of a per-plane value through the scalar cache (the array
+// is read-only while the kernel runs).  Scalar loads keep such reads off the
+// vector-memory counter, so they never make the accumulation wait for the
+// spectrum loads in flight.
+// The value is moved to scalar registers at once (readfirstlane), so a load
+// that had to go through a vector register is consumed on every path.
+__device__ __forceinline__ int const_load(const int* base, int i) {
+    return __builtin_amdgcn_readfirstlane(((const __attribute__((address_space(4))) int*)base)[i]);
+}
+__device__ __forceinline__ long long const_load(const long long* base, int i) {
+    const long long v = ((const __attribute__((address_space(4))) long long*)base)[i];
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v & 0xffffffffLL));
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)((unsigned long long)v >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+// s_waitcnt vmcnt(0) issued through the builtin, so that the compiler's own
+// wait counting knows every vector-memory access has completed.
+__device__ __forceinline__ void wait_vmem_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Copy of the chunk's small-FFT blocks of plane p, frames t0 .. t0+nf-1, into
+// s_stage ([f][NN][NH]) by global-to-LDS loads: every load of the block is in
+// flight at once and no registers are held; completion is awaited (vmcnt) by
+// mac_tables.  (Started earlier, during the previous plane's accumulation,
+// the copy would make the compiler drain all vector loads before every LDS
+// read of the accumulation.)  The block size is a multiple of 64 threads.
+template <int SLOTS>
+__device__ __forceinline__ void mac_stage_async(const MacArgs& a, int cidx, int p, int t0, int nf, float2* s_stage) {
+    constexpr int NH = SLOTS == 4 ? 4 : 2;
+    const int blk = a.N * a.N * NH;              // float2 per frame block
+    const int nd = 2 * blk;                       // dwords per frame block
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const long long sbase = const_load(a.S_off, p) + (long long)t0 * a.S_tstride + (long long)cidx * blk;
+    for (int f = 0; f < nf; ++f) {
+        const float* src = reinterpret_cast<const float*>(a.S + sbase + f * a.S_tstride);
+        float* dst = reinterpret_cast<float*>(s_stage + f * blk);
+        for (int c = wv * 64; c < nd; c += nw * 64)
+            if (c + lane < nd)
+                __builtin_amdgcn_global_load_lds(src + c + lane, (__attribute__((address_space(3))) void*)(dst + c), 4,
+                                                 0, 0);
+    }
+}
+
 // Per-plane set-up of a MAC workgroup: flip factors, then per frame f < nf
-// the chunk's small-FFT block staged in LDS and its P/M table
+// the P/M table of the chunk's small-FFT block (staged in LDS beforehand)
 //   pm[f][rho * rs + u * NPT + pt] = (P, M),  P = s_plain + s_conj,
 //   M = i (s_plain - s_conj)
 // for residue rho, stored spectrum u and pair type pt (all threads; contains
@@ -416,9 +461,9 @@ __device__ __forceinline__ void mac_tables(const MacArgs& a, const ChunkDesc* ch
     const int N = a.N, NN = N * N, c = N / 2;
     const int nu = a.nu;
     const int tab = NRES * rs;
-    const int s2 = a.s2[p];
-    __syncthreads();
-    // flip factors per residue: exp(-2 pi i (s2-1) kt / f)
+    const int s2 = const_load(a.s2, p);
+    // flip factors per residue: exp(-2 pi i (s2-1) kt / f) (psi was last read
+    // before the closing barrier of the previous plane's set-up)
     if (tid < 2 * NRES) {
         const int rho = tid >> 1;
         const bool isY = (tid & 1) == 0;
@@ -432,41 +477,55 @@ __device__ __forceinline__ void mac_tables(const MacArgs& a, const ChunkDesc* ch
         sincospif(-2.0f * (float)m / (float)f, &sn, &cs);
         psi[tid] = make_float2(cs, sn);
     }
-    // Stage the small-FFT blocks of all nf frames ([f][NN][NH]), then build
-    // every frame's table: one load round trip and two barriers per plane.
+    // Copy the small-FFT blocks of all nf frames into s_stage, wait for the
+    // copy, then build every frame's table (the barrier also orders these
+    // writes of pm_tab after every wave's reads of the previous plane's
+    // table).
     const int blk = NN * NH;
-    for (int i = tid; i < nf * blk; i += blockDim.x) {
-        const int f = i / blk, e = i - f * blk;
-        s_stage[i] = a.S[a.S_off[p] + (long long)(t0 + f) * a.S_tstride + (long long)cidx * blk + e];
-    }
+    mac_stage_async<SLOTS>(a, cidx, p, t0, nf, s_stage);
+    wait_vmem_all();
     __syncthreads();
     {
-        const int per = NRES * nu * NPT;
-        for (int fi = tid; fi < nf * per; fi += blockDim.x) {
-            const int f = fi / per, idx = fi - f * per;
-            float4* pmf = pm_tab + f * tab;
-            const float2* sf = s_stage + f * blk;
-            const int pt = idx % NPT;
-            const int ru = idx / NPT;
-            const int u = ru % nu, rho = ru / nu;
-            const int rb0 = rho < NH ? rho : rho - NH;
-            float2 sv[2];
+        // Thread -> table column q = (u, pt) and a share of the rows
+        // (f, rho): the column's two source offsets and flip flags are
+        // decoded once, the rows then cost two LDS reads and the P/M
+        // arithmetic each (no integer division per entry).
+        const int Q = nu * NPT;
+        const int nt = blockDim.x;
+        const bool fits = Q <= nt;
+        const int rstep = fits ? nt / Q : 1;
+        const int r0 = fits ? tid / Q : 0;
+        const int rows = nf * NRES;
+        for (int q = (fits ? (r0 < rstep ? tid - r0 * Q : Q) : tid); q < Q; q += nt) {
+            const int pt = q % NPT, u = q / NPT;
+            int src[2], flip[2];
 #pragma unroll
             for (int w = 0; w < 2; ++w) {
                 const int off = a.uoff[u * NUSE + (w == 0 ? kPairPlain[pt] : kPairConj[pt])];
-                float2 r = make_float2(0.f, 0.f);
-                if (off < NN) {
-                    const int bo = off / N, ao = off - bo * N;
-                    r = sf[off * NH + rb0];
-                    if (rho >= NH) r = cconj(r);
-                    if (bo > c) r = cmul(r, psi[2 * rho]);
-                    if (ao > c) r = cmul(r, psi[2 * rho + 1]);
-                }
-                sv[w] = r;
+                const int bo = off / N, ao = off - bo * N;
+                src[w] = off < NN ? off * NH : -1;
+                flip[w] = (bo > c ? 1 : 0) | (ao > c ? 2 : 0);
             }
-            const float2 P = cadd(sv[0], sv[1]);
-            const float2 d = csub(sv[0], sv[1]);
-            pmf[rho * rs + u * NPT + pt] = make_float4(P.x, P.y, -d.y, d.x);   // M = i * d
+            for (int row = r0; row < rows; row += rstep) {
+                const int f = row / NRES, rho = row % NRES;
+                const float2* sf = s_stage + f * blk;
+                const int rb0 = rho < NH ? rho : rho - NH;
+                float2 sv[2];
+#pragma unroll
+                for (int w = 0; w < 2; ++w) {
+                    float2 r = make_float2(0.f, 0.f);
+                    if (src[w] >= 0) {
+                        r = sf[src[w] + rb0];
+                        if (rho >= NH) r = cconj(r);
+                        if (flip[w] & 1) r = cmul(r, psi[2 * rho]);
+                        if (flip[w] & 2) r = cmul(r, psi[2 * rho + 1]);
+                    }
+                    sv[w] = r;
+                }
+                const float2 P = cadd(sv[0], sv[1]);
+                const float2 d = csub(sv[0], sv[1]);
+                pm_tab[f * tab + rho * rs + u * NPT + pt] = make_float4(P.x, P.y, -d.y, d.x);   // M = i * d
+            }
         }
     }
     __syncthreads();
@@ -495,7 +554,8 @@ __device__ __forceinline__ void mac_body(const MacArgs& a, int cidx, int range, 
     extern __shared__ float4 pm_tab[];            // [FB][NRES][rs: nu*NPT + pad], then staging
     constexpr int NRES = SLOTS == 4 ? 8 : 4;      // residues per orbit (canonical list)
     constexpr int NH = NRES / 2;                  // residues stored by the small-FFT kernel
-    constexpr int UNR = IPT >= 2 ? FLF_MAC_UNR2 : FLF_MAC_UNR;   // spectra whose loads are issued together
+    // spectra whose loads are issued together
+    constexpr int UNR = IPT >= 2 ? (FB == 1 ? FLF_MAC_UNR2_FB1 : FLF_MAC_UNR2) : FLF_MAC_UNR;
     const ChunkDesc* ch = a.chunks + cidx;
     const long long item_base = ch->item_base;
     const int n_items = ch->n_items;
@@ -530,12 +590,16 @@ __device__ __forceinline__ void mac_body(const MacArgs& a, int cidx, int range, 
 #pragma unroll
         for (int j = 0; j < 4; ++j) rb[j] = i0.res[j] * rs;
     }
-    // An opaque copy of the descriptor pointer, so that the compiler does not
-    // hoist the epilogue's loads out of the plane loop.
+    // The descriptor pointer plus an opaque zero, so that the compiler does
+    // not hoist the epilogue's loads out of the plane loop.  The pointer
+    // itself stays transparent: the compiler knows it addresses global
+    // memory and emits global loads (FLAT loads would make every later
+    // vector-memory wait drain all loads in flight, the next spectrum batch
+    // included).
     auto items_now = [&]() {
-        const ItemDesc* ip = itp;
-        asm volatile("" : "+v"(ip));
-        return ip;
+        int z = 0;
+        asm volatile("" : "+v"(z));
+        return itp + z;
     };
 
     constexpr int QT = IPT == 2 ? 1 : IPT;        // items whose totals a lane keeps
@@ -578,7 +642,7 @@ __device__ __forceinline__ void mac_body(const MacArgs& a, int cidx, int range, 
     // Store the totals at A + slice * A_pstride (each lane of a pair stores
     // its own item), frames of this block only.
     auto store = [&](int slice) {
-        const ItemDesc* ip = items_now();
+        const auto ip = items_now();
 #pragma unroll
         for (int f = 0; f < FB; ++f) {
             if (f >= nf) break;
@@ -596,7 +660,7 @@ __device__ __forceinline__ void mac_body(const MacArgs& a, int cidx, int range, 
     float2 acc[FB][IPT][4];
     // Epilogue of plane p: outer phase and the 'same'-crop shift.
     auto epilogue = [&](int p) {
-        const int s2 = a.s2[p];
+        const int s2 = const_load(a.s2, p);
         const int sig = s2 - 2 + N;
         const int c2 = (s2 - 1) / 2;
         // IPT = 2: each lane keeps its own item (q = lsub): it sends the
@@ -612,7 +676,7 @@ __device__ __forceinline__ void mac_body(const MacArgs& a, int cidx, int range, 
                     own[f][j] = make_float2(mine.x + __shfl_xor(give.x, 1), mine.y + __shfl_xor(give.y, 1));
                 }
         }
-        const ItemDesc* ip = items_now();
+        const auto ip = items_now();
 #pragma unroll
         for (int q = 0; q < QT; ++q) {
             const ItemDesc itq = ip[IPT == 2 ? lsub : q];
@@ -663,6 +727,10 @@ __device__ __forceinline__ void mac_body(const MacArgs& a, int cidx, int range, 
         load_batch(gA, p, 0);
         int b = 0;
         auto step = [&](float2 (&cur)[UNR][IPT][4], float2 (&nxt)[UNR][IPT][4]) -> bool {
+            // The current batch (and anything older) has landed: the only
+            // loads in flight during the accumulation below are the next
+            // batch's and the staging copy (which writes no registers).
+            wait_vmem_all();
             if (b == 0) {
                 prologue(p);
 #pragma unroll
