@@ -21,8 +21,10 @@ Also reported:
   roofline      the multiply-accumulate kernel (dominant): algorithmic bytes
                 (compulsory cached-spectrum reads, 8 B per stored complex bin)
                 / measured kernel time (HIP events on its stream) vs 8 TB/s.
-  rl_iteration  the same ratio for a whole RL iteration: S_H + S_Ht +
-                B*(4*nz+2)*Y*X*4 bytes / iteration time.
+  rl_iteration  the same ratio for a whole RL iteration of one frame:
+                S_H + S_Ht + (4*nz+2)*Y*X*4 bytes / iteration time
+                (rl_iteration_batch: the same for all B frames of a step,
+                where the MAC is FP32-bound instead).
   cpu_baseline  the FP64 numpy oracle (single thread) timed on one plane's
                 projection at full size and scaled to a volume.
 
@@ -461,6 +463,9 @@ def batch_main(args):
     hbm_rate = (mac_bytes / max(mac_n, 1)) / (mac_avg_ms / 1e3)
     it_bytes = 2 * s_dir + T * (4 * nz + 2) * Y * X * 4
     it_s = (plain_ms / 1e3) / (iters + 0.5)
+    # one frame (Htf + iters iterations of frame 0): the HBM-bound case
+    it1_bytes = 2 * s_dir + (4 * nz + 2) * Y * X * 4
+    it1_s = (plain1_ms / 1e3) / (iters + 0.5)
     one = lambda a, f=float: [f(x) for x in a]
     desc = ("%s: %d frames of Nnum=%d, %dx%d px, %d planes (z=-%d..%d), PSF %d..%d px, %d..%d seeded blobs per "
             "frame (seed %d), Poisson peak %g; target = each frame's data fit after %d plain RL iterations from a "
@@ -506,8 +511,12 @@ def batch_main(args):
                      "note": "several frames per workgroup: the MAC is bound by FP32 FMA issue, not HBM",
                      "hbm": {"achieved": hbm_rate / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                              "frac": hbm_rate / PEAK_HBM, "bytes_per_launch": mac_bytes / max(mac_n, 1)}},
-        "rl_iteration": {"ms": it_s * 1e3, "frames": T, "algorithmic_bytes": it_bytes,
-                         "achieved_GBps": it_bytes / it_s / 1e9, "frac": it_bytes / it_s / PEAK_HBM},
+        "rl_iteration": {"ms": it1_s * 1e3, "frames": 1, "algorithmic_bytes": it1_bytes,
+                         "achieved_GBps": it1_bytes / it1_s / 1e9, "frac": it1_bytes / it1_s / PEAK_HBM,
+                         "note": "one frame: every cached spectrum read once per projection, HBM-bound"},
+        "rl_iteration_batch": {"ms": it_s * 1e3, "frames": T, "algorithmic_bytes": it_bytes,
+                               "achieved_GBps": it_bytes / it_s / 1e9, "frac": it_bytes / it_s / PEAK_HBM,
+                               "note": "all frames of a step: FP32-bound (see roofline), not HBM-bound"},
         "spectra_precompute_s": t_prep,
     }
     del X1s

@@ -177,6 +177,18 @@ struct IfftRowPlaneArgs {
 namespace flf {
 size_t small_fft_lds(int rY, int rX);
 size_t ifft_cols_lds(int CB, int fY);
+// LDS stride of one column of the column transform: fY padded so that the CB
+// columns of one row, written and read by consecutive threads, fall on
+// distinct banks (FLF_COLS_BANKS banks of 4 bytes; 0: unpadded).
+#ifndef FLF_COLS_BANKS
+#define FLF_COLS_BANKS 64
+#endif
+__host__ __device__ inline int ifft_col_stride(int fY, int CB) {
+    if (FLF_COLS_BANKS == 0 || CB < 2) return fY;
+    const int m = FLF_COLS_BANKS / 2;      // float2 entries per sweep of the banks
+    const int r = (m / CB) % m;
+    return fY + ((r - fY) % m + m) % m;
+}
 size_t ifft_rows_sum_lds(int maxfX);
 size_t ifft_rows_plane_lds(int wbuf, bool big);
 bool wave_big(int L);

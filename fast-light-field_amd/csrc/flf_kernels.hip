@@ -122,19 +122,29 @@ __device__ __forceinline__ void small_fft_body(const SmallFFTArgs& a, const floa
         const int y0 = 4 * q;
         const int np = (y0 + 2 < Ys) ? 2 : 1;      // row pairs in this group of four rows
         float2* P = H + (long long)y0 * rXh;      // pair s at P + 2 s rXh
-        for (int sq = 0; sq < np; ++sq) {
+        // All loads of the group's four rows are issued before any is used.
+        constexpr int XC = RMAX / 64;              // 64-lane column chunks per row
+        float v[2][XC][2];
+#pragma unroll
+        for (int sq = 0; sq < 2; ++sq) {
             const int ya = y0 + 2 * sq, yb = ya + 1;
             const float* r0 = src + (long long)(b + N * ya) * a.src_pitch + aa;
             const float* r1 = src + (long long)(b + N * yb) * a.src_pitch + aa;
-            for (int x = lane; x < rX; x += 64) {
-                float v0 = 0.f, v1 = 0.f;
-                if (x < Xs) {
-                    v0 = r0[N * x];
-                    if (yb < Ys) v1 = r1[N * x];
-                }
-                P[2 * sq * rXh + x] = make_float2(v0, v1);
+#pragma unroll
+            for (int c = 0; c < XC; ++c) {
+                const int x = lane + 64 * c;
+                const bool in = sq < np && x < Xs;
+                v[sq][c][0] = in ? r0[N * x] : 0.f;
+                v[sq][c][1] = (in && yb < Ys) ? r1[N * x] : 0.f;
             }
         }
+#pragma unroll
+        for (int sq = 0; sq < 2; ++sq)
+#pragma unroll
+            for (int c = 0; c < XC; ++c) {
+                const int x = lane + 64 * c;
+                if (sq < np && x < rX) P[2 * sq * rXh + x] = make_float2(v[sq][c][0], v[sq][c][1]);
+            }
         wave_sync();
         wave_fft_ns<RMAX, 2>(P, 1, 2 * rXh, np, a.rpX, twX, false, lane);
         // Separate: row ya = (Z + Zc)/2, row yb = (Z - Zc)/(2i), Zc = conj Z(-k);
@@ -189,13 +199,32 @@ __device__ __forceinline__ void small_fft_body(const SmallFFTArgs& a, const floa
     // residues of each chunk's residue list (the MAC reads one contiguous
     // block per chunk).
     float2* S = a.S + a.S_off[z] + (long long)t * a.S_tstride;
+    // OU entries per thread at a time: their residue loads (chunk
+    // descriptors, from the caches) are all in flight before the first is
+    // used.
     const int NH = a.NH, NN = N * N;
-    for (int i = threadIdx.x; i < a.nchunks * NH; i += blockDim.x) {
-        const int w = i / NH, j = i - w * NH;
-        const int k0 = a.chunks[w].res0[j], k1 = a.chunks[w].res1[j];
-        float2 v = (2 * k1 <= rX) ? H[k0 * rXh + k1] : cconj(H[((rY - k0) % rY) * rXh + (rX - k1)]);
-        v = cmul(v, cmul(phY[k0], phX[k1]));
-        S[((long long)w * NN + off) * NH + j] = v;
+    const int total = a.nchunks * NH;
+    constexpr int OU = 4;
+    for (int i0 = threadIdx.x; i0 < total; i0 += OU * blockDim.x) {
+        int k0[OU], k1[OU];
+#pragma unroll
+        for (int u = 0; u < OU; ++u) {
+            const int i = min(i0 + u * (int)blockDim.x, total - 1);
+            const int w = i / NH, j = i - w * NH;
+            k0[u] = a.chunks[w].res0[j];
+            k1[u] = a.chunks[w].res1[j];
+        }
+#pragma unroll
+        for (int u = 0; u < OU; ++u) {
+            const int i = i0 + u * (int)blockDim.x;
+            if (i < total) {
+                const int w = i / NH, j = i - w * NH;
+                float2 v = (2 * k1[u] <= rX) ? H[k0[u] * rXh + k1[u]]
+                                             : cconj(H[((rY - k0[u]) % rY) * rXh + (rX - k1[u])]);
+                v = cmul(v, cmul(phY[k0[u]], phX[k1[u]]));
+                S[((long long)w * NN + off) * NH + j] = v;
+            }
+        }
     }
 }
 
@@ -1063,9 +1092,10 @@ template <bool PRIME, bool BIG>
 __device__ __forceinline__ void ifft_cols_body(const IfftColArgs& a, int colblk, int batch) {
     extern __shared__ float2 lds[];
     const int fY = a.rpY.L, rfX = a.rfX, CB = a.CB;
-    float2* C = lds;                 // [CB][fY]
-    float2* C2 = C + CB * fY;        // BIG: [CB][fY], the second buffer of the out-of-place passes
-    float2* tw = BIG ? C2 + CB * fY : C2;   // [fY]
+    const int cs = ifft_col_stride(fY, CB);
+    float2* C = lds;                 // [CB][cs]
+    float2* C2 = C + CB * cs;        // BIG: [CB][cs], the second buffer of the out-of-place passes
+    float2* tw = BIG ? C2 + CB * cs : C2;   // [fY]
     const int c0 = colblk * CB;
     const int ncol = min(CB, rfX - c0);
     const float2* in = a.A + (long long)batch * a.A_bstride;
@@ -1092,22 +1122,22 @@ __device__ __forceinline__ void ifft_cols_body(const IfftColArgs& a, int colblk,
         for (int u = 0; u < B; ++u) {
             const int i = i0 + u * blockDim.x;
             const int y = i / CB, cc = i - y * CB;
-            if (i < fY * CB) C[cc * fY + y] = v[u];
+            if (i < fY * CB) C[cc * cs + y] = v[u];
         }
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if constexpr (BIG) {
-        if (wv < ncol) wave_fft_pp<PRIME>(C + wv * fY, C2 + wv * fY, a.rpY, tw, true, lane);
+        if (wv < ncol) wave_fft_pp<PRIME>(C + wv * cs, C2 + wv * cs, a.rpY, tw, true, lane);
     } else {
-        if (wv < ncol) wave_fft<2048, PRIME>(C + wv * fY, 1, a.rpY, tw, true, lane);
+        if (wv < ncol) wave_fft<2048, PRIME>(C + wv * cs, 1, a.rpY, tw, true, lane);
     }
     __syncthreads();
     const float2* Res = (BIG && (a.rpY.n & 1)) ? C2 : C;   // after an odd pass count: the second buffer
     float2* out = a.out + (long long)batch * a.out_bstride;
     for (int i = threadIdx.x; i < a.Yout * CB; i += blockDim.x) {
         const int y = i / CB, cc = i - y * CB;
-        if (cc < ncol) out[(long long)y * rfX + c0 + cc] = Res[cc * fY + y];
+        if (cc < ncol) out[(long long)y * rfX + c0 + cc] = Res[cc * cs + y];
     }
 }
 template <bool PRIME, bool BIG>
@@ -1518,7 +1548,7 @@ hipError_t launch_mac_grouped(const MacGrouped& ga, int slots, int ipt, int fb, 
 // Transforms longer than 2048 points run out of place (two LDS buffers).
 bool wave_big(int L) { return L > 2048; }
 size_t ifft_cols_lds(int CB, int fY) {
-    return ((size_t)CB * fY * (wave_big(fY) ? 2 : 1) + fY) * sizeof(float2);
+    return ((size_t)CB * ifft_col_stride(fY, CB) * (wave_big(fY) ? 2 : 1) + fY) * sizeof(float2);
 }
 hipError_t launch_ifft_cols_grouped(const IfftColGrouped& ga, int CB, bool prime, size_t lds, hipStream_t s) {
     const GroupedGrid& gg = ga.gg;
