@@ -15,8 +15,10 @@
 // The volume guess is y upsampled (nearest) to full resolution.
 //
 // Arithmetic: one BF16 MFMA GEMM kernel (v_mfma_f32_32x32x16_bf16, FP32
-// accumulation) C = A * B^T with both operands K-contiguous, fed by im2col /
-// transposition kernels:
+// accumulation) C = A * B^T with both operands K-contiguous.  Forward and
+// dgrad read the activation itself as an implicit im2col (each K step of 16
+// channels lies in one tap: rows shifted by the tap, zero outside the frame);
+// wgrad is fed by a transposed im2col and a transposition:
 //   forward  out[p, co]   = sum_k im2col(x)[p, k] * W[co, k]          k = (tap, ci)
 //   dgrad    dx[p, ci]    = sum_k im2col_flip(dy)[p, k] * Wt[ci, k]   k = (tap, co)
 //   wgrad    dW[co, k]    = sum_p dy^T[co, p] * im2col^T(x)[k, p]     (split over p)
@@ -50,6 +52,12 @@ __device__ __forceinline__ float from_bf16(bf16 x) { return (float)x; }
 // K must be a multiple of 16.  grid (ceil(N/128), ceil(M/64), splitK).
 // ---------------------------------------------------------------------------
 struct GemmArgs {
+    // conv != 0: A is not a matrix but an NHWC activation (convN frames of
+    // convH x convW positions, row stride lda, convC channels, convC a
+    // multiple of 16, channels past the real ones zero) read as its implicit
+    // 3x3 pad-1 im2col: A[p][tap * convC + c] = x[shift(p, tap)][c], zero
+    // outside the frame; conv == 2 mirrors the taps (dgrad).
+    int conv, convH, convW, convC;
     const bf16* A; long long lda;
     const bf16* B; long long ldb;
     int M, N, K;
@@ -71,6 +79,7 @@ __device__ __forceinline__ bf16x8 load8(const bf16* p, bool ok) {
     return z;
 }
 
+template <int CONV>
 __global__ __launch_bounds__(256) void k_gemm_nt(GemmArgs g) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int m0 = blockIdx.y * 64 + (wv >> 1) * 32;
@@ -82,6 +91,21 @@ __global__ __launch_bounds__(256) void k_gemm_nt(GemmArgs g) {
     const int bn0 = n0 + li, bn1 = n0 + 32 + li;
     const bool aok = am < g.M, b0ok = bn0 < g.N, b1ok = bn1 < g.N;
     const bf16* pa = g.A + (long long)(aok ? am : 0) * g.lda + lk;
+    // implicit im2col: the lane's position in its frame
+    const int ax = CONV ? am % g.convW : 0;
+    const int ay = CONV ? (am / g.convW) % g.convH : 0;
+    auto load_a = [&](int k, bool in) -> bf16x8 {
+        if constexpr (CONV == 0) {
+            return load8(pa + k, aok && in);
+        } else {
+            const int tap = k / g.convC, c = k - tap * g.convC;
+            const int tt = CONV == 2 ? 8 - tap : tap;
+            const int dy = tt / 3 - 1, dx = tt - 3 * (tt / 3) - 1;
+            const bool ok = in && aok && (unsigned)(ay + dy) < (unsigned)g.convH &&
+                            (unsigned)(ax + dx) < (unsigned)g.convW;
+            return load8(ok ? g.A + ((long long)am + dy * g.convW + dx) * g.lda + c + lk : g.A, ok);
+        }
+    };
     const bf16* pb0 = g.B + (long long)(b0ok ? bn0 : 0) * g.ldb + lk;
     const bf16* pb1 = g.B + (long long)(b1ok ? bn1 : 0) * g.ldb + lk;
     f32x16 c0 = {}, c1 = {};
@@ -95,7 +119,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(GemmArgs g) {
     for (int d = 0; d < PF; ++d) {
         const bool in = d < nsteps;
         const int k = kb + 16 * d;
-        ra[d] = load8(pa + (in ? k : kb), aok && in);
+        ra[d] = load_a(in ? k : kb, in);
         rb0[d] = load8(pb0 + (in ? k : kb), b0ok && in);
         rb1[d] = load8(pb1 + (in ? k : kb), b1ok && in);
     }
@@ -108,7 +132,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(GemmArgs g) {
                 c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ra[d], rb1[d], c1, 0, 0, 0);
                 if (st + PF < nsteps) {
                     const int k = kb + 16 * (st + PF);
-                    ra[d] = load8(pa + k, aok);
+                    ra[d] = load_a(k, true);
                     rb0[d] = load8(pb0 + k, b0ok);
                     rb1[d] = load8(pb1 + k, b1ok);
                 }
@@ -135,55 +159,52 @@ __global__ __launch_bounds__(256) void k_gemm_nt(GemmArgs g) {
     }
 }
 
-// im2col of an NHWC bf16 tensor for a 3x3, pad-1 convolution:
-// out[p][tap*Cp + c] = x[shift(p, tap)][c]  (zero outside the image or c >= C).
-// flip selects the mirrored tap (dgrad).  Cp multiple of 8.
-__global__ void k_im2col(const bf16* x, int n, int H, int W, int C, int ldx, int Cp, int flip, bf16* out) {
-    const long long P = (long long)n * H * W;
-    const long long total = P * 9 * (Cp / 8);
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (long long)gridDim.x * blockDim.x) {
-        const int c8 = (int)(idx % (Cp / 8));
-        const long long r = idx / (Cp / 8);
-        const int tap = (int)(r % 9);
-        const long long p = r / 9;
-        const int xx = (int)(p % W), yy = (int)((p / W) % H);
-        const long long nn = p / ((long long)W * H);
-        const int tt = flip ? 8 - tap : tap;
-        const int sy = yy + tt / 3 - 1, sx = xx + tt % 3 - 1;
-        bf16x8 v;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = (bf16)0.0f;
-        if (sy >= 0 && sy < H && sx >= 0 && sx < W) {
-            const bf16* src = x + ((nn * H + sy) * W + sx) * ldx;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int cc = c8 * 8 + e;
-                if (cc < C) v[e] = src[cc];
-            }
-        }
-        *reinterpret_cast<bf16x8*>(out + (p * 9 + tap) * Cp + c8 * 8) = v;
-    }
-}
-
 // Transposed im2col for wgrad: out[tap*Cp + c][p] (row length ldo >= P, zero
-// padded to a multiple of 16), from NHWC x.
-__global__ void k_im2col_t(const bf16* x, int n, int H, int W, int C, int ldx, int Cp, long long ldo, bf16* out) {
+// padded to a multiple of 16), from NHWC x (ldx a multiple of 8; channels
+// c >= C read as zero).  grid (ceil(ldo / 64), ceil(Cp / 64)), block 256: a
+// tile of 64 positions x 64 channels goes through LDS once per tap, read as
+// 16-byte channel runs of x and written as 16-byte position runs of out.
+__global__ __launch_bounds__(256) void k_im2col_t(const bf16* x, int n, int H, int W, int C, int ldx, int Cp, long long ldo,
+                                                  bf16* out) {
+    __shared__ __align__(16) bf16 tile[64][72];   // [channel][position], rows 144 B apart
     const long long P = (long long)n * H * W;
-    const long long total = (long long)9 * Cp * ldo;
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (long long)gridDim.x * blockDim.x) {
-        const long long p = idx % ldo;
-        const long long row = idx / ldo;
-        const int c = (int)(row % Cp), tap = (int)(row / Cp);
-        bf16 v = (bf16)0.0f;
-        if (p < P && c < C) {
-            const int xx = (int)(p % W), yy = (int)((p / W) % H);
-            const long long nn = p / ((long long)W * H);
-            const int sy = yy + tap / 3 - 1, sx = xx + tap % 3 - 1;
-            if (sy >= 0 && sy < H && sx >= 0 && sx < W) v = x[((nn * H + sy) * W + sx) * ldx + c];
+    const long long p0 = (long long)blockIdx.x * 64;
+    const int c0 = blockIdx.y * 64;
+    const int t = threadIdx.x;
+    for (int tap = 0; tap < 9; ++tap) {
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        for (int e = t; e < 512; e += 256) {
+            const int pl = e >> 3, ch = (e & 7) * 8;
+            const long long p = p0 + pl;
+            bf16x8 v;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = (bf16)0.0f;
+            if (p < P && c0 + ch < C) {
+                const int xx = (int)(p % W), yy = (int)((p / W) % H);
+                if ((unsigned)(yy + dy) < (unsigned)H && (unsigned)(xx + dx) < (unsigned)W) {
+                    const bf16* src = x + (p + dy * W + dx) * ldx + c0 + ch;
+                    if (c0 + ch + 8 <= C) {
+                        v = *reinterpret_cast<const bf16x8*>(src);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 8; ++k)
+                            if (c0 + ch + k < C) v[k] = src[k];
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) tile[ch + k][pl] = v[k];
         }
-        out[idx] = v;
+        __syncthreads();
+        for (int e = t; e < 512; e += 256) {
+            const int cl = e >> 3, pc = (e & 7) * 8;
+            const int c = c0 + cl;
+            const long long p = p0 + pc;
+            if (c < Cp && p < ldo)
+                *reinterpret_cast<bf16x8*>(out + ((long long)tap * Cp + c) * ldo + p) =
+                    *reinterpret_cast<const bf16x8*>(&tile[cl][pc]);
+        }
+        __syncthreads();
     }
 }
 
@@ -314,8 +335,7 @@ __global__ void k_relu_bwd(bf16* dx, const bf16* act, long long n) {
         if (!(from_bf16(act[i]) > 0.f)) dx[i] = (bf16)0.0f;
 }
 
-// Sum split-K partials: out[i] = sum_z part[z][i]; bias gradient from dy:
-// db[c] = sum_p dy[p][c] (fixed order, one thread per channel).
+// Sum split-K partials: out[i] = sum_z part[z][i] (fixed order).
 __global__ void k_reduce_splits(const float* part, int splits, long long n, float* out) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         float s = 0.f;
@@ -323,11 +343,18 @@ __global__ void k_reduce_splits(const float* part, int splits, long long n, floa
         out[i] = s;
     }
 }
-__global__ __launch_bounds__(256) void k_bias_grad(const bf16* dy, long long P, int C, int ldy, float* db) {
+// db[c] = sum_p dyt[c][p] over the transposed gradient (rows of ldt >= P
+// entries, zero past P, ldt a multiple of 8): contiguous 16-byte loads.
+__global__ __launch_bounds__(256) void k_bias_grad(const bf16* dyt, long long ldt, float* db) {
     __shared__ float sh[256];
     const int c = blockIdx.x;
+    const bf16* row = dyt + (long long)c * ldt;
     float s = 0.f;
-    for (long long p = threadIdx.x; p < P; p += blockDim.x) s += from_bf16(dy[p * ldy + c]);
+    for (long long p = 8LL * threadIdx.x; p < ldt; p += 8LL * blockDim.x) {
+        const bf16x8 v = *reinterpret_cast<const bf16x8*>(row + p);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s += from_bf16(v[e]);
+    }
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int k = 128; k > 0; k >>= 1) {
@@ -351,7 +378,7 @@ __global__ void k_adam(float* w, const float* g, float* m, float* v, long long n
 }
 __global__ void k_weights_bf16(const float* w, int Co, int Ci, int Cip, int Cop, bf16* Wb, bf16* Wt) {
     // w: [Co][9][Ci] fp32 (unpadded);  Wb: [Co][9][Cip];  Wt: [Ci][9][Cop], the
-    // dgrad operand (the tap mirroring is done by the flipped im2col).
+    // dgrad operand (the tap mirroring is done by the dgrad's mirrored implicit im2col).
     const long long total = (long long)Co * 9 * Ci;
     for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
          idx += (long long)gridDim.x * blockDim.x) {
@@ -408,7 +435,7 @@ struct flf_net {
     int step = 0;
     // activation buffers (sized for the largest batch seen)
     long long cap_P = 0;
-    bf16 *x = nullptr, *col = nullptr, *colt = nullptr, *dyt = nullptr, *dbuf0 = nullptr, *dbuf1 = nullptr;
+    bf16 *x = nullptr, *colt = nullptr, *dyt = nullptr, *dbuf0 = nullptr, *dbuf1 = nullptr;
     std::vector<bf16*> h, t;     // h[0..nblocks], t[1..nblocks] (t[0] unused)
     float *o = nullptr, *scale = nullptr, *wpart = nullptr;
     double *lpart = nullptr, *lout = nullptr, *mpart = nullptr;
@@ -459,7 +486,6 @@ int ensure_acts(flf_net* net, long long P) {
     auto& A = net->acts;
     int rc;
     if ((rc = nalloc(net, &net->x, (size_t)P * Cin, A))) return rc;
-    if ((rc = nalloc(net, &net->col, (size_t)P * 9 * maxC, A))) return rc;
     if ((rc = nalloc(net, &net->colt, (size_t)9 * maxC * Pp, A))) return rc;
     if ((rc = nalloc(net, &net->dyt, (size_t)std::max(Wd, Cop) * Pp, A))) return rc;
     if ((rc = nalloc(net, &net->dbuf0, (size_t)P * std::max(Wd, Cop), A))) return rc;
@@ -484,7 +510,15 @@ int ensure_acts(flf_net* net, long long P) {
 
 hipError_t gemm(const GemmArgs& a, int splits, hipStream_t s) {
     dim3 grid((a.N + 127) / 128, (a.M + 63) / 64, splits);
-    hipLaunchKernelGGL(k_gemm_nt, grid, dim3(256), 0, s, a);
+    if (a.conv) {
+        // the implicit im2col reads A while the epilogue writes: no aliasing
+        if (a.convC % 16 || a.K != 9 * a.convC || a.lda % 8 || (const void*)a.A == (const void*)a.Cb)
+            return hipErrorInvalidValue;
+        if (a.conv == 2) hipLaunchKernelGGL(k_gemm_nt<2>, grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(k_gemm_nt<1>, grid, dim3(256), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(k_gemm_nt<0>, grid, dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 
@@ -492,11 +526,9 @@ hipError_t gemm(const GemmArgs& a, int splits, hipStream_t s) {
 int conv_fwd(flf_net* net, const NetLayer& l, const bf16* x, int ldx, int n, int H, int W, int relu, const bf16* resid,
              bf16* outb, float* outf, int ldo, hipStream_t s) {
     const long long P = (long long)n * H * W;
-    hipLaunchKernelGGL(k_im2col, dim3(blocks_for(P * 9 * l.Cip / 8)), dim3(256), 0, s, x, n, H, W, l.Ci, ldx, l.Cip,
-                       0, net->col);
-    NCHK(hipGetLastError());
     GemmArgs a{};
-    a.A = net->col; a.lda = 9LL * l.Cip;
+    a.conv = 1; a.convH = H; a.convW = W; a.convC = l.Cip;
+    a.A = x; a.lda = ldx;
     a.B = l.Wb; a.ldb = 9LL * l.Cip;
     a.M = (int)P; a.N = l.Co; a.K = 9 * l.Cip; a.kchunk = a.K;
     a.bias = net->w + l.boff;
@@ -512,11 +544,9 @@ int conv_fwd(flf_net* net, const NetLayer& l, const bf16* x, int ldx, int n, int
 int conv_dgrad(flf_net* net, const NetLayer& l, const bf16* dy, int ldy, int n, int H, int W, const bf16* add,
                bf16* dx, int ldx, hipStream_t s) {
     const long long P = (long long)n * H * W;
-    hipLaunchKernelGGL(k_im2col, dim3(blocks_for(P * 9 * l.Cop / 8)), dim3(256), 0, s, dy, n, H, W, l.Co, ldy, l.Cop,
-                       1, net->col);
-    NCHK(hipGetLastError());
     GemmArgs a{};
-    a.A = net->col; a.lda = 9LL * l.Cop;
+    a.conv = 2; a.convH = H; a.convW = W; a.convC = l.Cop;
+    a.A = dy; a.lda = ldy;
     a.B = l.Wt; a.ldb = 9LL * l.Cop;
     a.M = (int)P; a.N = l.Ci; a.K = 9 * l.Cop; a.kchunk = a.K;
     a.resid = add; a.ldr = ldx;
@@ -530,8 +560,9 @@ int conv_wgrad(flf_net* net, const NetLayer& l, const bf16* x, int ldx, const bf
                hipStream_t s) {
     const long long P = (long long)n * H * W;
     const long long Pp = (P + 15) / 16 * 16;
-    hipLaunchKernelGGL(k_im2col_t, dim3(blocks_for(9LL * l.Cip * Pp)), dim3(256), 0, s, x, n, H, W, l.Ci, ldx, l.Cip,
-                       Pp, net->colt);
+    if (ldx % 8) return nfail(net, FLF_EINVAL, "wgrad: activation row stride must be a multiple of 8");
+    hipLaunchKernelGGL(k_im2col_t, dim3((unsigned)((Pp + 63) / 64), (l.Cip + 63) / 64), dim3(256), 0, s, x, n, H, W,
+                       l.Ci, ldx, l.Cip, Pp, net->colt);
     hipLaunchKernelGGL(k_transpose, dim3(blocks_for((long long)l.Co * Pp)), dim3(256), 0, s, dy, P, l.Co, ldy, Pp,
                        l.Co, net->dyt);
     NCHK(hipGetLastError());
@@ -553,7 +584,7 @@ int conv_wgrad(flf_net* net, const NetLayer& l, const bf16* x, int ldx, const bf
     // compaction on the host side of the layout: copy rows of Ci out of Cip
     NCHK(hipMemcpy2DAsync(net->g + l.woff, sizeof(float) * l.Ci, net->wpart + (long long)splits * n_pad,
                           sizeof(float) * l.Cip, sizeof(float) * l.Ci, (size_t)l.Co * 9, hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(k_bias_grad, dim3(l.Co), dim3(256), 0, s, dy, P, l.Co, ldy, net->g + l.boff);
+    hipLaunchKernelGGL(k_bias_grad, dim3(l.Co), dim3(256), 0, s, net->dyt, Pp, net->g + l.boff);
     NCHK(hipGetLastError());
     return FLF_OK;
 }
@@ -616,9 +647,9 @@ int backward(flf_net* net, const float* tgt, int T, int Ly, int Lx, hipStream_t 
         hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(P * Wd)), dim3(256), 0, s, dt, net->t[i], P * Wd);
         NCHK(hipGetLastError());
         if ((rc = conv_wgrad(net, la, net->h[i - 1], Wd, dt, Wd, T, Ly, Lx, s))) return rc;
-        // dh_{i-1} = dh_i + dgrad_a(dt): accumulate into dt's buffer, then swap
-        if ((rc = conv_dgrad(net, la, dt, Wd, T, Ly, Lx, dh, dt, Wd, s))) return rc;
-        std::swap(dh, dt);
+        // dh_{i-1} = dh_i + dgrad_a(dt), accumulated in place in dh (each
+        // element is read and written by the same thread; dt is only read)
+        if ((rc = conv_dgrad(net, la, dt, Wd, T, Ly, Lx, dh, dh, Wd, s))) return rc;
     }
     hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(P * Wd)), dim3(256), 0, s, dh, net->h[0], P * Wd);
     NCHK(hipGetLastError());
