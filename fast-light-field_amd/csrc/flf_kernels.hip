@@ -906,8 +906,8 @@ __global__ __launch_bounds__(64 * TC_WAVES) void k_mac_tc(MacArgs a) {
     const int pe = (int)(((long long)(blockIdx.y + 1) * a.nP) / gridDim.y);
     for (int p = pb; p < pe; ++p) {
         if (!a.enable.on(p)) continue;
-        const int s2 = a.s2[p];
-        __syncthreads();                          // the previous plane is done with Bt and psi
+        const int s2 = const_load(a.s2, p);
+        __syncthreads();                          // the previous plane is done with Bt, psi and s_stage
         if (tid < 2 * NRES) {                     // flip factors per residue: exp(-2 pi i (s2-1) kt / f)
             const int rho = tid >> 1;
             const bool isY = (tid & 1) == 0;
@@ -927,54 +927,63 @@ __global__ __launch_bounds__(64 * TC_WAVES) void k_mac_tc(MacArgs a) {
 #pragma unroll
             for (int j = 0; j < SLOTS; ++j) acc[q][j] = v4f{0.f, 0.f, 0.f, 0.f};
         const float2* Gp = a.G + item_base * SLOTS * nu * a.nP + ((long long)p * nu) * ustride;
-        {
-            // the chunk's small-FFT block of every frame, staged once per plane
-            const int blk = NN * NH;
-            const float2* Sp = a.S + a.S_off[p] + (long long)cidx * blk;
-            for (int i = tid; i < nf * blk; i += blockDim.x) {
-                const int f = i / blk, e = i - f * blk;
-                s_stage[i] = Sp[(long long)(t0 + f) * a.S_tstride + e];
-            }
-        }
+        // the chunk's small-FFT block of every frame, staged once per plane by
+        // global-to-LDS loads all in flight at once (awaited here; the
+        // barrier at the top of the batch loop makes it visible)
+        mac_stage_async<SLOTS>(a, cidx, p, t0, nf, s_stage);
+        wait_vmem_all();
         for (int u0 = 0; u0 < nu; u0 += TC_UB) {
             const int ubn = min(TC_UB, nu - u0);
             __syncthreads();                      // psi written; the previous batch's products are done
-            // B of spectra u0 .. u0 + ubn - 1: one (P, M) pair per (spectrum, residue, pair type, frame)
-            for (int e = tid; e < TC_UB * NRES * NPT * TC_FB; e += blockDim.x) {
-                const int f = e % TC_FB;
-                int rest = e / TC_FB;
-                const int pt = rest % NPT;
-                rest /= NPT;
-                const int rho = rest % NRES;
-                const int ub = rest / NRES;
-                float4 pm = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (ub < ubn && f < nf) {
+            // B of spectra u0 .. u0 + ubn - 1: one (P, M) pair per (spectrum,
+            // residue, pair type, frame).  Thread -> column (ub, pt), decoded
+            // once, and a share of the (residue, frame) rows.
+            {
+                constexpr int Q = TC_UB * NPT;
+                constexpr int ROWS = NRES * TC_FB;
+                const int rstep = (int)blockDim.x / Q;
+                const int r0 = tid / Q;
+                const int col = tid - r0 * Q;
+                const int pt = col % NPT, ub = col / NPT;
+                int src[2] = {-1, -1}, flip[2] = {0, 0};
+                if (r0 < rstep && ub < ubn) {
                     const int u = u0 + ub;
-                    const int rb0 = rho < NH ? rho : rho - NH;
-                    const float2* sf = s_stage + f * NN * NH;
-                    float2 sv[2];
 #pragma unroll
                     for (int w = 0; w < 2; ++w) {
                         const int off = a.uoff[u * NUSE + (w == 0 ? kPairPlain[pt] : kPairConj[pt])];
-                        float2 r = make_float2(0.f, 0.f);
-                        if (off < NN) {
-                            const int bo = off / N, ao = off - bo * N;
-                            r = sf[off * NH + rb0];
-                            if (rho >= NH) r = cconj(r);
-                            if (bo > c) r = cmul(r, psi[2 * rho]);
-                            if (ao > c) r = cmul(r, psi[2 * rho + 1]);
-                        }
-                        sv[w] = r;
+                        const int bo = off / N, ao = off - bo * N;
+                        src[w] = off < NN ? off * NH : -1;
+                        flip[w] = (bo > c ? 1 : 0) | (ao > c ? 2 : 0);
                     }
-                    const float2 P = cadd(sv[0], sv[1]);
-                    const float2 d = csub(sv[0], sv[1]);
-                    pm = make_float4(P.x, P.y, -d.y, d.x);                    // M = i * d
                 }
-                float* b = Bt + ((ub * NRES + rho) * KS + (pt >> 1)) * 64 + 2 * (pt & 1) * 16 + 2 * f;
-                b[0] = pm.x;                      // k row 2 (pt & 1): Re g -> P
-                b[1] = pm.y;
-                b[16] = pm.z;                     // k row 2 (pt & 1) + 1: Im g -> M
-                b[17] = pm.w;
+                for (int rw = r0; r0 < rstep && rw < ROWS; rw += rstep) {
+                    const int f = rw % TC_FB, rho = rw / TC_FB;
+                    float4 pm = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (ub < ubn && f < nf) {
+                        const int rb0 = rho < NH ? rho : rho - NH;
+                        const float2* sf = s_stage + f * NN * NH;
+                        float2 sv[2];
+#pragma unroll
+                        for (int w = 0; w < 2; ++w) {
+                            float2 r = make_float2(0.f, 0.f);
+                            if (src[w] >= 0) {
+                                r = sf[src[w] + rb0];
+                                if (rho >= NH) r = cconj(r);
+                                if (flip[w] & 1) r = cmul(r, psi[2 * rho]);
+                                if (flip[w] & 2) r = cmul(r, psi[2 * rho + 1]);
+                            }
+                            sv[w] = r;
+                        }
+                        const float2 P = cadd(sv[0], sv[1]);
+                        const float2 d = csub(sv[0], sv[1]);
+                        pm = make_float4(P.x, P.y, -d.y, d.x);                    // M = i * d
+                    }
+                    float* b = Bt + ((ub * NRES + rho) * KS + (pt >> 1)) * 64 + 2 * (pt & 1) * 16 + 2 * f;
+                    b[0] = pm.x;                      // k row 2 (pt & 1): Re g -> P
+                    b[1] = pm.y;
+                    b[16] = pm.z;                     // k row 2 (pt & 1) + 1: Im g -> M
+                    b[17] = pm.w;
+                }
             }
             __syncthreads();
             // Spectra outer, tiles inner: the loads of spectrum ub + 1 for all
@@ -1018,13 +1027,15 @@ __global__ __launch_bounds__(64 * TC_WAVES) void k_mac_tc(MacArgs a) {
                     }
                 }
             };
+            // The loads are unconditional (past the batch they re-read its
+            // last spectrum, in bounds and unused): with a branch around
+            // them the compiler's single wait before the products would
+            // have to drain the loads just issued.
             for (int ub = 0; ub < ubn; ub += 2) {
-                if (ub + 1 < ubn) load_u(gB, ub + 1);
+                load_u(gB, min(ub + 1, ubn - 1));
                 products(gA, ub);
-                if (ub + 1 < ubn) {
-                    if (ub + 2 < ubn) load_u(gA, ub + 2);
-                    products(gB, ub + 1);
-                }
+                load_u(gA, min(ub + 2, ubn - 1));
+                if (ub + 1 < ubn) products(gB, ub + 1);
             }
         }
         // Epilogue of plane p: outer phase and 'same'-crop shift per item and
