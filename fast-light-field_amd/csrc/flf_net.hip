@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -79,84 +80,156 @@ __device__ __forceinline__ bf16x8 load8(const bf16* p, bool ok) {
     return z;
 }
 
-template <int CONV>
+// 16 bytes at p, or zeros: the load always happens (from a valid address) and
+// invalid lanes are cleared by a select, so no branch surrounds it.
+__device__ __forceinline__ bf16x8 load8_sel(const bf16* p, const bf16* valid_base, bool ok) {
+    const bf16x8 v = *reinterpret_cast<const bf16x8*>(ok ? p : valid_base);
+    bf16x8 z;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) z[e] = ok ? v[e] : (bf16)0.0f;
+    return z;
+}
+
+// C[M x N] (+)= A[M x K] * B[N x K]^T through LDS: workgroup tile BM x 128
+// (four waves, 2 x 2, each BM/2 x 64 of 32x32 MFMA tiles), K in steps of 32
+// staged in LDS (two buffers).  Rows are padded to 40 bf16 (80 bytes), so the
+// 16-byte fragment reads of 16 consecutive rows fall on distinct banks.
+// Global loads run PF steps ahead in a register ring: a step's loads are
+// issued PF - 1 steps before its LDS store.  Loads are unconditional (masked
+// by selects, clamped addresses) and the step count is padded to a multiple
+// of PF with zero steps, so the unrolled loop has no branches and the
+// compiler's waits cover only the step being stored.  Operand layout per
+// lane l (32x32x16 bf16):
+//   A[i = l % 32][k = 8 (l / 32) + e]     B[k = 8 (l / 32) + e][j = l % 32]
+//   D[i = 8 (r / 4) + 4 (l / 32) + r % 4][j = l % 32]   r = 0..15
+// K a multiple of 16 (a step may end in zeros past ke).  grid (ceil(N/128),
+// ceil(M/BM), splitK).
+template <int CONV, int BM>
 __global__ __launch_bounds__(256) void k_gemm_nt(GemmArgs g) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int m0 = blockIdx.y * 64 + (wv >> 1) * 32;
-    const int n0 = blockIdx.x * 128 + (wv & 1) * 64;
+    constexpr int BN = 128, BK = 32, LDK = BK + 8, PF = 4;
+    constexpr int WM = BM / 2, MT = WM / 32, NT = 2;
+    constexpr int ACH = BM * BK / 8 / 256, BCH = BN * BK / 8 / 256;   // 16-byte chunks per thread and step
+    __shared__ __align__(16) bf16 As[2][BM][LDK];
+    __shared__ __align__(16) bf16 Bs[2][BN][LDK];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int mb = blockIdx.y * BM, nb = blockIdx.x * BN;
+    const int wm0 = (wv >> 1) * WM, wn0 = (wv & 1) * 64;
     const int kb = blockIdx.z * g.kchunk;
     const int ke = min(g.K, kb + g.kchunk);
-    const int li = lane & 31, lk = (lane >> 5) * 8;
-    const int am = m0 + li;
-    const int bn0 = n0 + li, bn1 = n0 + 32 + li;
-    const bool aok = am < g.M, b0ok = bn0 < g.N, b1ok = bn1 < g.N;
-    const bf16* pa = g.A + (long long)(aok ? am : 0) * g.lda + lk;
-    // implicit im2col: the lane's position in its frame
-    const int ax = CONV ? am % g.convW : 0;
-    const int ay = CONV ? (am / g.convW) % g.convH : 0;
-    auto load_a = [&](int k, bool in) -> bf16x8 {
-        if constexpr (CONV == 0) {
-            return load8(pa + k, aok && in);
-        } else {
-            const int tap = k / g.convC, c = k - tap * g.convC;
-            const int tt = CONV == 2 ? 8 - tap : tap;
-            const int dy = tt / 3 - 1, dx = tt - 3 * (tt / 3) - 1;
-            const bool ok = in && aok && (unsigned)(ay + dy) < (unsigned)g.convH &&
-                            (unsigned)(ax + dx) < (unsigned)g.convW;
-            return load8(ok ? g.A + ((long long)am + dy * g.convW + dx) * g.lda + c + lk : g.A, ok);
+    const int nsteps = ke > kb ? (ke - kb + BK - 1) / BK : 0;
+    const int npad = (nsteps + PF - 1) / PF * PF;
+    // this thread's load chunks: row c / 4, k offset 8 (c % 4) of the step
+    int arow[ACH], akc[ACH], ay[ACH], ax[ACH];
+#pragma unroll
+    for (int h = 0; h < ACH; ++h) {
+        const int c = tid + 256 * h;
+        arow[h] = c >> 2;
+        akc[h] = (c & 3) * 8;
+        const int am = mb + arow[h];
+        ax[h] = CONV ? am % g.convW : 0;
+        ay[h] = CONV ? (am / g.convW) % g.convH : 0;
+    }
+    int brow[BCH], bkc[BCH];
+#pragma unroll
+    for (int h = 0; h < BCH; ++h) {
+        const int c = tid + 256 * h;
+        brow[h] = c >> 2;
+        bkc[h] = (c & 3) * 8;
+    }
+    bf16x8 ra[PF][ACH], rb[PF][BCH];
+    auto gload = [&](int slot, int st) {      // step st (past the last: zeros)
+        const int k0 = kb + st * BK;
+#pragma unroll
+        for (int h = 0; h < ACH; ++h) {
+            const int am = mb + arow[h], k = k0 + akc[h];
+            bool ok = am < g.M && k < ke;
+            const bf16* src;
+            if constexpr (CONV == 0) {
+                src = g.A + (long long)am * g.lda + k;
+            } else {
+                const int tap = k / g.convC, c = k - tap * g.convC;
+                const int tt = CONV == 2 ? 8 - tap : tap;
+                const int dy = tt / 3 - 1, dx = tt - 3 * (tt / 3) - 1;
+                ok = ok && (unsigned)(ay[h] + dy) < (unsigned)g.convH && (unsigned)(ax[h] + dx) < (unsigned)g.convW;
+                src = g.A + ((long long)am + dy * g.convW + dx) * g.lda + c;
+            }
+            ra[slot][h] = load8_sel(src, g.A, ok);
+        }
+#pragma unroll
+        for (int h = 0; h < BCH; ++h) {
+            const int bn = nb + brow[h], k = k0 + bkc[h];
+            const bool ok = bn < g.N && k < ke;
+            rb[slot][h] = load8_sel(g.B + (long long)bn * g.ldb + k, g.B, ok);
         }
     };
-    const bf16* pb0 = g.B + (long long)(b0ok ? bn0 : 0) * g.ldb + lk;
-    const bf16* pb1 = g.B + (long long)(b1ok ? bn1 : 0) * g.ldb + lk;
-    f32x16 c0 = {}, c1 = {};
-    // Ring of PF k-steps whose operand loads are in flight while earlier
-    // steps run on the matrix cores (the small-M inference GEMMs are bound
-    // by load latency); the MFMA order along K is unchanged.
-    constexpr int PF = 4;
-    const int nsteps = ke > kb ? (ke - kb) / 16 : 0;
-    bf16x8 ra[PF], rb0[PF], rb1[PF];
+    auto lstore = [&](int slot, int buf) {
 #pragma unroll
-    for (int d = 0; d < PF; ++d) {
-        const bool in = d < nsteps;
-        const int k = kb + 16 * d;
-        ra[d] = load_a(in ? k : kb, in);
-        rb0[d] = load8(pb0 + (in ? k : kb), b0ok && in);
-        rb1[d] = load8(pb1 + (in ? k : kb), b1ok && in);
-    }
-    for (int s0 = 0; s0 < nsteps; s0 += PF) {
+        for (int h = 0; h < ACH; ++h) *reinterpret_cast<bf16x8*>(&As[buf][arow[h]][akc[h]]) = ra[slot][h];
 #pragma unroll
-        for (int d = 0; d < PF; ++d) {
-            const int st = s0 + d;
-            if (st < nsteps) {
-                c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ra[d], rb0[d], c0, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ra[d], rb1[d], c1, 0, 0, 0);
-                if (st + PF < nsteps) {
-                    const int k = kb + 16 * (st + PF);
-                    ra[d] = load_a(k, true);
-                    rb0[d] = load8(pb0 + k, b0ok);
-                    rb1[d] = load8(pb1 + k, b1ok);
+        for (int h = 0; h < BCH; ++h) *reinterpret_cast<bf16x8*>(&Bs[buf][brow[h]][bkc[h]]) = rb[slot][h];
+    };
+    f32x16 acc[MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x16{};
+    const int li = lane & 31, lk = (lane >> 5) * 8;
+    if (npad > 0) {
+#pragma unroll
+        for (int d = 0; d < PF; ++d) gload(d, d);
+        // ring rule: once slot s (step t) is stored, it takes step t + PF
+        lstore(0, 0);
+        gload(0, PF);
+        __syncthreads();
+        for (int st0 = 0; st0 < npad; st0 += PF) {
+#pragma unroll
+            for (int d = 0; d < PF; ++d) {
+                const int cur = d & 1;            // PF even: buffer of step st0 + d
+#pragma unroll
+                for (int kk = 0; kk < BK / 16; ++kk) {
+                    bf16x8 af[MT], bfr[NT];
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+                        af[mt] = *reinterpret_cast<const bf16x8*>(&As[cur][wm0 + mt * 32 + li][kk * 16 + lk]);
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt)
+                        bfr[nt] = *reinterpret_cast<const bf16x8*>(&Bs[cur][wn0 + nt * 32 + li][kk * 16 + lk]);
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                        for (int nt = 0; nt < NT; ++nt)
+                            acc[mt][nt] =
+                                __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[mt], bfr[nt], acc[mt][nt], 0, 0, 0);
                 }
+                // step st0 + d + 1 (loaded PF - 1 steps ago) into the other
+                // buffer, last read one step ago; then its ring slot takes
+                // step st0 + d + 1 + PF
+                lstore((d + 1) % PF, cur ^ 1);
+                gload((d + 1) % PF, st0 + d + 1 + PF);
+                __syncthreads();
             }
         }
     }
     // epilogue
 #pragma unroll
-    for (int half = 0; half < 2; ++half) {
-        const f32x16& c = half ? c1 : c0;
-        const int j = n0 + half * 32 + li;
-        if (j >= g.N) continue;
-        const float bj = g.bias ? g.bias[j] : 0.f;
+    for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int i = m0 + 8 * (r / 4) + 4 * (lane >> 5) + (r & 3);
-            if (i >= g.M) continue;
-            float v = c[r] + bj;
-            if (g.relu) v = fmaxf(v, 0.f);
-            if (g.resid) v += from_bf16(g.resid[(long long)i * g.ldr + j]);
-            if (g.Cf) g.Cf[(long long)blockIdx.z * g.M * g.ldcf + (long long)i * g.ldcf + j] = v;
-            if (g.Cb) g.Cb[(long long)i * g.ldcb + j] = to_bf16(v);
+        for (int nt = 0; nt < NT; ++nt) {
+            const f32x16& c = acc[mt][nt];
+            const int j = nb + wn0 + nt * 32 + li;
+            if (j >= g.N) continue;
+            const float bj = g.bias ? g.bias[j] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = mb + wm0 + mt * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r & 3);
+                if (i >= g.M) continue;
+                float v = c[r] + bj;
+                if (g.relu) v = fmaxf(v, 0.f);
+                if (g.resid) v += from_bf16(g.resid[(long long)i * g.ldr + j]);
+                if (g.Cf) g.Cf[(long long)blockIdx.z * g.M * g.ldcf + (long long)i * g.ldcf + j] = v;
+                if (g.Cb) g.Cb[(long long)i * g.ldcb + j] = to_bf16(v);
+            }
         }
-    }
 }
 
 // Transposed im2col for wgrad: out[tap*Cp + c][p] (row length ldo >= P, zero
@@ -508,16 +581,33 @@ int ensure_acts(flf_net* net, long long P) {
     return FLF_OK;
 }
 
+// Workgroup tile rows: 128 when that still gives every CU a workgroup,
+// else 64 (FLF_GEMM_BM overrides).
+int gemm_bm(const GemmArgs& a, int splits) {
+    static const int env = std::getenv("FLF_GEMM_BM") ? std::atoi(std::getenv("FLF_GEMM_BM")) : 0;
+    if (env == 64 || env == 128) return env;
+    const long long wg128 = (long long)((a.N + 127) / 128) * ((a.M + 127) / 128) * splits;
+    return wg128 >= 256 ? 128 : 64;
+}
+
+template <int CONV>
+void launch_gemm(const GemmArgs& a, int splits, hipStream_t s) {
+    if (gemm_bm(a, splits) == 128)
+        hipLaunchKernelGGL((k_gemm_nt<CONV, 128>), dim3((a.N + 127) / 128, (a.M + 127) / 128, splits), dim3(256), 0, s,
+                           a);
+    else
+        hipLaunchKernelGGL((k_gemm_nt<CONV, 64>), dim3((a.N + 127) / 128, (a.M + 63) / 64, splits), dim3(256), 0, s, a);
+}
+
 hipError_t gemm(const GemmArgs& a, int splits, hipStream_t s) {
-    dim3 grid((a.N + 127) / 128, (a.M + 63) / 64, splits);
+    if (a.K % 16 || a.kchunk % 16 || a.lda % 8 || a.ldb % 8) return hipErrorInvalidValue;
     if (a.conv) {
         // the implicit im2col reads A while the epilogue writes: no aliasing
-        if (a.convC % 16 || a.K != 9 * a.convC || a.lda % 8 || (const void*)a.A == (const void*)a.Cb)
-            return hipErrorInvalidValue;
-        if (a.conv == 2) hipLaunchKernelGGL(k_gemm_nt<2>, grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL(k_gemm_nt<1>, grid, dim3(256), 0, s, a);
+        if (a.convC % 16 || a.K != 9 * a.convC || (const void*)a.A == (const void*)a.Cb) return hipErrorInvalidValue;
+        if (a.conv == 2) launch_gemm<2>(a, splits, s);
+        else launch_gemm<1>(a, splits, s);
     } else {
-        hipLaunchKernelGGL(k_gemm_nt<0>, grid, dim3(256), 0, s, a);
+        launch_gemm<0>(a, splits, s);
     }
     return hipGetLastError();
 }
