@@ -470,6 +470,7 @@ int launch_grouped(flf_ctx* ctx, const std::vector<GroupWork>& ws, hipStream_t s
     MacGrouped mg{};
     IfftColGrouped cg{};
     sg.gg.ngroups = mg.gg.ngroups = cg.gg.ngroups = (int)ws.size();
+    mg.nfb = (ctx->T + fb - 1) / fb;
     size_t lds_s = 0, lds_m = 0, lds_c = 0;
     double bytes = 0;
     for (size_t k = 0; k < ws.size(); ++k) {
@@ -481,7 +482,7 @@ int launch_grouped(flf_ctx* ctx, const std::vector<GroupWork>& ws, hipStream_t s
         lds_s = std::max(lds_s, small_fft_lds(w.sa.rpY.L, w.sa.rpX.L));
         mg.a[k] = w.ma;
         const int nch = (int)g.gt.chunks.size();
-        mg.gg.start[k + 1] = mg.gg.start[k] + (nch * w.nranges + 7) / 8 * 8;   // XCD-aligned (k_mac_grouped)
+        mg.gg.start[k + 1] = mg.gg.start[k] + (nch * w.nranges * mg.nfb + 7) / 8 * 8;   // XCD-aligned (k_mac_grouped)
         mg.gg.nchunks[k] = nch;
         mg.gg.nranges[k] = w.nranges;
         lds_m = std::max(lds_m, mac_lds(g.gt.slots, g.gt.nu, ctx->N, fb));
@@ -504,7 +505,7 @@ int launch_grouped(flf_ctx* ctx, const std::vector<GroupWork>& ws, hipStream_t s
         eb = take_event(ctx);
         if (ea && eb) HIPCHK(hipEventRecord(ea, st));
     }
-    HIPCHK(launch_mac_grouped(mg, g0.slots, g0.ipt, fb, (ctx->T + fb - 1) / fb, lds_m, st));
+    HIPCHK(launch_mac_grouped(mg, g0.slots, g0.ipt, fb, mg.nfb, lds_m, st));
     if (ctx->prof && ea && eb) {
         HIPCHK(hipEventRecord(eb, st));
         ctx->prof_recs.push_back({ea, eb, bytes});

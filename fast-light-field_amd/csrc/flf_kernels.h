@@ -117,7 +117,8 @@ struct SmallFFTGrouped {
 };
 struct MacGrouped {
     MacArgs a[FLF_MAX_GROUPS];
-    GroupedGrid gg;
+    GroupedGrid gg;                               // blocks per group: nchunks x nranges x nfb, padded to 8
+    int nfb;                                      // frame blocks
 };
 struct IfftColGrouped {
     IfftColArgs a[FLF_MAX_GROUPS];

@@ -574,8 +574,8 @@ __device__ __forceinline__ void mac_tables(const MacArgs& a, const ChunkDesc* ch
 // (projector.py:792-796) folded into phases.  IPT = 1: one item per
 // thread; IPT = 2: a lane pair owns two consecutive items that share their
 // residues and splits the spectra between its lanes.  FB frames per
-// workgroup: every stored spectrum value read from HBM serves FB frames
-// (grid z = frame blocks).
+// workgroup: every stored spectrum value loaded serves FB frames, and the
+// frame blocks of one chunk run side by side on one XCD (block ids below).
 // One workgroup: chunk cidx, plane range `range` of nranges, frame block
 // fblock.
 template <int SLOTS, int IPT, int FB>
@@ -796,22 +796,32 @@ __device__ __forceinline__ void mac_body(const MacArgs& a, int cidx, int range, 
 template <int SLOTS, int IPT, int FB>
 __global__ __launch_bounds__(FLF_CHUNK_MAX) __attribute__((amdgpu_waves_per_eu(IPT * FB >= 3 ? 2 : (IPT * FB == 2 ? 3 : FLF_MAC_WAVES))))
 void k_mac(MacArgs a) {
-    mac_body<SLOTS, IPT, FB>(a, xcd_block(blockIdx.x, gridDim.x), blockIdx.y, gridDim.y, blockIdx.z);
+    // grid x = chunk x frame block, frame block fastest: after the XCD remap
+    // the frame blocks of a chunk share one XCD's L2 for the spectra they
+    // all read
+    const int nfb = (a.T + FB - 1) / FB;
+    const int w = xcd_block(blockIdx.x, gridDim.x);
+    mac_body<SLOTS, IPT, FB>(a, w / nfb, blockIdx.y, gridDim.y, w % nfb);
 }
 
 // All active grid groups of one projection: grid x flattened over (group,
-// plane range, chunk), z = frame blocks.  Each group's range starts at a
-// multiple of 8 (padded with idle workgroups), and the XCD remap applies
-// within the group: every XCD then takes an eighth of every group, whose
-// workgroups cost about the same, where one remap over the whole grid would
-// give some XCDs only a cheap group's workgroups and leave them idle early.
+// plane range, chunk, frame block), frame block fastest.  Each group's range
+// starts at a multiple of 8 (padded with idle workgroups), and the XCD remap
+// applies within the group: every XCD then takes an eighth of every group,
+// whose workgroups cost about the same, where one remap over the whole grid
+// would give some XCDs only a cheap group's workgroups and leave them idle
+// early.  After the remap the frame blocks of one (chunk, plane range) are
+// consecutive ids of one XCD, dispatched together: they read the same
+// stored spectra at about the same time, which the XCD's L2 then serves
+// once from HBM for all of them.
 template <int SLOTS, int IPT, int FB>
 __global__ __launch_bounds__(FLF_CHUNK_MAX) __attribute__((amdgpu_waves_per_eu(IPT * FB >= 3 ? 2 : (IPT * FB == 2 ? 3 : FLF_MAC_WAVES))))
 void k_mac_grouped(MacGrouped ga) {
     const GroupPick p = pick_group(ga.gg, blockIdx.x);
     const int w = xcd_block(p.rem, p.n);
-    if (w >= p.nch * p.nr) return;                // padding
-    mac_body<SLOTS, IPT, FB>(ga.a[p.g], w % p.nch, w / p.nch, p.nr, blockIdx.z);
+    const int fblock = w % ga.nfb, r = w / ga.nfb;
+    if (r >= p.nch * p.nr) return;                // padding
+    mac_body<SLOTS, IPT, FB>(ga.a[p.g], r % p.nch, r / p.nch, p.nr, fblock);
 }
 
 // Matrix-core multiply-accumulate, TC_FB = 8 frames per workgroup
@@ -848,7 +858,10 @@ __global__ __launch_bounds__(64 * TC_WAVES) void k_mac_tc(MacArgs a) {
     float* Bt = reinterpret_cast<float*>(tc_lds);                          // [TC_UB][NRES][KS][64]
     float2* psi = reinterpret_cast<float2*>(Bt + TC_UB * NRES * KS * 64);  // [NRES][2]
     float2* s_stage = psi + 2 * NRES;                                      // [TC_FB][N*N][NH]
-    const int cidx = xcd_block(blockIdx.x, gridDim.x);
+    // grid x = chunk x frame block, frame block fastest (as k_mac)
+    const int nfb = (a.T + TC_FB - 1) / TC_FB;
+    const int wblk = xcd_block(blockIdx.x, gridDim.x);
+    const int cidx = wblk / nfb;
     const ChunkDesc* ch = a.chunks + cidx;
     const long long item_base = ch->item_base;
     const int n_items = ch->n_items;
@@ -856,7 +869,7 @@ __global__ __launch_bounds__(64 * TC_WAVES) void k_mac_tc(MacArgs a) {
     const int row = lane & 15, kq = lane >> 4;
     const int col = row, f_col = col >> 1, c_col = col & 1;
     const int N = a.N, NN = N * N, c = N / 2, fourN = 4 * N;
-    const int t0 = blockIdx.z * TC_FB;
+    const int t0 = (wblk - cidx * nfb) * TC_FB;
     const int nf = min(TC_FB, a.T - t0);
     const int nu = a.nu;
     const TileDesc* tiles = a.tiles + ch->tile_base;
@@ -1514,10 +1527,10 @@ hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks,
     if (fb < 1 || fb > 4 || (ipt != 1 && ipt != 2) || a.T < 1 || a.rs != mac_row_stride(slots, a.nu))
         return hipErrorInvalidValue;
     const size_t lds = mac_lds(slots, a.nu, a.N, fb);
-    dim3 grid(nchunks, nplanes_grid, (a.T + fb - 1) / fb);
+    dim3 grid(nchunks * ((a.T + fb - 1) / fb), nplanes_grid);
     if (mfma && a.tiles) {
         // matrix-core kernel: TC_FB frames per workgroup
-        dim3 tgrid(nchunks, nplanes_grid, (a.T + TC_FB - 1) / TC_FB);
+        dim3 tgrid(nchunks * ((a.T + TC_FB - 1) / TC_FB), nplanes_grid);
         const size_t tlds = mac_tc_lds(slots, a.N);
         if (slots == 4) hipLaunchKernelGGL((k_mac_tc<4>), tgrid, dim3(64 * TC_WAVES), tlds, s, a);
         else hipLaunchKernelGGL((k_mac_tc<2>), tgrid, dim3(64 * TC_WAVES), tlds, s, a);
@@ -1545,7 +1558,10 @@ hipError_t launch_mac_grouped(const MacGrouped& ga, int slots, int ipt, int fb, 
     if (fb < 1 || fb > 4 || (ipt != 1 && ipt != 2) || (slots != 2 && slots != 4) || nfb < 1 || gg.ngroups < 1 ||
         gg.ngroups > FLF_MAX_GROUPS)
         return hipErrorInvalidValue;
-    dim3 grid(gg.start[gg.ngroups], 1, nfb);
+    if (ga.nfb != nfb) return hipErrorInvalidValue;
+    for (int k = 0; k < gg.ngroups; ++k)
+        if (gg.start[k + 1] - gg.start[k] < gg.nchunks[k] * gg.nranges[k] * nfb) return hipErrorInvalidValue;
+    dim3 grid(gg.start[gg.ngroups]);
     dim3 block(FLF_CHUNK_MAX);
     if (slots == 4) {
         if (ipt == 1) launch_mac_grouped_fb<4, 1>(ga, fb, grid, block, lds, s);
