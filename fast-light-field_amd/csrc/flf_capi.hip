@@ -944,7 +944,7 @@ int flf_set_option(flf_ctx* ctx, const char* name, int value) {
         if (value < 0 || value >= FLF_RADIX_ORDERS) return fail(ctx, FLF_EINVAL, "radix_order must be 0..3");
         ctx->radix_order = value;
     } else if (n == "mac_ipt") {                 // takes effect at the next flf_plan
-        if (value != 1 && value != 2) return fail(ctx, FLF_EINVAL, "mac_ipt must be 1 or 2");
+        if (value != 1 && value != 2 && value != 4) return fail(ctx, FLF_EINVAL, "mac_ipt must be 1, 2 or 4");
         ctx->ipt = value;
     } else if (n == "ifft_col_batch") {
         if (!ctx->planned) return fail(ctx, FLF_ESTATE, "flf_plan has not been called");

@@ -572,8 +572,8 @@ __device__ __forceinline__ void mac_tables(const MacArgs& a, const ChunkDesc* ch
 // This is the accumulation of py_light_field.cpp:303-341, 1029-1087 with the
 // mirror multipliers of projector.py:137-151 and the 'same' crop
 // (projector.py:792-796) folded into phases.  IPT = 1: one item per
-// thread; IPT = 2: a lane pair owns two consecutive items that share their
-// residues and splits the spectra between its lanes.  FB frames per
+// thread; IPT = 2 (4): a lane pair (quad) owns two (four) consecutive items
+// that share their residues and splits the spectra between its lanes.  FB frames per
 // workgroup: every stored spectrum value loaded serves FB frames, and the
 // frame blocks of one chunk run side by side on one XCD (block ids below).
 // One workgroup: chunk cidx, plane range `range` of nranges, frame block
@@ -589,14 +589,15 @@ __device__ __forceinline__ void mac_body(const MacArgs& a, int cidx, int range, 
     const long long item_base = ch->item_base;
     const int n_items = ch->n_items;
     const int tid = threadIdx.x;
-    // IPT = 2: a lane pair owns two items that share their residues and
-    // splits the stored spectra between its lanes (lane parity lsub takes
-    // u = lsub, lsub + 2, ...), so every P/M table read serves two items
-    // while the workgroup keeps one thread per item slot; the pair adds its
-    // partial sums before each plane's epilogue.
-    constexpr int USTEP = IPT == 2 ? 2 : 1;
-    const int lsub = IPT == 2 ? (tid & 1) : 0;
-    const int slot = IPT == 2 ? (tid >> 1) : tid;
+    // IPT = 2 or 4: a group of IPT lanes owns IPT items that share their
+    // residues and splits the stored spectra between its lanes (lane lsub
+    // takes u = lsub, lsub + IPT, ...), so every P/M table read serves IPT
+    // items while the workgroup keeps one thread per item slot; the group
+    // adds its partial sums before each plane's epilogue.
+    static_assert(IPT == 1 || IPT == 2 || IPT == 4, "items per lane group: 1, 2 or 4");
+    constexpr int USTEP = IPT;
+    const int lsub = tid % USTEP;
+    const int slot = tid / USTEP;
     const bool valid = slot * IPT < n_items;
     const int tsafe = valid ? slot : 0;          // idle lanes read item 0 (in bounds) and never write
     const int N = a.N, NN = N * N, fourN = 4 * N;
@@ -631,7 +632,7 @@ __device__ __forceinline__ void mac_body(const MacArgs& a, int cidx, int range, 
         return itp + z;
     };
 
-    constexpr int QT = IPT == 2 ? 1 : IPT;        // items whose totals a lane keeps
+    constexpr int QT = 1;                         // items whose totals a lane keeps
     float2 tot[FB][QT][4];
 #pragma unroll
     for (int f = 0; f < FB; ++f)
@@ -678,7 +679,7 @@ __device__ __forceinline__ void mac_body(const MacArgs& a, int cidx, int range, 
             float2* A = a.A + (long long)(t0 + f) * a.A_tstride + (long long)slice * a.A_pstride;
 #pragma unroll
             for (int q = 0; q < QT; ++q) {
-                const ItemDesc& it = ip[IPT == 2 ? lsub : q];
+                const ItemDesc& it = ip[IPT >= 2 ? lsub : q];
 #pragma unroll
                 for (int j = 0; j < SLOTS; ++j)
                     if ((it.writemask >> j) & 1) A[it.pos[j]] = tot[f][q][j];
@@ -694,6 +695,9 @@ __device__ __forceinline__ void mac_body(const MacArgs& a, int cidx, int range, 
         const int c2 = (s2 - 1) / 2;
         // IPT = 2: each lane keeps its own item (q = lsub): it sends the
         // partner's item and adds what the partner sends for its own.
+        // IPT = 4: the same in two steps, first between lanes lsub and
+        // lsub ^ 2 (each keeps the two items whose bit 1 is its own), then
+        // between lsub and lsub ^ 1.
         float2 own[FB][4];
         if constexpr (IPT == 2) {
 #pragma unroll
@@ -704,11 +708,28 @@ __device__ __forceinline__ void mac_body(const MacArgs& a, int cidx, int range, 
                     const float2 give = lsub ? acc[f][0][j] : acc[f][1][j];
                     own[f][j] = make_float2(mine.x + __shfl_xor(give.x, 1), mine.y + __shfl_xor(give.y, 1));
                 }
+        } else if constexpr (IPT == 4) {
+            const bool hi = (lsub & 2) != 0, odd = (lsub & 1) != 0;
+#pragma unroll
+            for (int f = 0; f < FB; ++f)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    float2 k[2];
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        const float2 keep = hi ? acc[f][2 + c][j] : acc[f][c][j];
+                        const float2 give = hi ? acc[f][c][j] : acc[f][2 + c][j];
+                        k[c] = make_float2(keep.x + __shfl_xor(give.x, 2), keep.y + __shfl_xor(give.y, 2));
+                    }
+                    const float2 mine = odd ? k[1] : k[0];
+                    const float2 give = odd ? k[0] : k[1];
+                    own[f][j] = make_float2(mine.x + __shfl_xor(give.x, 1), mine.y + __shfl_xor(give.y, 1));
+                }
         }
         const auto ip = items_now();
 #pragma unroll
         for (int q = 0; q < QT; ++q) {
-            const ItemDesc itq = ip[IPT == 2 ? lsub : q];
+            const ItemDesc itq = ip[IPT >= 2 ? lsub : q];
             const int m2 = itq.m2[0], n2 = itq.n2[0];
             int outer[4];
             outer[0] = sig * (m2 + n2);
@@ -730,7 +751,7 @@ __device__ __forceinline__ void mac_body(const MacArgs& a, int cidx, int range, 
                 const float2 pc = make_float2(cs, sn);
 #pragma unroll
                 for (int f = 0; f < FB; ++f) {
-                    const float2 av = IPT == 2 ? own[f][j] : acc[f][q][j];
+                    const float2 av = IPT >= 2 ? own[f][j] : acc[f][q][j];
                     tot[f][q][j] = cadd(tot[f][q][j], cmul(cmul(av, po), pc));
                 }
             }
@@ -1524,7 +1545,7 @@ static void launch_mac_fb(const MacArgs& a, int fb, dim3 grid, dim3 block, size_
 }
 hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks, int nplanes_grid, bool mfma,
                       hipStream_t s) {
-    if (fb < 1 || fb > 4 || (ipt != 1 && ipt != 2) || a.T < 1 || a.rs != mac_row_stride(slots, a.nu))
+    if (fb < 1 || fb > 4 || (ipt != 1 && ipt != 2 && ipt != 4) || a.T < 1 || a.rs != mac_row_stride(slots, a.nu))
         return hipErrorInvalidValue;
     const size_t lds = mac_lds(slots, a.nu, a.N, fb);
     dim3 grid(nchunks * ((a.T + fb - 1) / fb), nplanes_grid);
@@ -1536,13 +1557,15 @@ hipError_t launch_mac(const MacArgs& a, int slots, int ipt, int fb, int nchunks,
         else hipLaunchKernelGGL((k_mac_tc<2>), tgrid, dim3(64 * TC_WAVES), tlds, s, a);
         return hipGetLastError();
     }
-    dim3 block(FLF_CHUNK_MAX);                   // ipt 2: lane pairs share two items
+    dim3 block(FLF_CHUNK_MAX);                   // ipt 2 / 4: lane groups share ipt items
     if (slots == 4) {
         if (ipt == 1) launch_mac_fb<4, 1>(a, fb, grid, block, lds, s);
-        else launch_mac_fb<4, 2>(a, fb, grid, block, lds, s);
+        else if (ipt == 2) launch_mac_fb<4, 2>(a, fb, grid, block, lds, s);
+        else launch_mac_fb<4, 4>(a, fb, grid, block, lds, s);
     } else {
         if (ipt == 1) launch_mac_fb<2, 1>(a, fb, grid, block, lds, s);
-        else launch_mac_fb<2, 2>(a, fb, grid, block, lds, s);
+        else if (ipt == 2) launch_mac_fb<2, 2>(a, fb, grid, block, lds, s);
+        else launch_mac_fb<2, 4>(a, fb, grid, block, lds, s);
     }
     return hipGetLastError();
 }
@@ -1555,7 +1578,7 @@ static void launch_mac_grouped_fb(const MacGrouped& ga, int fb, dim3 grid, dim3 
 }
 hipError_t launch_mac_grouped(const MacGrouped& ga, int slots, int ipt, int fb, int nfb, size_t lds, hipStream_t s) {
     const GroupedGrid& gg = ga.gg;
-    if (fb < 1 || fb > 4 || (ipt != 1 && ipt != 2) || (slots != 2 && slots != 4) || nfb < 1 || gg.ngroups < 1 ||
+    if (fb < 1 || fb > 4 || (ipt != 1 && ipt != 2 && ipt != 4) || (slots != 2 && slots != 4) || nfb < 1 || gg.ngroups < 1 ||
         gg.ngroups > FLF_MAX_GROUPS)
         return hipErrorInvalidValue;
     if (ga.nfb != nfb) return hipErrorInvalidValue;
@@ -1565,10 +1588,12 @@ hipError_t launch_mac_grouped(const MacGrouped& ga, int slots, int ipt, int fb, 
     dim3 block(FLF_CHUNK_MAX);
     if (slots == 4) {
         if (ipt == 1) launch_mac_grouped_fb<4, 1>(ga, fb, grid, block, lds, s);
-        else launch_mac_grouped_fb<4, 2>(ga, fb, grid, block, lds, s);
+        else if (ipt == 2) launch_mac_grouped_fb<4, 2>(ga, fb, grid, block, lds, s);
+        else launch_mac_grouped_fb<4, 4>(ga, fb, grid, block, lds, s);
     } else {
         if (ipt == 1) launch_mac_grouped_fb<2, 1>(ga, fb, grid, block, lds, s);
-        else launch_mac_grouped_fb<2, 2>(ga, fb, grid, block, lds, s);
+        else if (ipt == 2) launch_mac_grouped_fb<2, 2>(ga, fb, grid, block, lds, s);
+        else launch_mac_grouped_fb<2, 4>(ga, fb, grid, block, lds, s);
     }
     return hipGetLastError();
 }

@@ -19,7 +19,7 @@ import torch
 
 PLAN_CANDIDATES = {
     "radix_order": [0, 1, 2, 3],
-    "mac_ipt": [2, 1],
+    "mac_ipt": [2, 4, 1],
 }
 
 CANDIDATES = {

@@ -120,7 +120,7 @@ int flf_set_frame_dims(flf_ctx* ctx, const int* dims, int T, void* stream);
  *   "radix_order"    0..3 pass order of every FFT: 0 largest radices first,
  *                    1 reversed, 2 no radix 8 (wave-level transforms), 3 odd
  *                    radices first (wave-level); a prime pass stays last
- *   "mac_ipt"        1 or 2 items per MAC thread (2: lane pairs share two)
+ *   "mac_ipt"        1, 2 or 4 items per MAC lane group (2 / 4: lane pairs / quads share them)
  * Results do not depend on them beyond FP32 rounding (group_streams: not at
  * all).  flfhip.autotune picks them by timing. */
 int flf_set_option(flf_ctx* ctx, const char* name, int value);

@@ -254,6 +254,50 @@ def test_one_item_per_thread_matches_lane_pairs(projector):
     del p1
 
 
+@pytest.mark.parametrize("grouped", [1, 0])
+@pytest.mark.parametrize("N,Y,X,sides", [(15, 150, 165, [15, 45, 61]), (13, 130, 143, [27, 53])])
+def test_lane_quads_match_lane_pairs(grouped, N, Y, X, sides):
+    """Four items per lane group (mac_ipt 4: every P/M table read serves four
+    items and the quad adds its partial sums in two shuffle steps) and the
+    lane pairs give the same projections for one to four frames per
+    workgroup, in grouped and per-group launches; the one-frame case also
+    against the FP64 oracle."""
+    import torch
+    from flfhip import synthetic
+    from flfhip.projector import Projector_hip
+    zs = list(range(len(sides)))
+    hm = synthetic.make_psf_set(N, zs, sides, seed=N + 40)
+    p2, p4 = Projector_hip(), Projector_hip()
+    p2.set_option("mac_ipt", 2)
+    p4.set_option("mac_ipt", 4)
+    T = 3
+    for p in (p2, p4):
+        p.prepare(hm, T, Y, X)
+        p.set_option("grouped_launch", grouped)
+    assert p4.get_option("mac_ipt") == 4
+    g = torch.Generator(device="cuda")
+    g.manual_seed(N + grouped)
+    vol = torch.rand((len(zs), T, Y, X), generator=g, device="cuda")
+    img = torch.rand((T, Y, X), generator=g, device="cuda")
+    ref_f, ref_b = p2.forward_device(hm, vol).clone(), p2.backward_device(hm, img).clone()
+    for fb in (1, 2, 3, 4):
+        p4.set_option("mac_frames", fb)
+        got_f, got_b = p4.forward_device(hm, vol), p4.backward_device(hm, img)
+        torch.cuda.synchronize()
+        ef = float((got_f - ref_f).norm() / ref_f.norm())
+        eb = float((got_b - ref_b).norm() / ref_b.norm())
+        print("grouped %d N %d fb %d: lane quads vs pairs forward %.2e backward %.2e" % (grouped, N, fb, ef, eb),
+              flush=True)
+        assert ef < 1e-6 and eb < 1e-6
+    x1 = vol[:, :1].cpu().numpy()
+    got1 = p4.ForwardProjectACC(hm, x1, zs)
+    ref1 = sum(oproj.project_special(x1[z], hm.Hcc(z, False)) for z in range(len(zs)))
+    e1 = rel_l2(got1, ref1)
+    print("lane quads one frame vs oracle: %.2e" % e1, flush=True)
+    assert e1 < 1e-5
+    del p2, p4
+
+
 def test_forward_pair_equals_concatenated_frames(projector):
     """flf_forward_pair (frames from two buffers) gives the same image as
     flf_forward on the two volumes concatenated along frames, bit for bit;
@@ -416,8 +460,8 @@ def test_prime_pitch_small_direct(projector, N, Y, X, s2):
 @pytest.mark.parametrize("order", [0, 1, 2, 3])
 @pytest.mark.parametrize("N,Y,X,s2s", [(15, 90, 150, [31, 61]), (13, 65, 78, [27, 53])])
 def test_plan_options_radix_order_and_ipt(projector, order, N, Y, X, s2s):
-    """Every FFT pass order (radix_order 0..3) with one and two items per
-    MAC thread (mac_ipt), on grids of lengths 120..225 (radices 8, 6, 5, 3,
+    """Every FFT pass order (radix_order 0..3) with one, two and four items
+    per MAC lane group (mac_ipt), on grids of lengths 120..225 (radices 8, 6, 5, 3,
     2) and of Nnum 13 (prime pass last): forward and backward projection of
     two planes against the FP64 oracle.  Own projectors: the plan options
     stay off the session's projector."""
@@ -426,7 +470,7 @@ def test_plan_options_radix_order_and_ipt(projector, order, N, Y, X, s2s):
     rng = np.random.default_rng(order)
     x = rng.random((2, 1, Y, X)).astype(np.float32)
     ref = sum(oproj.project_special(x[z], hm.Hcc(z, False)) for z in range(2))
-    for ipt in (1, 2):
+    for ipt in (1, 2, 4):
         p = Projector_hip()
         p.set_option("radix_order", order)
         p.set_option("mac_ipt", ipt)
