@@ -80,6 +80,7 @@ struct flf_ctx {
     int fb_max = 2;                   // frames per MAC workgroup at T >= 2 (FLF_MAC_FRAMES overrides)
     int fwd_split = 0;                // forward plane ranges, 0 = automatic (FLF_FWD_SPLIT overrides)
     int bwd_planes = 1;               // planes per backward MAC workgroup
+    int bwd_fgroup = 0;               // backward MAC frame blocks dispatched side by side (0: all)
     bool mfma = false;                // matrix-core MAC (k_mac_tc, 8 frames per workgroup) for calls of >= 2
                                       // frames (option mac_mfma, FLF_MAC_MFMA=1; chosen by the autotuner)
     // Fourier-grid groups of one projection side by side on streams of
@@ -574,6 +575,7 @@ int run_backward_groups(flf_ctx* ctx, const float* img, const std::vector<char>&
         // Each workgroup takes bwd_planes consecutive planes (loads run on
         // across the plane boundary) and stores every plane's accumulator.
         m.per_plane = 1;
+        m.fgroup = ctx->bwd_fgroup;
         const int bp = std::max(1, ctx->bwd_planes);
         w.nranges = (m.nP + bp - 1) / bp;
         IfftColArgs& ic = w.ca;
@@ -936,6 +938,9 @@ int flf_set_option(flf_ctx* ctx, const char* name, int value) {
     } else if (n == "bwd_planes") {
         if (value < 1 || value > 8) return fail(ctx, FLF_EINVAL, "bwd_planes must be 1..8");
         ctx->bwd_planes = value;
+    } else if (n == "bwd_frame_group") {
+        if (value < 0 || value > 64) return fail(ctx, FLF_EINVAL, "bwd_frame_group must be 0..64");
+        ctx->bwd_fgroup = value;
     } else if (n == "group_streams") {
         ctx->group_streams = value != 0;
     } else if (n == "grouped_launch") {
@@ -966,6 +971,7 @@ int flf_get_option(const flf_ctx* ctx, const char* name, int* value) {
     else if (n == "mac_frames") *value = ctx->fb_max;
     else if (n == "mac_mfma") *value = ctx->mfma ? 1 : 0;
     else if (n == "bwd_planes") *value = ctx->bwd_planes;
+    else if (n == "bwd_frame_group") *value = ctx->bwd_fgroup;
     else if (n == "group_streams") *value = ctx->group_streams ? 1 : 0;
     else if (n == "grouped_launch") *value = ctx->grouped ? 1 : 0;
     else if (n == "radix_order") *value = ctx->radix_order;

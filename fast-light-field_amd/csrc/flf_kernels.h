@@ -94,6 +94,7 @@ struct MacArgs {
     int per_plane;               // 1: store each plane's accumulator at A + p * A_pstride (back
                                  // projection); 0: sum the planes of a range, store at
                                  // A + range * A_pstride (forward projection)
+    int fgroup;                  // frame blocks dispatched side by side per chunk (0: all)
 };
 
 struct IfftColArgs {

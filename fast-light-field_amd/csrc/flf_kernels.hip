@@ -814,15 +814,30 @@ __device__ __forceinline__ void mac_body(const MacArgs& a, int cidx, int range, 
     if (valid && !a.per_plane) store(range);
 }
 
+// Block id w in [0, R * nfb) -> frame block and the rest r in [0, R):
+// frame blocks in super-blocks of S (the last one possibly shorter), and
+// within a super-block the frame block fastest.  S = nfb: all frame blocks of
+// an r consecutive; S = 1: frame block slowest.
+__device__ __forceinline__ void split_frames(int w, int R, int nfb, int S, int& fblock, int& r) {
+    S = (S <= 0 || S > nfb) ? nfb : S;
+    const int sb = w / (S * R);
+    const int sc = min(S, nfb - sb * S);
+    const int rem = w - sb * S * R;
+    fblock = sb * S + rem % sc;
+    r = rem / sc;
+}
+
 template <int SLOTS, int IPT, int FB>
 __global__ __launch_bounds__(FLF_CHUNK_MAX) __attribute__((amdgpu_waves_per_eu(IPT * FB >= 3 ? 2 : (IPT * FB == 2 ? 3 : FLF_MAC_WAVES))))
 void k_mac(MacArgs a) {
-    // grid x = chunk x frame block, frame block fastest: after the XCD remap
-    // the frame blocks of a chunk share one XCD's L2 for the spectra they
-    // all read
+    // grid x = chunk x frame block, frame blocks of a chunk consecutive (in
+    // super-blocks of a.fgroup): after the XCD remap they share one XCD's L2
+    // for the spectra they all read
     const int nfb = (a.T + FB - 1) / FB;
     const int w = xcd_block(blockIdx.x, gridDim.x);
-    mac_body<SLOTS, IPT, FB>(a, w / nfb, blockIdx.y, gridDim.y, w % nfb);
+    int fblock, cidx;
+    split_frames(w, gridDim.x / nfb, nfb, a.fgroup, fblock, cidx);
+    mac_body<SLOTS, IPT, FB>(a, cidx, blockIdx.y, gridDim.y, fblock);
 }
 
 // All active grid groups of one projection: grid x flattened over (group,
@@ -834,14 +849,18 @@ void k_mac(MacArgs a) {
 // early.  After the remap the frame blocks of one (chunk, plane range) are
 // consecutive ids of one XCD, dispatched together: they read the same
 // stored spectra at about the same time, which the XCD's L2 then serves
-// once from HBM for all of them.
+// once from HBM for all of them.  fgroup < nfb dispatches them in
+// super-blocks of fgroup instead, so that more chunks run side by side
+// (their scattered accumulator stores then combine in L2 more often).
 template <int SLOTS, int IPT, int FB>
 __global__ __launch_bounds__(FLF_CHUNK_MAX) __attribute__((amdgpu_waves_per_eu(IPT * FB >= 3 ? 2 : (IPT * FB == 2 ? 3 : FLF_MAC_WAVES))))
 void k_mac_grouped(MacGrouped ga) {
     const GroupPick p = pick_group(ga.gg, blockIdx.x);
     const int w = xcd_block(p.rem, p.n);
-    const int fblock = w % ga.nfb, r = w / ga.nfb;
-    if (r >= p.nch * p.nr) return;                // padding
+    const int R = p.nch * p.nr;
+    if (w >= R * ga.nfb) return;                  // padding
+    int fblock, r;
+    split_frames(w, R, ga.nfb, ga.a[p.g].fgroup, fblock, r);
     mac_body<SLOTS, IPT, FB>(ga.a[p.g], r % p.nch, r / p.nch, p.nr, fblock);
 }
 

@@ -106,6 +106,8 @@ int flf_set_frame_dims(flf_ctx* ctx, const int* dims, int T, void* stream);
  *   "mac_mfma"       0/1: matrix-core MAC (FP32 MFMA, 8 frames per
  *                    workgroup) for calls of >= 2 frames
  *   "bwd_planes"     1..8 planes per back-projection MAC workgroup
+ *   "bwd_frame_group" 0..64 frame blocks of a chunk the back-projection MAC
+ *                    dispatches side by side (0: all; bit-identical results)
  *   "ifft_col_batch" 1..8 columns per column-transform workgroup (after plan)
  *   "grouped_launch" 0/1 (default 1): the small FFTs, the multiply-accumulate
  *                    and the column transforms of all Fourier-grid groups of

@@ -255,6 +255,37 @@ def test_one_item_per_thread_matches_lane_pairs(projector):
 
 
 @pytest.mark.parametrize("grouped", [1, 0])
+def test_bwd_frame_group_bit_identical(grouped):
+    """The back projection with a chunk's frame blocks dispatched in
+    super-blocks of 1, 2, 3, 5 or 64 (option bwd_frame_group; 5 frame blocks
+    of 2 frames here, so 2 and 3 leave a shorter last super-block, 1 is the
+    frame-block-slowest order and 64 clamps to all) equals the default order
+    bit for bit: only the dispatch order changes."""
+    import torch
+    from flfhip.projector import Projector_hip
+    hm = synthetic.make_psf_set(15, [0, 1, 2], [15, 45, 61], seed=71)
+    T, Y, X = 9, 150, 165
+    p = Projector_hip()
+    p.prepare(hm, T, Y, X)
+    p.set_option("grouped_launch", grouped)
+    p.set_option("mac_frames", 2)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(71 + grouped)
+    img = torch.rand((T, Y, X), generator=g, device="cuda")
+    ref = p.backward_device(hm, img).clone()
+    for fg in (1, 2, 3, 5, 64):
+        p.set_option("bwd_frame_group", fg)
+        assert p.get_option("bwd_frame_group") == fg
+        got = p.backward_device(hm, img)
+        torch.cuda.synchronize()
+        print("grouped %d bwd_frame_group %d: identical %s" % (grouped, fg, bool(torch.equal(got, ref))), flush=True)
+        assert torch.equal(got, ref)
+    with pytest.raises(ValueError):
+        p.set_option("bwd_frame_group", 65)
+    del p
+
+
+@pytest.mark.parametrize("grouped", [1, 0])
 @pytest.mark.parametrize("N,Y,X,sides", [(15, 150, 165, [15, 45, 61]), (13, 130, 143, [27, 53])])
 def test_lane_quads_match_lane_pairs(grouped, N, Y, X, sides):
     """Four items per lane group (mac_ipt 4: every P/M table read serves four
