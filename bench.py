@@ -510,9 +510,9 @@ def batch_main(args):
                      "frac": flop_rate / PEAK_FP32, "traffic": measured_traffic(args.workload), "kernel": "k_mac",
                      "launch_avg_ms": mac_avg_ms, "launches": mac_n,
                      "flops_per_launch": mac_flops / max(mac_n, 1),
-                     "note": "several frames per workgroup: the MAC is bound on the CU, not by HBM: P/M table "
-                             "reads from LDS at 112 B per clock and CU (88% of 128 B/clk), VALU 57% busy, two to "
-                             "three waves per SIMD (profiles/k_mac_rl1455x32_sq_counters.txt)",
+                     "note": "several frames per workgroup: the MAC is bound on the CU, not by HBM: VALU 57% "
+                             "busy, one 16-byte LDS table read per 8 FMAs, two to three waves per SIMD "
+                             "(profiles/k_mac_rl1455x32_sq_counters.txt)",
                      "hbm": {"achieved": hbm_rate / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                              "frac": hbm_rate / PEAK_HBM, "bytes_per_launch": mac_bytes / max(mac_n, 1)}},
         "rl_iteration": {"ms": it1_s * 1e3, "frames": 1, "algorithmic_bytes": it1_bytes,
