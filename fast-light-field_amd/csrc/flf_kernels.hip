@@ -1470,17 +1470,35 @@ __global__ __launch_bounds__(256) void k_fit_final(const double* part, int nblk,
 // Used for the first plain iterate from a uniform start, X1 = Htf / Ht(H 1),
 // whose denominator does not depend on the image (one slab per plane serves
 // every frame: v_frames = 1).  ratio: X = Htf / v (the same update of X = 1,
-// without reading X).  grid (x blocks, nz * T slabs).
-__global__ void k_rl_scale(float* X, const float* Htf, const float* v, int T, long long npix, int v_frames,
-                           int ratio) {
+// without reading X).  grid (x blocks, nz * T slabs), block RL_SCALE_THREADS;
+// each thread takes RL_SCALE_E pixels RL_SCALE_THREADS apart, all loads in
+// flight before the first use (indices clamped into the slab).
+constexpr int RL_SCALE_THREADS = 256, RL_SCALE_E = 16;
+__global__ __launch_bounds__(RL_SCALE_THREADS) void k_rl_scale(float* X, const float* Htf, const float* v, int T,
+                                                               long long npix, int v_frames, int ratio) {
     const int slab = blockIdx.y;
     const int z = slab / T, t = slab - z * T;
     const long long o = (long long)slab * npix;
     const float* vs = v + ((long long)z * v_frames + (v_frames == 1 ? 0 : t)) * npix;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += (long long)gridDim.x * blockDim.x) {
-        const float r = Htf[o + i] / vs[i];
-        const float xn = ratio ? r : X[o + i] * r;
-        X[o + i] = __builtin_isnan(xn) ? 0.f : xn;
+    constexpr long long CH = (long long)RL_SCALE_THREADS * RL_SCALE_E;
+    for (long long b0 = (long long)blockIdx.x * CH + threadIdx.x; b0 < npix; b0 += (long long)gridDim.x * CH) {
+        float h[RL_SCALE_E], d[RL_SCALE_E], x[RL_SCALE_E];
+#pragma unroll
+        for (int e = 0; e < RL_SCALE_E; ++e) {
+            const long long i = min(b0 + (long long)e * RL_SCALE_THREADS, npix - 1);
+            h[e] = Htf[o + i];
+            d[e] = vs[i];
+            x[e] = ratio ? 1.f : X[o + i];
+        }
+#pragma unroll
+        for (int e = 0; e < RL_SCALE_E; ++e) {
+            const long long i = b0 + (long long)e * RL_SCALE_THREADS;
+            if (i < npix) {
+                const float r = h[e] / d[e];
+                const float xn = ratio ? r : x[e] * r;
+                X[o + i] = __builtin_isnan(xn) ? 0.f : xn;
+            }
+        }
     }
 }
 
@@ -1719,12 +1737,13 @@ hipError_t configure_kernels(size_t max_lds) {
 
 hipError_t launch_rl_scale(float* X, const float* Htf, const float* v, int nz, int T, long long npix, int v_frames,
                            hipStream_t s, bool ratio) {
-    long long blocks = (npix + 255) / 256;
+    const long long ch = (long long)RL_SCALE_THREADS * RL_SCALE_E;
+    long long blocks = (npix + ch - 1) / ch;
     if (blocks > 65535) blocks = 65535;
     if (blocks < 1) blocks = 1;
     // slabs on y (nz * T <= 65535 is checked by the caller), pixels on x
-    hipLaunchKernelGGL(k_rl_scale, dim3((unsigned)blocks, (unsigned)(nz * T)), dim3(256), 0, s, X, Htf, v, T, npix,
-                       v_frames, ratio ? 1 : 0);
+    hipLaunchKernelGGL(k_rl_scale, dim3((unsigned)blocks, (unsigned)(nz * T)), dim3(RL_SCALE_THREADS), 0, s, X, Htf,
+                       v, T, npix, v_frames, ratio ? 1 : 0);
     return hipGetLastError();
 }
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t s) {
